@@ -1,0 +1,239 @@
+/*
+ * crystal2.h -- C-ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * Drop-in boundary for Crystal2's PathTracer + MIS hot path. Every entry point
+ * is `extern "C"`, takes plain pointers and sizes, returns an int status
+ * (C2_OK = 0, negative on error) and never lets a C++ exception cross the
+ * boundary; c2_last_error() returns a thread-local message for the last
+ * failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the Crystal2 repository):
+ *   Scene/config loaders   src/Core/Scene.cpp:23-199, src/Loaders/{JsonLoader,ConfigLoader,ObjLoader}.cpp,
+ *                          src/Cameras/Camera.cpp:7-20, src/Integrators/Integrator.cpp:14-47
+ *   Accelerator            src/Accelerators/Accelerator.h:8-22  (Build / Intersect / IntersectTest)
+ *   Integrator             src/Integrators/Integrator.h:8-18     (Render(Scene*, Camera*, FrameBuffer*)),
+ *                          src/Integrators/SamplerIntegrator.cpp:17-72,
+ *                          src/Integrators/PathTracingIntegrator.cpp:28-278
+ *   Sampler                src/Samplers/HaltonSampler.cpp:118-291 (index + value per dimension)
+ *   FrameBuffer            src/Core/FrameBuffer.cpp:20-75         (AddSample / GetImageDataRGB32F)
+ *
+ * Ownership: descriptors passed in are read during the call only; handles own
+ * their device memory. A handle is not internally synchronised: use one per
+ * host thread (the reference calls Intersect concurrently from its tile pool,
+ * src/Integrators/SamplerIntegrator.cpp:67; here one handle per thread plays
+ * that role).
+ */
+#ifndef CRYSTAL2_H
+#define CRYSTAL2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+enum {
+    C2_OK = 0,
+    C2_ERR_INVALID = -1,     /* bad argument / inconsistent descriptor            */
+    C2_ERR_PARSE = -2,       /* JSON/OBJ syntax or schema error (SJson semantics)  */
+    C2_ERR_IO = -3,          /* file could not be opened                          */
+    C2_ERR_DEVICE = -4,      /* HIP runtime error (message names the call)        */
+    C2_ERR_NOMEM = -5,       /* host or device allocation failed                  */
+    C2_ERR_UNSUPPORTED = -6  /* valid reference input this build does not handle  */
+};
+
+/* Thread-local message describing the last error on this thread ("" if none). */
+const char* c2_last_error(void);
+
+/* ---------------------------------------------------------------- scene description
+ * A flattened, already-assembled scene: what Scene::CreateScene builds
+ * (src/Core/Scene.cpp:94-199). Entity order is JSON object order; the entity
+ * index is the primitive ID everywhere (hits, ignore lists, light attachment).
+ */
+enum { C2_SHAPE_TRIANGLE = 0, C2_SHAPE_SPHERE = 1 };
+
+enum {
+    C2_MAT_DIFFUSE = 0,  /* Lambertain                     src/Materials/DiffuseMaterial.cpp:16-20 */
+    C2_MAT_MIRROR = 1,   /* SpecularReflection + NoOp       src/Materials/MirrorMaterial.cpp:11-13  */
+    C2_MAT_GLASS = 2,    /* SpecularFresnel | MicrofacetBlend src/Materials/Glass.cpp:16-34         */
+    C2_MAT_MICRO = 3,    /* MicrofacetReflection + Schlick  src/Materials/MicrofacetMaterial.cpp:18-42 */
+    C2_MAT_PHONG = 4     /* BlinnPhongReflection            src/Materials/PhongMaterial.cpp:13-22    */
+};
+enum { C2_NDF_GGX = 0, C2_NDF_BECKMANN = 1 };
+enum { C2_LIGHT_AREA = 0, C2_LIGHT_POINT = 1 };
+
+typedef struct c2_material {
+    int32_t type;          /* C2_MAT_*                                      */
+    int32_t ndf;           /* C2_NDF_* (Micro only; Beckmann unsupported)   */
+    int32_t phong_k;       /* Blinn-Phong exponent K                        */
+    int32_t pad_;
+    float color[3];        /* albedo / R (constant texture)                 */
+    float roughness[2];    /* GGX alpha_x, alpha_y (Glass: 0,0 = specular)  */
+    float ior;             /* IOR (Glass, Micro)                            */
+} c2_material;
+
+typedef struct c2_light {
+    int32_t type;          /* C2_LIGHT_*                                         */
+    int32_t entity;        /* AREA: entity whose shape emits (one light per tri) */
+    float radiance[3];     /* AREA: Lemit; POINT: Flux                          */
+    float position[3];     /* POINT only                                        */
+} c2_light;
+
+typedef struct c2_camera {
+    float position[3];
+    float look_at[3];
+    float up[3];
+    float fov;             /* radians, FOV/180*pi as Camera::CreateCamera computes it */
+} c2_camera;
+
+typedef struct c2_scene_desc {
+    int32_t num_entities;
+    const int32_t* entity_shape_kind;   /* C2_SHAPE_*                                */
+    const int32_t* entity_shape_index;  /* index into triangles[] or spheres[]       */
+    const int32_t* entity_material;     /* index into materials[], -1 = transparent  */
+    const int32_t* entity_light;        /* index into lights[] (attached area light), -1 = none */
+
+    int32_t num_triangles;
+    const float* triangles;   /* [num_triangles][3 vertices][8]: pos.xyz, normal.xyz, uv.xy,
+                                 world space, normals already transformed + normalised
+                                 (src/Shapes/TriangleMesh.cpp:29-51)                     */
+    int32_t num_spheres;
+    const float* spheres;     /* [num_spheres][22]: center.xyz, radius, local2world[9],
+                                 world2local[9] (glm column-major, src/Shapes/Sphere.cpp:19-30) */
+    int32_t num_materials;
+    const c2_material* materials;
+    int32_t num_lights;
+    const c2_light* lights;   /* the reference's light order (Scene::_lights)       */
+    c2_camera camera;
+} c2_scene_desc;
+
+/* ---------------------------------------------------------------- host scene builder
+ * Owns the arrays a c2_scene_desc points into. c2_scene_load reads the
+ * reference's scene JSON (+ OBJ files, relative to base_dir or the CWD when
+ * base_dir is NULL) with the reference's number parsing and object order.
+ */
+typedef struct c2_scene c2_scene;
+
+typedef struct c2_config {
+    int32_t width, height, spp, threads;
+    int32_t max_depth;
+    char accel[16];        /* "Acceleration": "BVH" | "Brute" | "KDTree" | "GPUBVH" */
+    char integrator[32];   /* "PathTracer" | "WavefrontPathTracer" | ...           */
+    char sampler[16];      /* "Halton" (the only sampler on this path)             */
+} c2_config;
+
+int  c2_config_load(const char* config_json_path, c2_config* out);
+int  c2_scene_load(const char* scene_json_path, const char* base_dir, c2_scene** out);
+int  c2_scene_create(c2_scene** out);
+void c2_scene_destroy(c2_scene* scene);
+/* returns material index in *index */
+int  c2_scene_add_material(c2_scene* scene, const c2_material* material, int32_t* index);
+/* Triangles with unshared per-corner vertex data [ntri][3][8] (pos, normal, uv),
+ * transformed by the optional glm-style column-major 4x4 `transform` exactly as
+ * TriangleMesh does (NULL = identity). material -1 = transparent; lemit != NULL
+ * attaches one Diffused area light per triangle. */
+int  c2_scene_add_triangles(c2_scene* scene, const float* vertex_data, int32_t ntri,
+                            const float* transform, int32_t material, const float* lemit);
+/* Sphere at center with radius and XYZ rotation in degrees (src/Shapes/Sphere.cpp:19-30). */
+int  c2_scene_add_sphere(c2_scene* scene, const float center[3], float radius,
+                         const float rotation_deg[3], int32_t material, const float* lemit);
+int  c2_scene_add_point_light(c2_scene* scene, const float position[3], const float flux[3]);
+int  c2_scene_set_camera(c2_scene* scene, const c2_camera* camera);
+/* Fills *out with pointers into the scene's arrays (valid until the scene changes). */
+int  c2_scene_get_desc(const c2_scene* scene, c2_scene_desc* out);
+
+/* ---------------------------------------------------------------- Accelerator (GPU BVH)
+ * Build: SAH binary BVH (leaf <= 4) collapsed to a 4-wide BVH with 128-byte,
+ * cache-line aligned nodes in HBM. Intersect / IntersectTest keep the
+ * reference's semantics (src/Accelerators/BVH.cpp:56-154): per-primitive box
+ * pre-clip, closest hit by strict t (ties: lowest entity index), any-hit over
+ * the box-clipped [tmin, tmax], `ignore` entity skipped.
+ *
+ * Rays are [n][8] float32: o.xyz, d.xyz, tmin, tmax. Closest hit ignores tmin
+ * and clips with tmax (the reference passes tMin=0, tMax=inf).
+ * `*_device` variants take device pointers and a hipStream_t (NULL = default
+ * stream) and return without synchronising; host variants copy in and out.
+ */
+typedef struct c2_accel c2_accel;
+
+typedef struct c2_accel_stats {
+    int64_t num_nodes_bvh2;       /* binary SAH BVH nodes                       */
+    int64_t num_nodes_bvh4;       /* 4-wide nodes after collapse                */
+    int32_t max_depth_bvh4;
+    int32_t leaf_size;
+    double  build_seconds;
+    int64_t device_bytes;         /* node + primitive bytes resident in HBM     */
+} c2_accel_stats;
+
+int  c2_accel_build(const c2_scene_desc* scene, c2_accel** out);
+void c2_accel_destroy(c2_accel* accel);
+int  c2_accel_get_stats(const c2_accel* accel, c2_accel_stats* out);
+int  c2_accel_intersect(c2_accel* accel, const float* rays, int64_t n,
+                        int32_t* prim, float* t);
+int  c2_accel_intersect_test(c2_accel* accel, const float* rays, const int32_t* ignore,
+                             int64_t n, uint8_t* hit);
+int  c2_accel_intersect_device(c2_accel* accel, const float* rays, int64_t n,
+                               int32_t* prim, float* t, void* stream);
+int  c2_accel_intersect_test_device(c2_accel* accel, const float* rays, const int32_t* ignore,
+                                    int64_t n, uint8_t* hit, void* stream);
+/* Algorithmic traversal counts on the host binary SAH BVH (32-byte nodes,
+ * leaf <= 4), the figure behind "algorithmic bytes per ray":
+ *   bytes/ray = 24 + 8 + nodes*32 + tris*36. any_hit selects IntersectTest. */
+int  c2_accel_count_visits(const c2_accel* accel, const float* rays, int64_t n, int32_t any_hit,
+                           double* mean_nodes, double* mean_tris);
+
+/* ---------------------------------------------------------------- Integrator (wavefront PT + MIS)
+ * One renderer per scene. c2_pt_render adds Halton samples
+ * [sample_begin, sample_begin + sample_count) of every pixel to the device
+ * film in sample order (bit-for-bit the reference's per-pixel sequential
+ * FrameBuffer::AddSample). One call with sample_begin=0, sample_count=spp is
+ * the reference's Render(); smaller calls are the bench's "steps".
+ */
+typedef struct c2_pt c2_pt;
+
+enum { C2_ARGORDER_RTL = 0, C2_ARGORDER_LTR = 1 };
+enum { C2_FILM_F32 = 0, C2_FILM_F32_KAHAN = 1, C2_FILM_F64 = 2 };
+
+typedef struct c2_render_opts {
+    int32_t width, height;
+    int32_t spp;              /* Halton sampler sample count (index stride)           */
+    int32_t max_depth;        /* PathTracer "Depth"                                    */
+    int32_t arg_order;        /* C2_ARGORDER_*: order of Get1D/Get2D at
+                                 PathTracingIntegrator.cpp:79 (RTL = GCC/MSVC)       */
+    int32_t film_mode;        /* C2_FILM_*                                             */
+    int32_t paths_in_flight;  /* 0 = all pixels of one sample pass at once             */
+    int32_t block_size;       /* 0 = default (256)                                     */
+} c2_render_opts;
+
+typedef struct c2_pt_stats {
+    int64_t samples;          /* camera paths evaluated                               */
+    int64_t extension_rays;   /* closest-hit path rays                                 */
+    int64_t shadow_rays;      /* any-hit NEE rays                                      */
+    int64_t mis_rays;         /* closest-hit BSDF-sampled MIS rays                     */
+    int64_t kernel_launches;
+} c2_pt_stats;
+
+int  c2_pt_create(const c2_scene_desc* scene, const c2_render_opts* opts, c2_pt** out);
+void c2_pt_destroy(c2_pt* pt);
+int  c2_pt_reset(c2_pt* pt);
+int  c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* stream);
+/* Film as FrameBuffer::GetImageDataRGB32F: W*H*3 float, rows top-down (host memory). */
+int  c2_pt_get_film(c2_pt* pt, float* rgb);
+int  c2_pt_get_stats(const c2_pt* pt, c2_pt_stats* out);
+/* Synchronise the renderer's stream; returns the first asynchronous error. */
+int  c2_pt_sync(c2_pt* pt);
+
+/* ---------------------------------------------------------------- Sampler (Halton)
+ * Reference HaltonSampler for a W x H film: vector index of (pixel, sample) and
+ * sample value per dimension (dims >= 100 use the documented deterministic
+ * substitute instead of the reference's shared mt19937). Host computation,
+ * identical to the device code's tables. */
+int  c2_halton_index(int32_t width, int32_t height, int32_t x, int32_t y, int32_t sample, int32_t* index);
+int  c2_halton_value(int32_t width, int32_t height, int32_t index, int32_t dim, float* value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRYSTAL2_H */

@@ -1,0 +1,175 @@
+// Headless driver for the reference renderer compiled from /root/reference.
+//
+// Test infrastructure only (oracle/_ref). It replaces the reference's GUI
+// entry point (Crystal2.cpp:8-37 + src/CrystalEngine.cpp) with a command line
+// that calls the reference's own loaders, accelerator, sampler and integrator
+// and dumps raw results, so the CPU restatement in oracle/ and the GPU path can
+// be checked against the reference itself:
+//
+//   crystal2_ref render <config.json> <scene.json> <out.f32>
+//       Scene::CreateScene + Camera::CreateCamera + Integrator::LoadIntegrator
+//       (src/Core/RayTracer.cpp:11-22), Integrator::Render timed with
+//       steady_clock, film dumped as FrameBuffer::GetImageDataRGB32F
+//       (src/Core/FrameBuffer.cpp:60-75): W*H*3 float32, rows top-down.
+//       Prints one JSON line {"render_s":..,"samples":..,"threads":..}.
+//   crystal2_ref hits <config.json> <scene.json> <rays.bin> <out.bin>
+//       rays.bin: int64 n, then n x 8 float32 (o.xyz, d.xyz, tmin, tmax).
+//       out.bin : n int32 entity index (-1 = miss) from Scene::Intersect
+//                 (src/Core/Scene.cpp:50-53), n float32 t, n float32 hit
+//                 position xyz x3 ... see write order below, and n uint8 from
+//                 Scene::IntersectTest(ray, tmin, tmax, nullptr).
+//   crystal2_ref halton <W> <H> <spp> <queries.bin> <ndims> <out.bin>
+//       queries.bin: int64 n, then n x 3 int32 (x, y, k). For each query:
+//       StartPixel((x,y)); idx = GetVectorIndexForSample(k)
+//       (src/Samplers/HaltonSampler.cpp:251-270); SampleValue(idx, d) for
+//       d < ndims (:272-291). out.bin: n int32 idx, then n x ndims float32.
+#include <Core/Scene.h>
+#include <Core/FrameBuffer.h>
+#include <Core/SurfaceInteraction.h>
+#include <Core/Entities.h>
+#include <Cameras/Camera.h>
+#include <Integrators/Integrator.h>
+#include <Loaders/JsonLoader.h>
+#include <Loaders/ConfigLoader.h>
+#include <Samplers/HaltonSampler.h>
+#include <SJson/SJson.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+struct Loaded {
+    std::shared_ptr<SJson::SJsonNode> cfgNode, sceneNode;
+    config::ConfigInfo info;
+    std::unique_ptr<Scene> scene;
+};
+
+Loaded load(const char* cfgPath, const char* scenePath) {
+    Loaded L;
+    L.cfgNode = loader::JsonLoader::LoadJsonFile(cfgPath);
+    L.sceneNode = loader::JsonLoader::LoadJsonFile(scenePath);
+    L.info = config::ConfigLoader::LoadConfigInfo(L.cfgNode);
+    L.scene = Scene::CreateScene(L.sceneNode->GetMember("Scene"), L.info);
+    return L;
+}
+
+int cmdRender(int argc, char** argv) {
+    if (argc < 5) return 2;
+    Loaded L = load(argv[2], argv[3]);
+    auto camera = Camera::CreateCamera(L.sceneNode->GetMember("Camera"), L.info);
+    auto integrator = crystal::Integrator::LoadIntegrator(L.cfgNode.get(), L.info);
+    FrameBuffer fb(L.info.Width, L.info.Height);
+    fb.Clear();
+    auto t0 = std::chrono::steady_clock::now();
+    integrator->Render(L.scene.get(), camera.get(), &fb);
+    auto t1 = std::chrono::steady_clock::now();
+    double secs = std::chrono::duration<double>(t1 - t0).count();
+    auto data = fb.GetImageDataRGB32F();
+    FILE* f = std::fopen(argv[4], "wb");
+    if (!f) return 3;
+    std::fwrite(data.get(), sizeof(float), (size_t)L.info.Width * L.info.Height * 3, f);
+    std::fclose(f);
+    long long samples = (long long)L.info.Width * L.info.Height * L.info.SamplesPerPixel;
+    std::printf("{\"render_s\": %.6f, \"samples\": %lld, \"threads\": %d, \"hw_threads\": %u}\n",
+                secs, samples, L.info.NumOfThreads, std::thread::hardware_concurrency());
+    return 0;
+}
+
+int cmdHits(int argc, char** argv) {
+    if (argc < 6) return 2;
+    Loaded L = load(argv[2], argv[3]);
+    auto objects = L.scene->GetObjects();
+    std::unordered_map<const crystal::Entity*, int> index;
+    for (size_t i = 0; i < objects.size(); i++) index[objects[i]] = (int)i;
+
+    FILE* f = std::fopen(argv[4], "rb");
+    if (!f) return 3;
+    long long n = 0;
+    if (std::fread(&n, sizeof(n), 1, f) != 1) return 3;
+    std::vector<float> rays((size_t)n * 8);
+    if (std::fread(rays.data(), sizeof(float), rays.size(), f) != rays.size()) return 3;
+    std::fclose(f);
+
+    std::vector<int> id((size_t)n, -1);
+    std::vector<float> t((size_t)n, 0.f), pos((size_t)n * 3, 0.f), nrm((size_t)n * 3, 0.f);
+    std::vector<unsigned char> any((size_t)n, 0);
+    auto t0 = std::chrono::steady_clock::now();
+    for (long long i = 0; i < n; i++) {
+        const float* r = &rays[(size_t)i * 8];
+        Ray3f ray(glm::vec3(r[0], r[1], r[2]), glm::vec3(r[3], r[4], r[5]));
+        SurfaceInteraction isec;
+        if (L.scene->Intersect(ray, &isec) && isec.GetHitEntity()) {
+            id[i] = index[isec.GetHitEntity()];
+            t[i] = isec.GetDistance();
+            glm::vec3 p = isec.GetHitPos(), nn = isec.GetNormal();
+            pos[i * 3] = p.x; pos[i * 3 + 1] = p.y; pos[i * 3 + 2] = p.z;
+            nrm[i * 3] = nn.x; nrm[i * 3 + 1] = nn.y; nrm[i * 3 + 2] = nn.z;
+        }
+        any[i] = L.scene->IntersectTest(ray, r[6], r[7], nullptr) ? 1 : 0;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    f = std::fopen(argv[5], "wb");
+    if (!f) return 3;
+    std::fwrite(id.data(), sizeof(int), id.size(), f);
+    std::fwrite(t.data(), sizeof(float), t.size(), f);
+    std::fwrite(pos.data(), sizeof(float), pos.size(), f);
+    std::fwrite(nrm.data(), sizeof(float), nrm.size(), f);
+    std::fwrite(any.data(), 1, any.size(), f);
+    std::fclose(f);
+    std::printf("{\"trace_s\": %.6f, \"rays\": %lld}\n",
+                std::chrono::duration<double>(t1 - t0).count(), n);
+    return 0;
+}
+
+int cmdHalton(int argc, char** argv) {
+    if (argc < 8) return 2;
+    int W = std::atoi(argv[2]), H = std::atoi(argv[3]), spp = std::atoi(argv[4]);
+    int ndims = std::atoi(argv[6]);
+    if (ndims > 100) return 4;  // dims >= 100 draw from a shared mt19937 (not reproducible)
+    crystal::HaltonSampler sampler(spp, Bound2i(Point2i(0, 0), Point2i(W, H)));
+    FILE* f = std::fopen(argv[5], "rb");
+    if (!f) return 3;
+    long long n = 0;
+    if (std::fread(&n, sizeof(n), 1, f) != 1) return 3;
+    std::vector<int> q((size_t)n * 3);
+    if (std::fread(q.data(), sizeof(int), q.size(), f) != q.size()) return 3;
+    std::fclose(f);
+    std::vector<int> idx((size_t)n);
+    std::vector<float> vals((size_t)n * ndims);
+    for (long long i = 0; i < n; i++) {
+        sampler.StartPixel(Point2i(q[i * 3], q[i * 3 + 1]));
+        idx[i] = sampler.GetVectorIndexForSample(q[i * 3 + 2]);
+        for (int d = 0; d < ndims; d++) vals[(size_t)i * ndims + d] = sampler.SampleValue(idx[i], d);
+    }
+    f = std::fopen(argv[7], "wb");
+    if (!f) return 3;
+    std::fwrite(idx.data(), sizeof(int), idx.size(), f);
+    std::fwrite(vals.data(), sizeof(float), vals.size(), f);
+    std::fclose(f);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: crystal2_ref render|hits|halton ...\n");
+        return 2;
+    }
+    std::string cmd = argv[1];
+    int rc = 2;
+    if (cmd == "render") rc = cmdRender(argc, argv);
+    else if (cmd == "hits") rc = cmdHits(argc, argv);
+    else if (cmd == "halton") rc = cmdHalton(argc, argv);
+    if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", cmd.c_str());
+    std::fflush(stdout);
+    // The reference's FixedThreadPool workers are detached and still blocked on
+    // their condition variable; leave without running static destructors.
+    std::_Exit(rc);
+}
