@@ -1,0 +1,281 @@
+"""crystal2_amd -- MI355X (gfx950) wavefront path tracer for Crystal2's PathTracer + MIS path.
+
+Python convenience layer over the C-ABI in ``include/crystal2.h`` (the product is
+the native library ``crystal2_amd/_lib/libcrystal2.so``; this module only moves
+arrays and handles). Names follow the reference's plugin interfaces:
+
+* :class:`Scene`       -- ``Scene::CreateScene`` from the reference JSON/OBJ formats,
+                          or built from arrays.
+* :class:`GpuBVH`      -- ``Accelerator`` (``Intersect`` / ``IntersectTest``) on the GPU.
+* :class:`WavefrontPathIntegrator` -- ``Integrator::Render`` on the GPU.
+
+Every call fails loudly (``Crystal2Error``) when the native library is missing
+or the device reports an error; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _abi
+from ._abi import Crystal2Error, check
+
+__all__ = ["Scene", "GpuBVH", "WavefrontPathIntegrator", "Config", "Crystal2Error", "halton_index",
+           "halton_value", "load_library"]
+
+
+def load_library():
+    return _abi.load()
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+@dataclass
+class Config:
+    """Render configuration, the reference's config JSON (ConfigLoader.cpp:12-16, Integrator.cpp:19-35)."""
+    width: int
+    height: int
+    spp: int
+    threads: int
+    max_depth: int
+    accel: str = "BVH"
+    integrator: str = "PathTracer"
+    sampler: str = "Halton"
+
+    @classmethod
+    def load(cls, path: str) -> "Config":
+        lib = load_library()
+        c = _abi.c2_config()
+        check(lib.c2_config_load(path.encode(), C.byref(c)))
+        return cls(c.width, c.height, c.spp, c.threads, c.max_depth, c.accel.decode(), c.integrator.decode(),
+                   c.sampler.decode())
+
+
+class Scene:
+    """Owns a host scene (entities, triangles, spheres, materials, lights, camera)."""
+
+    def __init__(self, handle):
+        self._lib = load_library()
+        self._h = handle
+
+    @classmethod
+    def load(cls, scene_json: str, base_dir: str | None = None) -> "Scene":
+        lib = load_library()
+        h = C.c_void_p()
+        check(lib.c2_scene_load(scene_json.encode(), base_dir.encode() if base_dir else None, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def create(cls) -> "Scene":
+        lib = load_library()
+        h = C.c_void_p()
+        check(lib.c2_scene_create(C.byref(h)))
+        return cls(h)
+
+    def add_material(self, mtype: int, color, roughness=(0.0, 0.0), ior=1.0, phong_k=0, ndf=0) -> int:
+        m = _abi.c2_material()
+        m.type = mtype
+        m.ndf = ndf
+        m.phong_k = phong_k
+        m.color[:] = [float(x) for x in color]
+        m.roughness[:] = [float(x) for x in roughness]
+        m.ior = float(ior)
+        idx = C.c_int32()
+        check(self._lib.c2_scene_add_material(self._h, C.byref(m), C.byref(idx)))
+        return idx.value
+
+    def add_triangles(self, vertex_data: np.ndarray, material: int, lemit=None, transform=None):
+        """vertex_data: float32 [ntri, 3, 8] (pos, normal, uv) -- unshared corners."""
+        vd = np.ascontiguousarray(vertex_data, dtype=np.float32).reshape(-1, 3, 8)
+        le = None if lemit is None else np.asarray(lemit, np.float32)
+        tr = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(16)
+        check(self._lib.c2_scene_add_triangles(self._h, _fp(vd), vd.shape[0], None if tr is None else _fp(tr),
+                                               material, None if le is None else _fp(le)))
+
+    def add_mesh(self, positions, normals, faces, material: int, lemit=None, uv=None):
+        p = np.asarray(positions, np.float32)
+        n = np.asarray(normals, np.float32)
+        f = np.asarray(faces, np.int64)
+        vd = np.zeros((len(f), 3, 8), np.float32)
+        vd[:, :, 0:3] = p[f]
+        vd[:, :, 3:6] = n[f]
+        if uv is not None:
+            vd[:, :, 6:8] = np.asarray(uv, np.float32)[f]
+        self.add_triangles(vd, material, lemit)
+
+    def add_sphere(self, center, radius: float, material: int, lemit=None, rotation=(0.0, 0.0, 0.0)):
+        c = np.asarray(center, np.float32)
+        r = np.asarray(rotation, np.float32)
+        le = None if lemit is None else np.asarray(lemit, np.float32)
+        check(self._lib.c2_scene_add_sphere(self._h, _fp(c), float(radius), _fp(r), material,
+                                            None if le is None else _fp(le)))
+
+    def add_point_light(self, position, flux):
+        p = np.asarray(position, np.float32)
+        f = np.asarray(flux, np.float32)
+        check(self._lib.c2_scene_add_point_light(self._h, _fp(p), _fp(f)))
+
+    def set_camera(self, position, look_at, up, fov_degrees: float):
+        cam = _abi.c2_camera()
+        cam.position[:] = [float(x) for x in position]
+        cam.look_at[:] = [float(x) for x in look_at]
+        cam.up[:] = [float(x) for x in up]
+        # Camera::CreateCamera: FOV / 180.f * pi<float>() evaluated in double, then float
+        cam.fov = float(np.float32(float(fov_degrees) / float(np.float32(180.0)) * float(np.float32(np.pi))))
+        check(self._lib.c2_scene_set_camera(self._h, C.byref(cam)))
+
+    def desc(self) -> _abi.c2_scene_desc:
+        d = _abi.c2_scene_desc()
+        check(self._lib.c2_scene_get_desc(self._h, C.byref(d)))
+        return d
+
+    @property
+    def num_entities(self) -> int:
+        return self.desc().num_entities
+
+    def close(self):
+        if self._h:
+            self._lib.c2_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuBVH:
+    """Accelerator on the GPU (src/Accelerators/Accelerator.h:8-22): Intersect / IntersectTest."""
+
+    def __init__(self, scene: Scene):
+        self._lib = load_library()
+        self._scene = scene
+        self._h = C.c_void_p()
+        d = scene.desc()
+        check(self._lib.c2_accel_build(C.byref(d), C.byref(self._h)))
+
+    def stats(self) -> dict:
+        s = _abi.c2_accel_stats()
+        check(self._lib.c2_accel_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def intersect(self, rays: np.ndarray):
+        """rays float32 [n, 8] -> (entity int32 [n] (-1 miss), t float32 [n])."""
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = r.shape[0]
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        check(self._lib.c2_accel_intersect(self._h, _fp(r), n, _ip(prim), _fp(t)))
+        return prim, t
+
+    def intersect_test(self, rays: np.ndarray, ignore: np.ndarray | None = None):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = r.shape[0]
+        hit = np.empty(n, np.uint8)
+        ig = None if ignore is None else np.ascontiguousarray(ignore, np.int32)
+        check(self._lib.c2_accel_intersect_test(self._h, _fp(r), None if ig is None else _ip(ig), n,
+                                                hit.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return hit.astype(bool)
+
+    def intersect_device(self, rays_ptr: int, n: int, prim_ptr: int, t_ptr: int, stream: int = 0):
+        check(self._lib.c2_accel_intersect_device(self._h, rays_ptr, n, prim_ptr, t_ptr, stream or None))
+
+    def intersect_test_device(self, rays_ptr: int, ignore_ptr: int, n: int, hit_ptr: int, stream: int = 0):
+        check(self._lib.c2_accel_intersect_test_device(self._h, rays_ptr, ignore_ptr or None, n, hit_ptr,
+                                                       stream or None))
+
+    def count_visits(self, rays: np.ndarray, any_hit: bool = False):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        a, b = C.c_double(), C.c_double()
+        check(self._lib.c2_accel_count_visits(self._h, _fp(r), r.shape[0], int(any_hit), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self._h:
+            self._lib.c2_accel_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WavefrontPathIntegrator:
+    """Integrator::Render (PathTracer + MIS, Halton) on the GPU.
+
+    ``arg_order``: order in which the reference's compiler evaluated
+    ``SampleDirection(sampler->Get1D(), sampler->Get2D(), ...)``
+    (PathTracingIntegrator.cpp:79): "rtl" (GCC, MSVC) or "ltr" (clang).
+    ``film``: "f32" (reference FrameBuffer), "kahan" or "f64".
+    """
+
+    FILMS = {"f32": _abi.C2_FILM_F32, "kahan": _abi.C2_FILM_F32_KAHAN, "f64": _abi.C2_FILM_F64}
+
+    def __init__(self, scene: Scene, width: int, height: int, spp: int, max_depth: int, arg_order: str = "rtl",
+                 film: str = "f32", paths_in_flight: int = 0):
+        self._lib = load_library()
+        self.width, self.height, self.spp, self.max_depth = width, height, spp, max_depth
+        o = _abi.c2_render_opts()
+        o.width, o.height, o.spp, o.max_depth = width, height, spp, max_depth
+        o.arg_order = _abi.C2_ARGORDER_LTR if arg_order == "ltr" else _abi.C2_ARGORDER_RTL
+        o.film_mode = self.FILMS[film]
+        o.paths_in_flight = paths_in_flight
+        self._h = C.c_void_p()
+        d = scene.desc()
+        check(self._lib.c2_pt_create(C.byref(d), C.byref(o), C.byref(self._h)))
+
+    def render(self, sample_begin: int = 0, sample_count: int | None = None, stream: int = 0):
+        n = self.spp - sample_begin if sample_count is None else sample_count
+        check(self._lib.c2_pt_render(self._h, sample_begin, n, stream or None))
+
+    def sync(self):
+        check(self._lib.c2_pt_sync(self._h))
+
+    def reset(self):
+        check(self._lib.c2_pt_reset(self._h))
+
+    def film(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 3), np.float32)
+        check(self._lib.c2_pt_get_film(self._h, _fp(out)))
+        return out
+
+    def stats(self) -> dict:
+        s = _abi.c2_pt_stats()
+        check(self._lib.c2_pt_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def close(self):
+        if self._h:
+            self._lib.c2_pt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def halton_index(width: int, height: int, x: int, y: int, k: int) -> int:
+    lib = load_library()
+    out = C.c_int32()
+    check(lib.c2_halton_index(width, height, x, y, k, C.byref(out)))
+    return out.value
+
+
+def halton_value(width: int, height: int, index: int, dim: int) -> float:
+    lib = load_library()
+    out = C.c_float()
+    check(lib.c2_halton_value(width, height, index, dim, C.byref(out)))
+    return out.value

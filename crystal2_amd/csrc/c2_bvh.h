@@ -1,0 +1,64 @@
+// BVH layouts shared by the host builder (c2_bvh.cpp) and the gfx950 kernels.
+//
+// Build: binned SAH over entity bounding boxes into a binary tree with at most
+// kLeafSize entities per leaf, then collapsed into a 4-wide tree whose nodes are
+// exactly one 128-byte cache line:
+//
+//   Node4 (128 B, 128-B aligned)
+//     lo_x[4] hi_x[4] lo_y[4] hi_y[4] lo_z[4] hi_z[4]   fp32 child boxes (96 B)
+//     child[4]   >= 0 : index of an inner Node4
+//                <  0 : leaf, first primitive slot = ~child
+//     count      4 x 8-bit primitive counts of leaf children (0 for inner / empty)
+//   Empty child slots carry an inverted box (+inf / -inf) and never pass the slab test.
+//
+// Child boxes are the exact fp32 unions (min/max) of the entity boxes below them,
+// so a node test computed with the reference's slab arithmetic
+// ((b - o) * invD, then max/min, src/Core/Geometry.h:136-157) passes whenever
+// the test of any entity box below it would pass: node culling never removes a
+// primitive the reference's per-primitive pre-clip would keep.
+//
+// Primitive slots (leaf order), 48 B = 3 x float4 each:
+//   triangle: (v0.x v0.y v0.z v1.x) (v1.y v1.z v2.x v2.y) (v2.z entity kind=0 -)
+//   sphere:   (c.x  c.y  c.z  r   ) (sphere_index - - -)  (-    entity kind=1 -)
+// Vertices are stored unmodified so e1 = v1 - v0, e2 = v2 - v0 and the entity box
+// (component-wise min/max of v0, v1, v2) are recomputed bit-exactly in the kernel.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace c2 {
+
+constexpr int kLeafSize = 4;
+constexpr int kStackSize = 64;      // per-lane traversal stack (entries)
+constexpr int kMaxBvh4Depth = 21;   // 3 * depth + 1 <= kStackSize
+constexpr int kMaxBvh2Depth = 42;
+
+struct alignas(128) Node4 {
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    int32_t child[4];
+    uint32_t count;  // 8 bits per child
+    uint32_t pad[3];
+};
+static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
+
+struct Box3 {
+    float lo[3], hi[3];
+};
+
+struct Bvh2Node {
+    Box3 box;
+    int32_t left, right;  // inner: children; leaf: left = first slot, right = -count
+    bool leaf() const { return right < 0; }
+};
+
+struct BuiltBvh {
+    std::vector<Bvh2Node> bvh2;        // root at 0
+    std::vector<int32_t> order;        // slot -> entity
+    std::vector<Node4> nodes;          // root at 0
+    int32_t depth2 = 0, depth4 = 0;
+};
+
+// Builds both trees from per-entity boxes. Throws c2::Error on depth overflow.
+void build_bvh(const std::vector<Box3>& boxes, BuiltBvh& out);
+
+}  // namespace c2

@@ -1,0 +1,323 @@
+// Device-side scene layout, ray/primitive intersection and 4-wide BVH traversal
+// for gfx950 (wave64). Included by c2_device.hip only.
+//
+// Arithmetic follows the reference operation by operation (the file is compiled
+// with -ffp-contract=off, divisions and square roots are IEEE correctly rounded):
+//   triangle solve   src/Shapes/Triangle.cpp:32-93 with glm::inverse(mat3)
+//                    (glm/detail/func_matrix.inl:269-291) and mat3*vec3
+//                    (glm/detail/type_mat3x3.inl:468-474)
+//   sphere           src/Shapes/Sphere.cpp:39-88
+//   slab test        src/Core/Geometry.h:136-157 (std::max/std::min semantics)
+//   traversal        src/Accelerators/BVH.cpp:56-154: per-entity box pre-clip with
+//                    [tmin, min(tmax, best)] (closest) / [tmin, tmax] (any), closest by
+//                    strict t with ties to the lowest entity index, any-hit over the
+//                    box-clipped interval, the `ignore` entity skipped.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "c2_bvh.h"
+#include "c2_vec.h"
+#include "../../include/crystal2.h"
+
+namespace c2 {
+
+struct DevLight {
+    int32_t type;       // C2_LIGHT_*
+    int32_t entity;     // AREA: attached entity
+    int32_t shape_kind; // AREA: C2_SHAPE_*
+    int32_t shape;      // AREA: triangle / sphere index
+    float L[3];         // Lemit or Flux
+    float pos[3];       // POINT position
+    float area;         // AREA: Shape::SurfaceArea()
+    int32_t eligible;   // Flux() != 0 (UniformSampleAllLights skips zero-flux lights)
+};
+
+struct DevScene {
+    const Node4* nodes;
+    const float4* prims;       // 3 float4 per slot
+    const int4* entities;      // (kind, shape, material, light)
+    const float* tri_pos;      // [ntri][9]
+    const float* tri_shade;    // [ntri][12]: n0, n1, n2, dpdu (0 = derive from N)
+    const float* spheres;      // [nsph][22]
+    const c2_material* materials;
+    const DevLight* lights;
+    const uint16_t* perm;      // Halton permutation tables, dims 0..99
+    const int32_t* perm_off;
+    int32_t num_lights;
+    int32_t num_entities;
+};
+
+// ------------------------------------------------------------------------- slab test
+struct RayPre {
+    V3 o, d, invD;
+    bool neg[3];
+};
+C2_HD RayPre make_ray(V3 o, V3 d) {
+    RayPre r;
+    r.o = o;
+    r.d = d;
+    r.invD = V3{1.f / d.x, 1.f / d.y, 1.f / d.z};
+    r.neg[0] = r.invD.x < 0;
+    r.neg[1] = r.invD.y < 0;
+    r.neg[2] = r.invD.z < 0;
+    return r;
+}
+// RayBoxTest(ray, inv, invD, box, tMin, tMax), Geometry.h:136-157
+C2_HD bool slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                     float& tMin, float& tMax) {
+    float a, b;
+    a = (lx - r.o.x) * r.invD.x;
+    b = (hx - r.o.x) * r.invD.x;
+    if (r.neg[0]) { float t = a; a = b; b = t; }
+    tMin = fmax_ref(tMin, a);
+    tMax = fmin_ref(tMax, b);
+    if (tMax < tMin) return false;
+    a = (ly - r.o.y) * r.invD.y;
+    b = (hy - r.o.y) * r.invD.y;
+    if (r.neg[1]) { float t = a; a = b; b = t; }
+    tMin = fmax_ref(tMin, a);
+    tMax = fmin_ref(tMax, b);
+    if (tMax < tMin) return false;
+    a = (lz - r.o.z) * r.invD.z;
+    b = (hz - r.o.z) * r.invD.z;
+    if (r.neg[2]) { float t = a; a = b; b = t; }
+    tMin = fmax_ref(tMin, a);
+    tMax = fmin_ref(tMax, b);
+    if (tMax < tMin) return false;
+    return true;
+}
+
+// ------------------------------------------------------------------------- triangle
+// res = glm::inverse(mat3(v1 - v0, v2 - v0, -d)) * (o - v0)
+C2_HD V3 tri_solve(V3 o, V3 d, V3 v0, V3 v1, V3 v2) {
+    V3 e1 = v1 - v0, e2 = v2 - v0, md = -d, P = o - v0;
+    float m00 = e1.x, m01 = e1.y, m02 = e1.z;
+    float m10 = e2.x, m11 = e2.y, m12 = e2.z;
+    float m20 = md.x, m21 = md.y, m22 = md.z;
+    float c0 = m11 * m22 - m21 * m12;
+    float c1 = m01 * m22 - m21 * m02;
+    float c2 = m01 * m12 - m11 * m02;
+    float ood = 1.f / (m00 * c0 - m10 * c1 + m20 * c2);
+    float i00 = c0 * ood;
+    float i10 = -(m10 * m22 - m20 * m12) * ood;
+    float i20 = (m10 * m21 - m20 * m11) * ood;
+    float i01 = -c1 * ood;
+    float i11 = (m00 * m22 - m20 * m02) * ood;
+    float i21 = -(m00 * m21 - m20 * m01) * ood;
+    float i02 = c2 * ood;
+    float i12 = -(m00 * m12 - m10 * m02) * ood;
+    float i22 = (m00 * m11 - m10 * m01) * ood;
+    V3 res;
+    res.x = i00 * P.x + i10 * P.y + i20 * P.z;
+    res.y = i01 * P.x + i11 * P.y + i21 * P.z;
+    res.z = i02 * P.x + i12 * P.y + i22 * P.z;
+    return res;
+}
+// Acceptance of Triangle::Intersect (closest hit, u+v slack EPS=1e-5), Triangle.cpp:39-40
+C2_HD bool tri_accept_closest(V3 res) {
+    if (res.x != res.x || res.y != res.y || res.z != res.z) return false;
+    if (res.x < 0.f || res.x > 1.f || res.y < 0.f || res.y > 1.f || res.x + res.y > 1.0f + 1e-5f || res.z < 0.f)
+        return false;
+    return true;
+}
+// Triangle::IntersectTest on the box-clipped interval, Triangle.cpp:83-93
+C2_HD bool tri_test(V3 o, V3 d, V3 v0, V3 v1, V3 v2, float tMin, float tMax) {
+    if (tMin > tMax) return false;
+    V3 res = tri_solve(o, d, v0, v1, v2);
+    if (res.x != res.x || res.y != res.y || res.z != res.z) return false;
+    if (res.x < 0 || res.x > 1 || res.y < 0 || res.y > 1 || res.x + res.y > 1 || res.z < 0) return false;
+    return res.z >= tMin && res.z <= tMax;
+}
+
+// ------------------------------------------------------------------------- sphere
+C2_HD V3 mat3_mul(const float* m, V3 v) {  // m: column-major 3x3
+    return V3{m[0] * v.x + m[3] * v.y + m[6] * v.z, m[1] * v.x + m[4] * v.y + m[7] * v.z,
+              m[2] * v.x + m[5] * v.y + m[8] * v.z};
+}
+C2_HD bool sphere_quad(const float* sp, V3 o, V3 d, V3& P, V3& dl, float& t1, float& t2) {
+    V3 c{sp[0], sp[1], sp[2]};
+    float r = sp[3];
+    P = mat3_mul(sp + 13, o - c);
+    dl = mat3_mul(sp + 13, d);
+    float a = dot(dl, dl);
+    float b = 2 * dot(dl, P);
+    float cc = dot(P, P) - r * r;
+    float disc = b * b - 4 * a * cc;
+    if (disc < 0) return false;
+    disc = sqrtf(disc);
+    t1 = (-b + disc) / (2 * a);
+    t2 = (-b - disc) / (2 * a);
+    return true;
+}
+// Sphere::Intersect distance selection, Sphere.cpp:44-49
+C2_HD bool sphere_closest(const float* sp, V3 o, V3 d, float& t) {
+    V3 P, dl;
+    float t1, t2;
+    if (!sphere_quad(sp, o, d, P, dl, t1, t2)) return false;
+    if (t1 > t2) { float x = t1; t1 = t2; t2 = x; }
+    t = t1;
+    if (t1 != t1 || t1 < 0) t = t2;
+    if (t != t || t < 0) return false;
+    return true;
+}
+C2_HD bool sphere_test(const float* sp, V3 o, V3 d, float tMin, float tMax) {
+    if (tMin > tMax) return false;
+    V3 P, dl;
+    float t1, t2;
+    if (!sphere_quad(sp, o, d, P, dl, t1, t2)) return false;
+    return (t1 >= tMin && t1 <= tMax) || (t2 >= tMin && t2 <= tMax);
+}
+
+// ------------------------------------------------------------------------- traversal
+struct Hit {
+    int32_t entity;  // -1: miss
+    float t, u, v;
+};
+
+// Stack entry: >= 0 inner node; < 0 leaf = ~(slot | (count - 1) << 28).
+constexpr int32_t kSlotBits = 28;
+
+__device__ __forceinline__ void load_prim(const DevScene& S, int slot, float4& a, float4& b, float4& c) {
+    a = S.prims[3 * slot + 0];
+    b = S.prims[3 * slot + 1];
+    c = S.prims[3 * slot + 2];
+}
+
+// Closest hit (ANY = false) or any hit (ANY = true). Closest hit: tmin = 0 and
+// tmax = ray tmax (the reference passes 0, +inf). Returns true on a hit.
+template <bool ANY>
+__device__ __forceinline__ bool traverse(const DevScene& S, V3 o, V3 d, float tmin, float tmax, int ignore,
+                                         Hit& hit) {
+    RayPre r = make_ray(o, d);
+    int32_t stack[kStackSize];
+    float stackT[kStackSize];
+    int sp = 0;
+    int32_t cur = 0;
+    float best = INFINITY;
+    int32_t bestE = -1;
+    float bestU = 0.f, bestV = 0.f;
+    while (true) {
+        if (cur >= 0) {
+            const Node4* nd = S.nodes + cur;
+            float4 lx = *reinterpret_cast<const float4*>(nd->lo_x);
+            float4 hx = *reinterpret_cast<const float4*>(nd->hi_x);
+            float4 ly = *reinterpret_cast<const float4*>(nd->lo_y);
+            float4 hy = *reinterpret_cast<const float4*>(nd->hi_y);
+            float4 lz = *reinterpret_cast<const float4*>(nd->lo_z);
+            float4 hz = *reinterpret_cast<const float4*>(nd->hi_z);
+            int4 ch = *reinterpret_cast<const int4*>(nd->child);
+            uint32_t cnt = nd->count;
+            float lim = ANY ? tmax : fmin_ref(tmax, best);
+            float k0 = INFINITY, k1 = INFINITY, k2 = INFINITY, k3 = INFINITY;
+            int32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
+#define C2_CHILD(I, LXC, HXC, LYC, HYC, LZC, HZC, CH, KOUT, EOUT)                                     \
+    {                                                                                                \
+        uint32_t c8 = (cnt >> (8 * I)) & 0xffu;                                                       \
+        if (CH >= 0 || c8 != 0) {                                                                    \
+            float t0 = tmin, t1 = lim;                                                               \
+            if (slab(r, LXC, LYC, LZC, HXC, HYC, HZC, t0, t1)) {                                     \
+                KOUT = t0;                                                                           \
+                EOUT = CH >= 0 ? CH : ~((int32_t)((uint32_t)(~CH) | ((c8 - 1u) << kSlotBits)));      \
+            }                                                                                        \
+        }                                                                                            \
+    }
+            C2_CHILD(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x, k0, e0)
+            C2_CHILD(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y, k1, e1)
+            C2_CHILD(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z, k2, e2)
+            C2_CHILD(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w, k3, e3)
+#undef C2_CHILD
+            // sort the four (key, entry) pairs ascending; misses carry +inf keys
+#define C2_CSWAP(KA, EA, KB, EB)                       \
+    if (KB < KA) {                                     \
+        float tk = KA; KA = KB; KB = tk;               \
+        int32_t te = EA; EA = EB; EB = te;             \
+    }
+            C2_CSWAP(k0, e0, k1, e1)
+            C2_CSWAP(k2, e2, k3, e3)
+            C2_CSWAP(k0, e0, k2, e2)
+            C2_CSWAP(k1, e1, k3, e3)
+            C2_CSWAP(k1, e1, k2, e2)
+#undef C2_CSWAP
+            if (k3 != INFINITY) { stack[sp] = e3; stackT[sp] = k3; sp++; }
+            if (k2 != INFINITY) { stack[sp] = e2; stackT[sp] = k2; sp++; }
+            if (k1 != INFINITY) { stack[sp] = e1; stackT[sp] = k1; sp++; }
+            if (k0 != INFINITY) {
+                cur = e0;
+                continue;
+            }
+        } else {
+            uint32_t enc = (uint32_t)(~cur);
+            int slot = (int)(enc & ((1u << kSlotBits) - 1u));
+            int n = (int)(enc >> kSlotBits) + 1;
+            for (int i = 0; i < n; i++) {
+                float4 a, b, c;
+                load_prim(S, slot + i, a, b, c);
+                int32_t ent = __float_as_int(c.y);
+                int32_t kind = __float_as_int(c.z);
+                float t1 = tmin, t2 = ANY ? tmax : fmin_ref(tmax, best);
+                if (kind == C2_SHAPE_TRIANGLE) {
+                    V3 v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
+                    float lxx = fmin_ref(fmin_ref(v0.x, v1.x), v2.x), lyy = fmin_ref(fmin_ref(v0.y, v1.y), v2.y);
+                    float lzz = fmin_ref(fmin_ref(v0.z, v1.z), v2.z), hxx = fmax_ref(fmax_ref(v0.x, v1.x), v2.x);
+                    float hyy = fmax_ref(fmax_ref(v0.y, v1.y), v2.y), hzz = fmax_ref(fmax_ref(v0.z, v1.z), v2.z);
+                    if (!slab(r, lxx, lyy, lzz, hxx, hyy, hzz, t1, t2)) continue;
+                    if (ANY) {
+                        if (ent == ignore) continue;
+                        if (tri_test(o, d, v0, v1, v2, t1, t2)) {
+                            hit.entity = ent;
+                            return true;
+                        }
+                    } else {
+                        V3 res = tri_solve(o, d, v0, v1, v2);
+                        if (!tri_accept_closest(res)) continue;
+                        float dis = res.z;
+                        if (dis < best || (dis == best && ent < bestE)) {
+                            best = dis;
+                            bestE = ent;
+                            bestU = res.x;
+                            bestV = res.y;
+                        }
+                    }
+                } else {
+                    const float* sp3 = S.spheres + (size_t)__float_as_int(b.x) * 22;
+                    float rr = a.w;
+                    if (!slab(r, a.x - rr, a.y - rr, a.z - rr, a.x + rr, a.y + rr, a.z + rr, t1, t2)) continue;
+                    if (ANY) {
+                        if (ent == ignore) continue;
+                        if (sphere_test(sp3, o, d, t1, t2)) {
+                            hit.entity = ent;
+                            return true;
+                        }
+                    } else {
+                        float dis;
+                        if (!sphere_closest(sp3, o, d, dis)) continue;
+                        if (dis < best || (dis == best && ent < bestE)) {
+                            best = dis;
+                            bestE = ent;
+                            bestU = 0.f;
+                            bestV = 0.f;
+                        }
+                    }
+                }
+            }
+        }
+        // pop; a closest-hit entry whose entry distance is already beyond the best hit
+        // cannot contain a closer one (its slab interval starts after `best`).
+        while (true) {
+            if (sp == 0) {
+                if (ANY) return false;
+                hit.entity = bestE;
+                hit.t = best;
+                hit.u = bestU;
+                hit.v = bestV;
+                return bestE >= 0;
+            }
+            sp--;
+            if (!ANY && stackT[sp] > best) continue;
+            cur = stack[sp];
+            break;
+        }
+    }
+}
+
+}  // namespace c2

@@ -1,0 +1,1126 @@
+// gfx950 kernels and the device half of the C-ABI (include/crystal2.h):
+// Accelerator (4-wide BVH closest / any hit) and the wavefront PathTracer + MIS.
+//
+// Wavefront pass for Halton sample k over a chunk of pixels (paths in flight):
+//   generate  camera ray + sampler state           SamplerIntegrator.cpp:42-52, PerspCamera.cpp:23-29
+//   per bounce b < Depth:
+//     extend   closest hit of every queued path     BVH::Intersect
+//     shade    emission, BSDF, NEE light samples -> shadow slots, BSDF-sampled MIS
+//              ray (one per vertex, see below), continuation + Russian roulette
+//                                                   PathTracingIntegrator.cpp:36-109,175-278
+//     occlude  any hit of every shadow ray          BVH::IntersectTest
+//     mis      closest hit of every MIS ray
+//     resolve  L += beta * sum over lights (in the reference's light order) of the
+//              unshadowed light samples and the MIS term, then beta <- beta'
+//   accumulate film += L (NaN/Inf -> 0), one sample per pixel per pass, in sample
+//              order                                FrameBuffer.cpp:20-32
+//
+// MIS ray: every supported material has a single BxDF, so the BSDF-sampled
+// direction in EsimateDirect (PathTracingIntegrator.cpp:240) is the same for all
+// lights (same sampleBSDF, BxDF index 0). The reference traces that ray once per
+// light whose own shape it hits (Pdf_Li != 0); here it is traced once and only
+// the light owning the hit entity can receive the term, which is exactly the
+// reference's sum.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "c2_bvh.h"
+#include "c2_device.h"
+#include "c2_host.h"
+#include "c2_shade.h"
+
+namespace c2 {
+
+#define HIP_CHECK(x)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess)                                                                         \
+            fail(C2_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_));                      \
+    } while (0)
+
+constexpr int kBlock = 256;
+
+// ============================================================================ accelerator kernels
+
+__global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, const float* __restrict__ rays, int64_t n,
+                                                      int32_t* __restrict__ prim, float* __restrict__ tout) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
+        float4 a = r4[0], b = r4[1];
+        Hit h;
+        bool hit = traverse<false>(S, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h);
+        prim[i] = hit ? h.entity : -1;
+        tout[i] = hit ? h.t : INFINITY;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_intersect_test(DevScene S, const float* __restrict__ rays,
+                                                           const int32_t* __restrict__ ignore, int64_t n,
+                                                           uint8_t* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
+        float4 a = r4[0], b = r4[1];
+        Hit h;
+        bool hit = traverse<true>(S, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ignore ? ignore[i] : -1, h);
+        out[i] = hit ? 1 : 0;
+    }
+}
+
+// ============================================================================ wavefront state
+
+enum : int32_t {
+    F_LIGHTPATH = 1,  // add emission at the next hit (specular or transmission bounce)
+    F_CONTINUE = 2,   // path continues after this vertex
+    F_NEE = 4,        // this vertex ran UniformSampleAllLights (resolve adds beta * U)
+    F_MIS = 8,        // MIS ray queued for this vertex
+};
+
+struct PathBuffers {
+    float4* ray_o;   // o.xyz
+    float4* ray_d;   // d.xyz
+    float4* beta;    // beta at the current vertex
+    float4* beta2;   // beta after the continuation (applied by resolve)
+    float4* L;       // path radiance
+    int4* meta;      // pixel, halton index, dimension, flags
+    float4* hit;     // t, u, v, entity bits
+    int4* nee;       // shadow slot count, own light, MIS hit light, -
+    float4* mis_p;   // surface p.xyz, pdf_bsdf
+    float4* mis_n;   // facing normal
+    float4* mis_w;   // BSDF-sampled direction
+    float4* mis_f;   // f * cos
+    float4* mis_L;   // resolved MIS term
+    float4* sh_o;    // shadow ray origin, ignore entity bits
+    float4* sh_d;    // shadow ray direction, light index bits
+    float4* sh_c;    // unshadowed contribution
+    uint8_t* sh_vis; // 1 = unoccluded
+    int32_t* queue[2];
+    int32_t* sh_list;
+    int32_t* mis_list;
+    int32_t* counters;  // [0..1] queue counts, [2] shadow count, [3] mis count, [4..7] ray totals (int)
+    unsigned long long* totals;  // extension, shadow, mis rays (64-bit running totals)
+};
+
+struct PassParams {
+    int32_t width, height;
+    int32_t sample;          // Halton sample number k
+    int32_t pix_begin, pix_count;
+    int32_t max_depth;
+    int32_t arg_order;
+    int32_t lmax;            // shadow slots per path
+    int32_t stride0, exp0, scale1, mult0, mult1, bscale0, bexp1;  // Halton constants
+    V3 eye, look, up, right;
+    float camW, camH;
+};
+
+__device__ __forceinline__ int32_t halton_index_dev(const PassParams& P, int x, int y, int k) {
+    // HaltonSampler::GetVectorIndexForSample, HaltonSampler.cpp:251-270
+    int64_t off = 0;
+    if (P.stride0 > 1) {
+        uint64_t v = (uint64_t)(x % 128), r0 = 0;
+        for (int i = 0; i < P.exp0; i++) { r0 = r0 * 2 + (v % 2); v /= 2; }
+        off += (int64_t)(r0 * (uint64_t)(int64_t)P.scale1 * (uint64_t)(int64_t)P.mult0);
+        v = (uint64_t)(y % 128);
+        uint64_t r1 = 0;
+        for (int i = 0; i < P.bexp1; i++) { r1 = r1 * 3 + (v % 3); v /= 3; }
+        off += (int64_t)(r1 * (uint64_t)(int64_t)P.bscale0 * (uint64_t)(int64_t)P.mult1);
+    }
+    return (int32_t)(off + (int64_t)(k * P.stride0));
+}
+
+// wave-aggregated append: one atomic per wavefront
+__device__ __forceinline__ int32_t wave_append(int32_t* counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((long long)mask) - 1;
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (int32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (!pred) return -1;
+    uint64_t lower = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return base + (int32_t)__popcll(lower);
+}
+
+__global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, PassParams P) {
+    HaltonDev hd{P.exp0, P.scale1};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.pix_count; i += gridDim.x * blockDim.x) {
+        int pixel = P.pix_begin + i;
+        int x = pixel % P.width, y = pixel / P.width;
+        int32_t idx = halton_index_dev(P, x, y, P.sample);
+        float s0 = halton_value(S, hd, idx, 0), s1 = halton_value(S, hd, idx, 1);
+        float px = (float)x + s0, py = (float)y + s1;
+        px = px / P.width;
+        py = py / P.height;
+        // PerspCamera::GenerateRay
+        px = px * 2 - 1;
+        py = py * 2 - 1;
+        px *= P.camW, py *= P.camH;
+        V3 d = normalize(P.right * px + P.up * py + P.look * 1.0f);
+        B.ray_o[i] = make_float4(P.eye.x, P.eye.y, P.eye.z, 0.f);
+        B.ray_d[i] = make_float4(d.x, d.y, d.z, 0.f);
+        B.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+        B.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        B.meta[i] = make_int4(pixel, idx, 2, F_LIGHTPATH);
+        B.queue[0][i] = i;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) B.counters[0] = P.pix_count;
+}
+
+__global__ void k_reset(PathBuffers B, int next_q) {
+    B.counters[next_q] = 0;
+    B.counters[2] = 0;
+    B.counters[3] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, int q) {
+    int n = B.counters[q];
+    const int32_t* queue = B.queue[q];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[0], (unsigned long long)n);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int p = queue[i];
+        float4 o = B.ray_o[p], d = B.ray_d[p];
+        Hit h;
+        bool hit = traverse<false>(S, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY, -1, h);
+        B.hit[p] = make_float4(h.t, h.u, h.v, __int_as_float(hit ? h.entity : -1));
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, PassParams P, int q, int bounce) {
+    int n = B.counters[q];
+    const int32_t* queue = B.queue[q];
+    int32_t* next = B.queue[q ^ 1];
+    HaltonDev hd{P.exp0, P.scale1};
+    // grid-stride loop over block-uniform bases: every lane of a wavefront runs the
+    // same iterations, as wave_append's ballot requires
+    int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        int i = base + (int)threadIdx.x;
+        bool live = i < n;
+        int p = live ? queue[i] : 0;
+        int4 meta = live ? B.meta[p] : make_int4(0, 0, 0, 0);
+        int flags = meta.w & F_LIGHTPATH;
+        float4 hh = live ? B.hit[p] : make_float4(0, 0, 0, 0);
+        int ent = __float_as_int(hh.w);
+        bool active = live && ent >= 0;
+        bool cont = false;
+        int nsh = 0, own = -1;
+        bool misNeeded = false;
+        V3 misF{0, 0, 0}, misW{0, 0, 0};
+        float misPdf = 0.f;
+        Inter it;
+        float4 ro = live ? B.ray_o[p] : make_float4(0, 0, 0, 0);
+        float4 rd = live ? B.ray_d[p] : make_float4(0, 0, 0, 0);
+        V3 o{ro.x, ro.y, ro.z}, d{rd.x, rd.y, rd.z};
+        float4 b4 = live ? B.beta[p] : make_float4(0, 0, 0, 0);
+        V3 beta{b4.x, b4.y, b4.z};
+        V3 beta2 = beta;
+        int32_t idx = meta.y, dim = meta.z;
+        if (active) {
+            Hit h{ent, hh.x, hh.y, hh.z};
+            it = make_inter(S, o, d, h);
+            own = it.light;
+            // emission (lightPath), :54-57
+            if (meta.w & F_LIGHTPATH) {
+                V3 Le{0, 0, 0};
+                if (it.light >= 0) {
+                    const DevLight& Ld = S.lights[it.light];
+                    V3 nact = it.front ? it.n : -it.n;
+                    Le = dot(nact, -d) < 0 ? V3{0, 0, 0} : V3{Ld.L[0], Ld.L[1], Ld.L[2]};
+                }
+                float4 L4 = B.L[p];
+                V3 L = V3{L4.x, L4.y, L4.z} + beta * Le;
+                B.L[p] = make_float4(L.x, L.y, L.z, 0.f);
+            }
+            if (it.material < 0) {
+                // transparent: SpawnRay(dir) and continue without consuming dimensions, :63-68
+                V3 no = spawn_origin(it.p, it.n, d);
+                B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
+                cont = true;
+                flags = meta.w & F_LIGHTPATH;
+            } else {
+                Bsdf bs = make_bsdf(S, it, (uint32_t)idx, (uint32_t)(dim + 2));
+                // UniformSampleAllLights, :175-186
+                float uL0 = halton_value(S, hd, idx, dim), uL1 = halton_value(S, hd, idx, dim + 1);
+                float uB0 = halton_value(S, hd, idx, dim + 2), uB1 = halton_value(S, hd, idx, dim + 3);
+                dim += 4;
+                V3 wo = -d;
+                const int sampleType = T_ALL & ~T_SPEC;
+                bool bsdfSampled = false;
+                V3 fB{0, 0, 0}, wiB{0, 0, 0};
+                float pdfB = 0.f;
+                int typeB = 0;
+                int slot0 = p * P.lmax;
+                for (int li = 0; li < S.num_lights; li++) {
+                    const DevLight& Lt = S.lights[li];
+                    if (!Lt.eligible || li == own) continue;
+                    bool delta = Lt.type == C2_LIGHT_POINT;
+                    // EsimateDirect (i): light sampling, :199-230
+                    V3 lightPos;
+                    float pdfL;
+                    V3 Li = sample_li(S, Lt, it.p, it.n, uL0, uL1, lightPos, pdfL);
+                    V3 wi = normalize(lightPos - it.p);
+                    if (pdfL != 0.f && !is_zero(Li)) {
+                        V3 f = bsdf_f(bs, it, wo, wi) * fmax_ref(0.f, dot(it.n, wi));
+                        float pB = bsdf_pdf(bs, it, wo, wi, sampleType);
+                        if (!is_zero(f)) {
+                            V3 C = delta ? f * Li / pdfL : f * Li * power_heuristic(pdfL, pB) / pdfL;
+                            // SpawnRayTo(lightPos)
+                            bool back = dot(lightPos - it.p, it.n) < 0;
+                            V3 so = it.p + (back ? -it.n : it.n) * 1e-4f;
+                            V3 sd = lightPos - so;
+                            int s = slot0 + nsh;
+                            B.sh_o[s] = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
+                            B.sh_d[s] = make_float4(sd.x, sd.y, sd.z, __int_as_float(li));
+                            B.sh_c[s] = make_float4(C.x, C.y, C.z, 0.f);
+                            B.sh_vis[s] = 0;
+                            nsh++;
+                        }
+                    }
+                    // EsimateDirect (ii): BSDF sampling, :235-274 (one Get1D per non-delta light)
+                    if (!delta) {
+                        dim++;
+                        if (!bsdfSampled) {
+                            bsdfSampled = true;
+                            fB = bsdf_sample(bs, it, wo, uB0, uB1, sampleType, wiB, pdfB, typeB);
+                            if (!is_zero(fB)) {
+                                misF = fB * fmax_ref(0.f, dot(it.n, wiB));
+                                misW = wiB;
+                                misPdf = pdfB;
+                            }
+                        }
+                        if (!is_zero(fB) && !misNeeded) {
+                            if (pdf_light(S, Lt, it.p, it.n, wiB) != 0.f) misNeeded = true;
+                        }
+                    }
+                }
+                // continuation sample: sampler->Get1D(), sampler->Get2D() as arguments of one call, :79
+                float u1, u2x, u2y;
+                uint32_t sdim;
+                if (P.arg_order == C2_ARGORDER_RTL) {
+                    sdim = (uint32_t)dim;
+                    u2x = halton_value(S, hd, idx, dim);
+                    u2y = halton_value(S, hd, idx, dim + 1);
+                    u1 = halton_value(S, hd, idx, dim + 2);
+                } else {
+                    u1 = halton_value(S, hd, idx, dim);
+                    sdim = (uint32_t)(dim + 1);
+                    u2x = halton_value(S, hd, idx, dim + 1);
+                    u2y = halton_value(S, hd, idx, dim + 2);
+                }
+                (void)u1;  // one BxDF: BSDF::SampleDirection picks index 0 for every u1
+                dim += 3;
+                bs.rdim = sdim;
+                V3 wiC{0, 0, 0};
+                float pdfC = 0.f;
+                int typeC = 0;
+                V3 fC = bsdf_sample(bs, it, wo, u2x, u2y, T_ALL, wiC, pdfC, typeC);
+                flags = F_NEE;
+                if (!(fabsf(pdfC) == 0.f || is_zero(fC))) {
+                    bool specular = (typeC & T_SPEC) != 0;
+                    float cosine = specular ? 1.0f
+                                            : fmax_ref(0.f, (typeC & T_TRANS) ? dot(-it.n, wiC) : dot(it.n, wiC));
+                    bool lp = specular || (typeC & T_TRANS);
+                    V3 no = spawn_origin(it.p, it.n, wiC);
+                    beta2 = beta * (fC * cosine / pdfC);
+                    cont = true;
+                    if (bounce > 3) {  // Russian roulette, :96-102
+                        float qq = fmax_ref(.05f, 1.f - length(beta2));
+                        float ur = halton_value(S, hd, idx, dim);
+                        dim++;
+                        if (ur < qq) cont = false;
+                        else beta2 = beta2 / (1 - qq);
+                    }
+                    if (cont) {
+                        B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
+                        B.ray_d[p] = make_float4(wiC.x, wiC.y, wiC.z, 0.f);
+                        if (lp) flags |= F_LIGHTPATH;
+                    }
+                }
+            }
+        }
+        if (live) {
+            if (cont) flags |= F_CONTINUE;
+            if (misNeeded) flags |= F_MIS;
+            B.meta[p] = make_int4(meta.x, idx, dim, flags);
+            B.beta2[p] = make_float4(beta2.x, beta2.y, beta2.z, 0.f);
+            B.nee[p] = make_int4(nsh, own, -1, 0);
+            if (misNeeded) {
+                B.mis_p[p] = make_float4(it.p.x, it.p.y, it.p.z, misPdf);
+                B.mis_n[p] = make_float4(it.n.x, it.n.y, it.n.z, 0.f);
+                B.mis_w[p] = make_float4(misW.x, misW.y, misW.z, 0.f);
+                B.mis_f[p] = make_float4(misF.x, misF.y, misF.z, 0.f);
+            }
+        }
+        // queue appends (all lanes of the wave participate)
+        int qpos = wave_append(&B.counters[q ^ 1], live && cont);
+        if (qpos >= 0) next[qpos] = p;
+        int mpos = wave_append(&B.counters[3], live && misNeeded);
+        if (mpos >= 0) B.mis_list[mpos] = p;
+        // shadow slots: append each valid slot index
+        int maxn = nsh;
+        for (int off = 32; off > 0; off >>= 1) maxn = max(maxn, __shfl_xor(maxn, off));
+        for (int j = 0; j < maxn; j++) {
+            int spos = wave_append(&B.counters[2], live && j < nsh);
+            if (spos >= 0) B.sh_list[spos] = p * P.lmax + j;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
+    int n = B.counters[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[1], (unsigned long long)n);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int s = B.sh_list[i];
+        float4 o = B.sh_o[s], d = B.sh_d[s];
+        Hit h;
+        bool occluded =
+            traverse<true>(S, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f, __float_as_int(o.w), h);
+        B.sh_vis[s] = occluded ? 0 : 1;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B) {
+    int n = B.counters[3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[2], (unsigned long long)n);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int p = B.mis_list[i];
+        float4 mp = B.mis_p[p], mn = B.mis_n[p], mw = B.mis_w[p], mf = B.mis_f[p];
+        V3 sp{mp.x, mp.y, mp.z}, sn{mn.x, mn.y, mn.z}, wi{mw.x, mw.y, mw.z}, f{mf.x, mf.y, mf.z};
+        float pdfB = mp.w;
+        int own = B.nee[p].y;
+        V3 o = spawn_origin(sp, sn, wi);
+        Hit h;
+        int lh = -1;
+        V3 Lb{0, 0, 0};
+        if (traverse<false>(S, o, wi, 0.f, INFINITY, -1, h)) {
+            int4 e = S.entities[h.entity];
+            int li = e.w;
+            if (li >= 0 && li != own) {
+                const DevLight& Lt = S.lights[li];
+                if (Lt.eligible && Lt.type == C2_LIGHT_AREA) {
+                    float pdfL = pdf_light(S, Lt, sp, sn, wi);
+                    if (pdfL != 0.f) {
+                        float weight = power_heuristic(pdfB, pdfL);
+                        // lightIsec.Le(-wi): un-flipped normal of the hit
+                        V3 hp, hn, hdu;
+                        bool front;
+                        if (e.x == C2_SHAPE_TRIANGLE) {
+                            tri_frame(S, e.y, wi, h.u, h.v, hn, hdu, front);
+                        } else {
+                            sphere_frame(S.spheres + (size_t)e.y * 22, o, wi, h.t, hp, hn, hdu, front);
+                        }
+                        V3 nact = front ? hn : -hn;
+                        V3 Li = dot(nact, -wi) < 0 ? V3{0, 0, 0} : V3{Lt.L[0], Lt.L[1], Lt.L[2]};
+                        if (!is_zero(Li)) {
+                            Lb = f * Li * weight / pdfB;
+                            lh = li;
+                        }
+                    }
+                }
+            }
+        }
+        B.mis_L[p] = make_float4(Lb.x, Lb.y, Lb.z, 0.f);
+        int4 ne = B.nee[p];
+        ne.z = lh;
+        B.nee[p] = ne;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_resolve(PathBuffers B, PassParams P, int q) {
+    int n = B.counters[q];
+    const int32_t* queue = B.queue[q];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int p = queue[i];
+        int4 meta = B.meta[p];
+        if (meta.w & F_NEE) {
+            int4 ne = B.nee[p];
+            int lh = (meta.w & F_MIS) ? ne.z : -1;
+            V3 Lb{0, 0, 0};
+            if (lh >= 0) {
+                float4 m = B.mis_L[p];
+                Lb = V3{m.x, m.y, m.z};
+            }
+            V3 U{0, 0, 0};
+            bool misDone = lh < 0;
+            int slot0 = p * P.lmax;
+            for (int j = 0; j < ne.x; j++) {
+                int s = slot0 + j;
+                int li = __float_as_int(B.sh_d[s].w);
+                if (!misDone && lh < li) {
+                    U = U + (V3{0, 0, 0} + Lb);
+                    misDone = true;
+                }
+                V3 E{0, 0, 0};
+                bool any = false;
+                if (B.sh_vis[s]) {
+                    float4 c = B.sh_c[s];
+                    E = E + V3{c.x, c.y, c.z};
+                    any = true;
+                }
+                if (!misDone && lh == li) {
+                    E = E + Lb;
+                    misDone = true;
+                    any = true;
+                }
+                if (any) U = U + E;
+            }
+            if (!misDone) U = U + (V3{0, 0, 0} + Lb);
+            float4 b4 = B.beta[p], L4 = B.L[p];
+            V3 L = V3{L4.x, L4.y, L4.z} + V3{b4.x, b4.y, b4.z} * U;
+            B.L[p] = make_float4(L.x, L.y, L.z, 0.f);
+        }
+        if (meta.w & F_CONTINUE) B.beta[p] = B.beta2[p];
+    }
+}
+
+// FrameBuffer::AddSample, FrameBuffer.cpp:20-32 (weight 1, in sample order)
+__global__ void __launch_bounds__(kBlock) k_accumulate(PathBuffers B, PassParams P, int mode, float4* film,
+                                                       float4* comp, double4* film64) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.pix_count; i += gridDim.x * blockDim.x) {
+        float4 L4 = B.L[i];
+        V3 c{L4.x, L4.y, L4.z};
+        if (any_nan(c)) c = V3{0, 0, 0};
+        if (any_inf(c)) c = V3{0, 0, 0};
+        int pixel = B.meta[i].x;
+        if (mode == C2_FILM_F64) {
+            double4 f = film64[pixel];
+            f.x += (double)c.x;
+            f.y += (double)c.y;
+            f.z += (double)c.z;
+            f.w += 1.0;
+            film64[pixel] = f;
+        } else if (mode == C2_FILM_F32_KAHAN) {
+            float4 f = film[pixel], cp = comp[pixel];
+            float y, t;
+            y = c.x - cp.x; t = f.x + y; cp.x = (t - f.x) - y; f.x = t;
+            y = c.y - cp.y; t = f.y + y; cp.y = (t - f.y) - y; f.y = t;
+            y = c.z - cp.z; t = f.z + y; cp.z = (t - f.z) - y; f.z = t;
+            f.w += 1.f;
+            film[pixel] = f;
+            comp[pixel] = cp;
+        } else {
+            float4 f = film[pixel];
+            V3 acc = V3{f.x, f.y, f.z} + c * 1.f;
+            film[pixel] = make_float4(acc.x, acc.y, acc.z, f.w + 1.f);
+        }
+    }
+}
+
+__global__ void k_zero4(float4* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = make_float4(0, 0, 0, 0);
+}
+__global__ void k_zero_d4(double4* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = make_double4(0, 0, 0, 0);
+}
+
+// ============================================================================ host side
+
+struct DeviceArena {
+    std::vector<void*> ptrs;
+    int64_t bytes = 0;
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        size_t b = std::max<size_t>(count * sizeof(T), 16);
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) fail(C2_ERR_NOMEM, "hipMalloc(" + std::to_string(b) + " bytes): " + hipGetErrorString(e));
+        ptrs.push_back(p);
+        bytes += (int64_t)b;
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>(v.size());
+        if (!v.empty()) HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    ~DeviceArena() {
+        for (void* p : ptrs) hipFree(p);
+    }
+};
+
+static void validate_desc(const c2_scene_desc* d) {
+    if (!d) fail(C2_ERR_INVALID, "null scene description");
+    if (d->num_entities < 0 || d->num_triangles < 0 || d->num_spheres < 0 || d->num_lights < 0 ||
+        d->num_materials < 0)
+        fail(C2_ERR_INVALID, "negative counts in scene description");
+    if (d->num_entities > 0 && (!d->entity_shape_kind || !d->entity_shape_index || !d->entity_material ||
+                                !d->entity_light))
+        fail(C2_ERR_INVALID, "missing entity arrays");
+    for (int e = 0; e < d->num_entities; e++) {
+        int k = d->entity_shape_kind[e], s = d->entity_shape_index[e];
+        if (k == C2_SHAPE_TRIANGLE) {
+            if (s < 0 || s >= d->num_triangles) fail(C2_ERR_INVALID, "entity triangle index out of range");
+        } else if (k == C2_SHAPE_SPHERE) {
+            if (s < 0 || s >= d->num_spheres) fail(C2_ERR_INVALID, "entity sphere index out of range");
+        } else {
+            fail(C2_ERR_INVALID, "unknown entity shape kind");
+        }
+        int m = d->entity_material[e], l = d->entity_light[e];
+        if (m < -1 || m >= d->num_materials) fail(C2_ERR_INVALID, "entity material out of range");
+        if (l < -1 || l >= d->num_lights) fail(C2_ERR_INVALID, "entity light out of range");
+    }
+    for (int l = 0; l < d->num_lights; l++) {
+        const c2_light& L = d->lights[l];
+        if (L.type == C2_LIGHT_AREA && (L.entity < 0 || L.entity >= d->num_entities))
+            fail(C2_ERR_INVALID, "area light entity out of range");
+        if (L.type != C2_LIGHT_AREA && L.type != C2_LIGHT_POINT) fail(C2_ERR_INVALID, "unknown light type");
+    }
+    for (int m = 0; m < d->num_materials; m++) {
+        if (d->materials[m].type == C2_MAT_MICRO && d->materials[m].ndf != C2_NDF_GGX)
+            fail(C2_ERR_UNSUPPORTED, "Micro material: only GGX");
+    }
+    if ((int64_t)d->num_entities >= (1ll << kSlotBits)) fail(C2_ERR_UNSUPPORTED, "too many entities");
+}
+
+// Host copy of the scene in kernel layout + its device mirror.
+struct SceneGPU {
+    DeviceArena arena;
+    DevScene dev{};
+    BuiltBvh bvh;
+    std::vector<float4> prims;
+    std::vector<float> tri_pos, tri_shade, spheres;
+    std::vector<int4> ents;
+    std::vector<DevLight> lights;
+    double build_seconds = 0;
+
+    void build(const c2_scene_desc* d) {
+        validate_desc(d);
+        int nt = d->num_triangles, ns = d->num_spheres, ne = d->num_entities;
+        tri_pos.resize((size_t)nt * 9);
+        tri_shade.resize((size_t)nt * 12);
+        for (int t = 0; t < nt; t++) {
+            const float* v = d->triangles + (size_t)t * 24;
+            for (int k = 0; k < 3; k++) {
+                for (int c = 0; c < 3; c++) {
+                    tri_pos[(size_t)t * 9 + k * 3 + c] = v[k * 8 + c];
+                    tri_shade[(size_t)t * 12 + k * 3 + c] = v[k * 8 + 3 + c];
+                }
+            }
+            // Triangle::calculateDerivative, Triangle.cpp:128-148
+            float a00 = v[6] - v[22], a01 = v[7] - v[23], a10 = v[14] - v[22], a11 = v[15] - v[23];
+            float ood = 1.f / (a00 * a11 - a10 * a01);
+            float i00 = +a11 * ood, i01 = -a01 * ood, i10 = -a10 * ood, i11 = +a00 * ood;
+            V3 du{0, 0, 0};
+            if (!(std::isinf(i00) || std::isinf(i01) || std::isnan(i00) || std::isnan(i01) || std::isnan(i10) ||
+                  std::isnan(i11))) {
+                V3 p0{v[0], v[1], v[2]}, p1{v[8], v[9], v[10]}, p2{v[16], v[17], v[18]};
+                V3 a = p0 - p2, b = p1 - p2;
+                du = normalize(V3{i00 * a.x + i10 * b.x, i00 * a.y + i10 * b.y, i00 * a.z + i10 * b.z});
+            }
+            tri_shade[(size_t)t * 12 + 9] = du.x;
+            tri_shade[(size_t)t * 12 + 10] = du.y;
+            tri_shade[(size_t)t * 12 + 11] = du.z;
+        }
+        spheres.assign(d->spheres, d->spheres + (size_t)ns * 22);
+        std::vector<Box3> boxes(ne);
+        ents.resize(ne);
+        for (int e = 0; e < ne; e++) {
+            int k = d->entity_shape_kind[e], s = d->entity_shape_index[e];
+            ents[e] = make_int4(k, s, d->entity_material[e], d->entity_light[e]);
+            Box3& b = boxes[e];
+            if (k == C2_SHAPE_TRIANGLE) {
+                const float* p = &tri_pos[(size_t)s * 9];
+                for (int c = 0; c < 3; c++) {
+                    b.lo[c] = fmin_ref(fmin_ref(p[c], p[3 + c]), p[6 + c]);
+                    b.hi[c] = fmax_ref(fmax_ref(p[c], p[3 + c]), p[6 + c]);
+                }
+            } else {
+                const float* sp = &spheres[(size_t)s * 22];
+                for (int c = 0; c < 3; c++) {
+                    b.lo[c] = sp[c] - sp[3];
+                    b.hi[c] = sp[c] + sp[3];
+                }
+            }
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        build_bvh(boxes, bvh);
+        build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        prims.resize(bvh.order.size() * 3);
+        for (size_t slot = 0; slot < bvh.order.size(); slot++) {
+            int e = bvh.order[slot];
+            int k = ents[e].x, s = ents[e].y;
+            if (k == C2_SHAPE_TRIANGLE) {
+                const float* p = &tri_pos[(size_t)s * 9];
+                prims[slot * 3 + 0] = make_float4(p[0], p[1], p[2], p[3]);
+                prims[slot * 3 + 1] = make_float4(p[4], p[5], p[6], p[7]);
+                float4 c;
+                c.x = p[8];
+                c.y = __int_as_float_host(e);
+                c.z = __int_as_float_host(C2_SHAPE_TRIANGLE);
+                c.w = 0.f;
+                prims[slot * 3 + 2] = c;
+            } else {
+                const float* sp = &spheres[(size_t)s * 22];
+                prims[slot * 3 + 0] = make_float4(sp[0], sp[1], sp[2], sp[3]);
+                float4 b;
+                b.x = __int_as_float_host(s);
+                b.y = b.z = b.w = 0.f;
+                prims[slot * 3 + 1] = b;
+                float4 c;
+                c.x = 0.f;
+                c.y = __int_as_float_host(e);
+                c.z = __int_as_float_host(C2_SHAPE_SPHERE);
+                c.w = 0.f;
+                prims[slot * 3 + 2] = c;
+            }
+        }
+        lights.resize(d->num_lights);
+        for (int l = 0; l < d->num_lights; l++) {
+            const c2_light& L = d->lights[l];
+            DevLight& D = lights[l];
+            std::memset(&D, 0, sizeof(D));
+            D.type = L.type;
+            D.entity = L.entity;
+            for (int c = 0; c < 3; c++) {
+                D.L[c] = L.radiance[c];
+                D.pos[c] = L.position[c];
+            }
+            if (L.type == C2_LIGHT_AREA) {
+                D.shape_kind = ents[L.entity].x;
+                D.shape = ents[L.entity].y;
+                if (D.shape_kind == C2_SHAPE_TRIANGLE) {  // Triangle::SurfaceArea
+                    const float* p = &tri_pos[(size_t)D.shape * 9];
+                    V3 v0{p[0], p[1], p[2]}, v1{p[3], p[4], p[5]}, v2{p[6], p[7], p[8]};
+                    D.area = length(cross(v1 - v0, v2 - v0)) / 2.f;
+                } else {  // Sphere::SurfaceArea
+                    float r = spheres[(size_t)D.shape * 22 + 3];
+                    D.area = 4 * kPi * r * r;
+                }
+                V3 flux = V3{D.L[0], D.L[1], D.L[2]} * kPi * D.area;
+                D.eligible = !is_zero(flux);
+            } else {
+                D.eligible = !(D.L[0] == 0.f && D.L[1] == 0.f && D.L[2] == 0.f);
+            }
+        }
+        dev.nodes = arena.upload(bvh.nodes);
+        dev.prims = arena.upload(prims);
+        dev.entities = arena.upload(ents);
+        dev.tri_pos = arena.upload(tri_pos);
+        dev.tri_shade = arena.upload(tri_shade);
+        dev.spheres = arena.upload(spheres);
+        std::vector<c2_material> mats(d->materials, d->materials + d->num_materials);
+        dev.materials = arena.upload(mats);
+        dev.lights = arena.upload(lights);
+        dev.perm = arena.upload(halton_perm_data());
+        dev.perm_off = arena.upload(halton_perm_offsets());
+        dev.num_lights = d->num_lights;
+        dev.num_entities = ne;
+    }
+
+    static float __int_as_float_host(int32_t i) {
+        float f;
+        std::memcpy(&f, &i, 4);
+        return f;
+    }
+
+    // Algorithmic traversal counts on the binary SAH tree (32-B nodes, leaf <= 4):
+    // node box tests and primitive tests per ray, same semantics as the kernels.
+    void count_visits(const float* rays, int64_t n, bool any, double& nodes, double& tris) const {
+        int64_t cn = 0, ct = 0;
+        for (int64_t i = 0; i < n; i++) {
+            const float* r = rays + i * 8;
+            V3 o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
+            RayPre rp = make_ray(o, d);
+            float tmin = any ? r[6] : 0.f, tmax = r[7];
+            float best = INFINITY;
+            int stack[128], sp = 0, cur = 0;
+            bool done = false;
+            while (!done) {
+                const Bvh2Node& nd = bvh.bvh2[cur];
+                cn++;
+                float t0 = tmin, t1 = any ? tmax : fmin_ref(tmax, best);
+                bool hitBox = slab(rp, nd.box.lo[0], nd.box.lo[1], nd.box.lo[2], nd.box.hi[0], nd.box.hi[1],
+                                   nd.box.hi[2], t0, t1);
+                if (hitBox && nd.leaf()) {
+                    for (int k = 0; k < -nd.right; k++) {
+                        int e = bvh.order[nd.left + k];
+                        ct++;
+                        int kind = ents[e].x, s = ents[e].y;
+                        if (kind != C2_SHAPE_TRIANGLE) continue;
+                        const float* p = &tri_pos[(size_t)s * 9];
+                        V3 v0{p[0], p[1], p[2]}, v1{p[3], p[4], p[5]}, v2{p[6], p[7], p[8]};
+                        if (any) {
+                            if (tri_test(o, d, v0, v1, v2, tmin, tmax)) { done = true; break; }
+                        } else {
+                            V3 res = tri_solve(o, d, v0, v1, v2);
+                            if (tri_accept_closest(res) && res.z < best) best = res.z;
+                        }
+                    }
+                } else if (hitBox) {
+                    int a = nd.left, b = nd.right;
+                    int axis = 0;
+                    float ex[3];
+                    for (int c = 0; c < 3; c++) ex[c] = nd.box.hi[c] - nd.box.lo[c];
+                    axis = (ex[0] >= ex[1] && ex[0] >= ex[2]) ? 0 : (ex[1] >= ex[2] ? 1 : 2);
+                    bool neg = axis == 0 ? rp.neg[0] : (axis == 1 ? rp.neg[1] : rp.neg[2]);
+                    if (neg) std::swap(a, b);
+                    stack[sp++] = b;
+                    cur = a;
+                    continue;
+                }
+                if (done || sp == 0) break;
+                cur = stack[--sp];
+            }
+        }
+        nodes = n ? (double)cn / n : 0;
+        tris = n ? (double)ct / n : 0;
+    }
+};
+
+}  // namespace c2
+
+using namespace c2;
+
+struct c2_accel {
+    SceneGPU scene;
+    c2_accel_stats stats{};
+};
+
+static int grid_for(int64_t n) {
+    int64_t g = (n + kBlock - 1) / kBlock;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65535));
+}
+
+extern "C" {
+
+int c2_accel_build(const c2_scene_desc* desc, c2_accel** out) {
+    return guard([&] {
+        if (!out) fail(C2_ERR_INVALID, "null output");
+        std::unique_ptr<c2_accel> a(new c2_accel());
+        a->scene.build(desc);
+        a->stats.num_nodes_bvh2 = (int64_t)a->scene.bvh.bvh2.size();
+        a->stats.num_nodes_bvh4 = (int64_t)a->scene.bvh.nodes.size();
+        a->stats.max_depth_bvh4 = a->scene.bvh.depth4;
+        a->stats.leaf_size = kLeafSize;
+        a->stats.build_seconds = a->scene.build_seconds;
+        a->stats.device_bytes = a->scene.arena.bytes;
+        *out = a.release();
+    });
+}
+
+void c2_accel_destroy(c2_accel* a) { delete a; }
+
+int c2_accel_get_stats(const c2_accel* a, c2_accel_stats* out) {
+    return guard([&] {
+        if (!a || !out) fail(C2_ERR_INVALID, "null argument");
+        *out = a->stats;
+    });
+}
+
+int c2_accel_intersect_device(c2_accel* a, const float* rays, int64_t n, int32_t* prim, float* t, void* stream) {
+    return guard([&] {
+        if (!a || n < 0 || (n > 0 && (!rays || !prim || !t))) fail(C2_ERR_INVALID, "bad arguments");
+        if (n == 0 || a->scene.ents.empty()) {
+            if (n > 0) {
+                HIP_CHECK(hipMemsetAsync(prim, 0xff, n * sizeof(int32_t), (hipStream_t)stream));
+            }
+            return;
+        }
+        hipLaunchKernelGGL(k_intersect, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, a->scene.dev, rays,
+                           n, prim, t);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int c2_accel_intersect_test_device(c2_accel* a, const float* rays, const int32_t* ignore, int64_t n, uint8_t* hit,
+                                   void* stream) {
+    return guard([&] {
+        if (!a || n < 0 || (n > 0 && (!rays || !hit))) fail(C2_ERR_INVALID, "bad arguments");
+        if (n == 0) return;
+        if (a->scene.ents.empty()) {
+            HIP_CHECK(hipMemsetAsync(hit, 0, n, (hipStream_t)stream));
+            return;
+        }
+        hipLaunchKernelGGL(k_intersect_test, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, a->scene.dev,
+                           rays, ignore, n, hit);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int c2_accel_intersect(c2_accel* a, const float* rays, int64_t n, int32_t* prim, float* t) {
+    return guard([&] {
+        if (!a || n < 0 || (n > 0 && (!rays || !prim || !t))) fail(C2_ERR_INVALID, "bad arguments");
+        if (n == 0) return;
+        DeviceArena tmp;
+        float* dr = tmp.alloc<float>((size_t)n * 8);
+        int32_t* dp = tmp.alloc<int32_t>(n);
+        float* dt = tmp.alloc<float>(n);
+        HIP_CHECK(hipMemcpy(dr, rays, (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice));
+        int rc = c2_accel_intersect_device(a, dr, n, dp, dt, nullptr);
+        if (rc != C2_OK) fail(rc, c2_last_error());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(prim, dp, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(t, dt, n * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int c2_accel_intersect_test(c2_accel* a, const float* rays, const int32_t* ignore, int64_t n, uint8_t* hit) {
+    return guard([&] {
+        if (!a || n < 0 || (n > 0 && (!rays || !hit))) fail(C2_ERR_INVALID, "bad arguments");
+        if (n == 0) return;
+        DeviceArena tmp;
+        float* dr = tmp.alloc<float>((size_t)n * 8);
+        int32_t* di = ignore ? tmp.alloc<int32_t>(n) : nullptr;
+        uint8_t* dh = tmp.alloc<uint8_t>(n);
+        HIP_CHECK(hipMemcpy(dr, rays, (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice));
+        if (ignore) HIP_CHECK(hipMemcpy(di, ignore, n * sizeof(int32_t), hipMemcpyHostToDevice));
+        int rc = c2_accel_intersect_test_device(a, dr, di, n, dh, nullptr);
+        if (rc != C2_OK) fail(rc, c2_last_error());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(hit, dh, n, hipMemcpyDeviceToHost));
+    });
+}
+
+int c2_accel_count_visits(const c2_accel* a, const float* rays, int64_t n, int32_t any_hit, double* mean_nodes,
+                          double* mean_tris) {
+    return guard([&] {
+        if (!a || n < 0 || (n > 0 && !rays) || !mean_nodes || !mean_tris) fail(C2_ERR_INVALID, "bad arguments");
+        if (a->scene.ents.empty()) {
+            *mean_nodes = *mean_tris = 0;
+            return;
+        }
+        a->scene.count_visits(rays, n, any_hit != 0, *mean_nodes, *mean_tris);
+    });
+}
+
+}  // extern "C"
+
+// ============================================================================ path tracer handle
+
+struct c2_pt {
+    std::unique_ptr<c2_accel> accel;
+    c2_render_opts opts{};
+    PassParams pp{};
+    DeviceArena arena;
+    PathBuffers B{};
+    int32_t chunk = 0;
+    float4* film = nullptr;
+    float4* comp = nullptr;
+    double4* film64 = nullptr;
+    hipStream_t stream = nullptr;
+    int64_t launches = 0;
+    int64_t samples = 0;
+};
+
+extern "C" {
+
+int c2_pt_create(const c2_scene_desc* desc, const c2_render_opts* opts, c2_pt** out) {
+    return guard([&] {
+        if (!opts || !out) fail(C2_ERR_INVALID, "null argument");
+        if (opts->width <= 0 || opts->height <= 0 || opts->spp <= 0 || opts->max_depth < 0)
+            fail(C2_ERR_INVALID, "render options: non-positive size");
+        if (opts->arg_order != C2_ARGORDER_RTL && opts->arg_order != C2_ARGORDER_LTR)
+            fail(C2_ERR_INVALID, "render options: bad arg_order");
+        if (opts->film_mode < C2_FILM_F32 || opts->film_mode > C2_FILM_F64)
+            fail(C2_ERR_INVALID, "render options: bad film_mode");
+        std::unique_ptr<c2_pt> pt(new c2_pt());
+        pt->opts = *opts;
+        c2_accel* a = nullptr;
+        int rc = c2_accel_build(desc, &a);
+        if (rc != C2_OK) fail(rc, c2_last_error());
+        pt->accel.reset(a);
+        // camera (PerspCamera ctor: tan evaluated on the host with the float libm call)
+        const c2_camera& c = desc->camera;
+        PassParams& P = pt->pp;
+        P.width = opts->width;
+        P.height = opts->height;
+        P.max_depth = opts->max_depth;
+        P.arg_order = opts->arg_order;
+        float aspect = (float)opts->width / opts->height;
+        P.camH = 1.0f * std::tan(c.fov / 2);
+        P.camW = P.camH * aspect;
+        P.eye = V3{c.position[0], c.position[1], c.position[2]};
+        V3 at{c.look_at[0], c.look_at[1], c.look_at[2]}, up0{c.up[0], c.up[1], c.up[2]};
+        P.look = normalize(at - P.eye);
+        V3 pr = dot(up0, P.look) * P.look;
+        P.up = normalize(up0 - pr);
+        P.right = normalize(cross(P.look, up0));
+        HaltonParams hp = halton_params(opts->width, opts->height);
+        P.stride0 = hp.stride;
+        P.exp0 = hp.base_exponents[0];
+        P.bexp1 = hp.base_exponents[1];
+        P.scale1 = hp.base_scales[1];
+        P.bscale0 = hp.base_scales[0];
+        P.mult0 = hp.mult_inverse[0];
+        P.mult1 = hp.mult_inverse[1];
+        int lmax = 0;
+        for (const DevLight& L : pt->accel->scene.lights) lmax += L.eligible ? 1 : 0;
+        P.lmax = std::max(lmax, 1);
+        int64_t npix = (int64_t)opts->width * opts->height;
+        int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : npix;
+        // keep the per-path shadow slots within ~6 GiB
+        int64_t perPath = (int64_t)P.lmax * (3 * 16 + 1 + 4) + 16 * 14 + 12;
+        int64_t maxChunk = std::max<int64_t>(4096, ((int64_t)6 << 30) / perPath);
+        chunk = std::min(std::min(chunk, npix), maxChunk);
+        pt->chunk = (int32_t)chunk;
+        PathBuffers& B = pt->B;
+        DeviceArena& A = pt->arena;
+        size_t ch = (size_t)chunk, slots = (size_t)chunk * P.lmax;
+        B.ray_o = A.alloc<float4>(ch);
+        B.ray_d = A.alloc<float4>(ch);
+        B.beta = A.alloc<float4>(ch);
+        B.beta2 = A.alloc<float4>(ch);
+        B.L = A.alloc<float4>(ch);
+        B.meta = A.alloc<int4>(ch);
+        B.hit = A.alloc<float4>(ch);
+        B.nee = A.alloc<int4>(ch);
+        B.mis_p = A.alloc<float4>(ch);
+        B.mis_n = A.alloc<float4>(ch);
+        B.mis_w = A.alloc<float4>(ch);
+        B.mis_f = A.alloc<float4>(ch);
+        B.mis_L = A.alloc<float4>(ch);
+        B.sh_o = A.alloc<float4>(slots);
+        B.sh_d = A.alloc<float4>(slots);
+        B.sh_c = A.alloc<float4>(slots);
+        B.sh_vis = A.alloc<uint8_t>(slots);
+        B.queue[0] = A.alloc<int32_t>(ch);
+        B.queue[1] = A.alloc<int32_t>(ch);
+        B.sh_list = A.alloc<int32_t>(slots);
+        B.mis_list = A.alloc<int32_t>(ch);
+        B.counters = A.alloc<int32_t>(8);
+        B.totals = A.alloc<unsigned long long>(4);
+        HIP_CHECK(hipMemset(B.counters, 0, 8 * sizeof(int32_t)));
+        HIP_CHECK(hipMemset(B.totals, 0, 4 * sizeof(unsigned long long)));
+        if (opts->film_mode == C2_FILM_F64) {
+            pt->film64 = A.alloc<double4>(npix);
+        } else {
+            pt->film = A.alloc<float4>(npix);
+            if (opts->film_mode == C2_FILM_F32_KAHAN) pt->comp = A.alloc<float4>(npix);
+        }
+        HIP_CHECK(hipStreamCreateWithFlags(&pt->stream, hipStreamNonBlocking));
+        *out = pt.release();
+        int rc2 = c2_pt_reset(*out);
+        if (rc2 != C2_OK) fail(rc2, c2_last_error());
+    });
+}
+
+void c2_pt_destroy(c2_pt* pt) {
+    if (!pt) return;
+    if (pt->stream) {
+        hipStreamSynchronize(pt->stream);
+        hipStreamDestroy(pt->stream);
+    }
+    delete pt;
+}
+
+int c2_pt_reset(c2_pt* pt) {
+    return guard([&] {
+        if (!pt) fail(C2_ERR_INVALID, "null renderer");
+        int64_t npix = (int64_t)pt->opts.width * pt->opts.height;
+        if (pt->film64) {
+            hipLaunchKernelGGL(k_zero_d4, dim3(grid_for(npix)), dim3(kBlock), 0, pt->stream, pt->film64, npix);
+        } else {
+            hipLaunchKernelGGL(k_zero4, dim3(grid_for(npix)), dim3(kBlock), 0, pt->stream, pt->film, npix);
+            if (pt->comp) hipLaunchKernelGGL(k_zero4, dim3(grid_for(npix)), dim3(kBlock), 0, pt->stream, pt->comp, npix);
+        }
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(pt->stream));
+        pt->samples = 0;
+    });
+}
+
+int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* stream) {
+    return guard([&] {
+        if (!pt) fail(C2_ERR_INVALID, "null renderer");
+        if (sample_begin < 0 || sample_count < 0) fail(C2_ERR_INVALID, "negative sample range");
+        hipStream_t s = stream ? (hipStream_t)stream : pt->stream;
+        PassParams P = pt->pp;
+        DevScene S = pt->accel->scene.dev;
+        int64_t npix = (int64_t)P.width * P.height;
+        for (int k = sample_begin; k < sample_begin + sample_count; k++) {
+            P.sample = k;
+            for (int64_t beg = 0; beg < npix; beg += pt->chunk) {
+                P.pix_begin = (int32_t)beg;
+                P.pix_count = (int32_t)std::min<int64_t>(pt->chunk, npix - beg);
+                int g = grid_for(P.pix_count);
+                hipLaunchKernelGGL(k_generate, dim3(g), dim3(kBlock), 0, s, S, pt->B, P);
+                pt->launches++;
+                int q = 0;
+                if (S.num_entities > 0) {
+                    for (int b = 0; b < P.max_depth; b++) {
+                        hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, pt->B, q ^ 1);
+                        hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBlock), 0, s, S, pt->B, q);
+                        hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q, b);
+                        int gs = grid_for((int64_t)P.pix_count * P.lmax);
+                        hipLaunchKernelGGL(k_occlude, dim3(gs), dim3(kBlock), 0, s, S, pt->B);
+                        hipLaunchKernelGGL(k_mis, dim3(g), dim3(kBlock), 0, s, S, pt->B);
+                        hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kBlock), 0, s, pt->B, P, q);
+                        pt->launches += 6;
+                        q ^= 1;
+                    }
+                }
+                hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, s, pt->B, P, pt->opts.film_mode, pt->film,
+                                   pt->comp, pt->film64);
+                pt->launches++;
+                HIP_CHECK(hipGetLastError());
+            }
+            pt->samples += npix;
+        }
+    });
+}
+
+int c2_pt_sync(c2_pt* pt) {
+    return guard([&] {
+        if (!pt) fail(C2_ERR_INVALID, "null renderer");
+        HIP_CHECK(hipStreamSynchronize(pt->stream));
+        HIP_CHECK(hipDeviceSynchronize());
+    });
+}
+
+int c2_pt_get_film(c2_pt* pt, float* rgb) {
+    return guard([&] {
+        if (!pt || !rgb) fail(C2_ERR_INVALID, "null argument");
+        HIP_CHECK(hipDeviceSynchronize());
+        int W = pt->opts.width, H = pt->opts.height;
+        int64_t npix = (int64_t)W * H;
+        if (pt->film64) {
+            std::vector<double4> f(npix);
+            HIP_CHECK(hipMemcpy(f.data(), pt->film64, npix * sizeof(double4), hipMemcpyDeviceToHost));
+            for (int y = 0; y < H; y++)
+                for (int x = 0; x < W; x++) {
+                    const double4& c = f[(size_t)y * W + x];
+                    float* o = rgb + ((size_t)(H - y - 1) * W + x) * 3;
+                    o[0] = (float)(c.x / c.w);
+                    o[1] = (float)(c.y / c.w);
+                    o[2] = (float)(c.z / c.w);
+                }
+        } else {
+            std::vector<float4> f(npix);
+            HIP_CHECK(hipMemcpy(f.data(), pt->film, npix * sizeof(float4), hipMemcpyDeviceToHost));
+            for (int y = 0; y < H; y++)
+                for (int x = 0; x < W; x++) {
+                    const float4& c = f[(size_t)y * W + x];
+                    V3 v = V3{c.x, c.y, c.z} / c.w;  // GetImageDataRGB32F: color / weight, rows flipped
+                    float* o = rgb + ((size_t)(H - y - 1) * W + x) * 3;
+                    o[0] = v.x;
+                    o[1] = v.y;
+                    o[2] = v.z;
+                }
+        }
+    });
+}
+
+int c2_pt_get_stats(const c2_pt* pt, c2_pt_stats* out) {
+    return guard([&] {
+        if (!pt || !out) fail(C2_ERR_INVALID, "null argument");
+        unsigned long long t[4] = {0, 0, 0, 0};
+        HIP_CHECK(hipMemcpy(t, pt->B.totals, sizeof(t), hipMemcpyDeviceToHost));
+        out->samples = pt->samples;
+        out->extension_rays = (int64_t)t[0];
+        out->shadow_rays = (int64_t)t[1];
+        out->mis_rays = (int64_t)t[2];
+        out->kernel_launches = pt->launches;
+    });
+}
+
+}  // extern "C"
