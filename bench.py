@@ -1,0 +1,197 @@
+#!/usr/bin/env python
+"""Headline benchmark: plain GPU PathTracer + MIS on the ~2M-triangle sphere-field scene.
+
+Workload: ground quad + 400 sphere meshes (5,120 triangles each) + 4 area-light
+quads (8 light triangles), materials drawn from diffuse / GGX / anisotropic GGX /
+glass / rough glass, 1920x1080, Halton, depth 8 (64 spp = 64 steps for the full
+frame). One step = one Halton sample pass over all 2,073,600 pixels
+(generate -> 8 x (extend, shade, occlude, mis, resolve) -> accumulate), with the
+scene, BVH and film resident in HBM.
+
+Prints ONE JSON line (metric/value in Msamples/s) with:
+  roofline      dominant kernel: algorithmic bytes per launch / average launch
+                time (HIP events on the renderer's stream), vs the 8 TB/s HBM peak;
+                `traffic` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary.
+  cpu_baseline  the CPU oracle (C++ restatement of the reference path) on a
+                bounded sample of the same workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "sphere-field path tracing (400 spheres x 5120 tris + ground + 4 area quads), 1920x1080, depth 8"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_scene(n_spheres=400, level=4, seed=2024):
+    import crystal2_amd as c2
+    from crystal2_amd import scenes
+    d = scenes.spheres_scene_objects(n_spheres, level, seed)
+    sc = c2.Scene.create()
+    for (p, n, f, mat, le) in d["meshes"]:
+        if mat is None:
+            m = -1
+        else:
+            mt = {"Diffuse": 0, "Mirror": 1, "Glass": 2, "Micro": 3, "Phong": 4}[mat["Type"]]
+            m = sc.add_material(mt, mat["Color"], mat.get("Roughness", (0, 0)), mat.get("IOR", 1.0), mat.get("K", 0))
+        sc.add_mesh(p, n, f, m, lemit=le)
+    cam = d["camera"]
+    sc.set_camera(cam["Position"], cam["LookAt"], cam["Up"], cam["FOV"])
+    return sc
+
+
+def per_ray_bytes(nodes, tris):
+    # 24 B ray in + 8 B hit out + 32 B per binary-BVH node visit + 36 B per triangle test
+    return 24 + 8 + nodes * 32 + tris * 36
+
+
+def load_traffic(kernel):
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    return t.get(kernel)
+
+
+def cpu_baseline(scene, width, height, depth, seconds_budget=20.0):
+    """Oracle (C++ port of the reference path) on a band of rows at sample 0."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import pyoracle
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.time()
+    orc = pyoracle.OracleScene(scene, variant="portable")
+    log(f"cpu baseline: oracle scene built in {time.time() - t0:.1f}s, {threads} threads")
+    # calibrate on 8 rows, then size the band to about seconds_budget
+    mid = height // 2
+    _, s = orc.render(width, height, 1, depth, threads=threads, y_range=(mid - 4, mid + 4))
+    rate = 8 * width / max(s, 1e-6)
+    rows = int(max(8, min(height, seconds_budget * rate / width)))
+    y0 = max(0, mid - rows // 2)
+    _, s = orc.render(width, height, 1, depth, threads=threads, y_range=(y0, y0 + rows))
+    samples = rows * width
+    return {"value": samples / s / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"rows {y0}..{y0 + rows} of {width}x{height}, Halton sample 0 ({samples} samples, {s:.1f}s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--spheres", type=int, default=400)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(the path shards by Halton sample range, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import crystal2_amd as c2
+    W, H, K = args.width, args.height, args.steps
+    t0 = time.time()
+    scene = build_scene(args.spheres)
+    log(f"scene: {scene.num_entities} entities built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    pt = c2.WavefrontPathIntegrator(scene, W, H, spp=max(64, args.warmup + K), max_depth=args.depth,
+                                    arg_order="rtl", film="f32")
+    log(f"renderer + BVH ready in {time.time() - t0:.1f}s")
+
+    # algorithmic bytes per ray from the rays the renderer actually traces (sample 0)
+    bvh_for_counts = c2.GpuBVH(scene)
+    counts = {}
+    for kind, bounce in (("extension", 1), ("shadow", 0)):
+        rays, _ = pt.capture_rays(0, bounce, "shadow" if kind == "shadow" else "extension", n=1 << 14)
+        nodes, tris = bvh_for_counts.count_visits(rays, any_hit=(kind == "shadow"))
+        counts[kind] = (nodes, tris, per_ray_bytes(nodes, tris))
+        log(f"{kind} rays: {nodes:.1f} node visits, {tris:.1f} tri tests -> {counts[kind][2]:.0f} B/ray")
+    bvh_for_counts.close()
+    pt.reset()
+
+    if args.warmup:
+        pt.render(0, args.warmup)
+        pt.sync()
+    pt.reset()
+    st0 = pt.stats()
+    pt.set_profiling(True)
+    pt.sync()
+    t0 = time.perf_counter()
+    pt.render(0, K)
+    pt.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kt = pt.kernel_times()
+    st = pt.stats()
+    samples = W * H * K
+    value = samples / elapsed / 1e6
+    ext = st["extension_rays"] - st0["extension_rays"]
+    sh = st["shadow_rays"] - st0["shadow_rays"]
+    mis = st["mis_rays"] - st0["mis_rays"]
+    log(f"timed {K} passes in {elapsed:.3f}s: {value:.2f} Msamples/s; rays/sample ext {ext / samples:.2f} "
+        f"shadow {sh / samples:.2f} mis {mis / samples:.3f}")
+    for name, v in kt.items():
+        if v["launches"]:
+            log(f"  {name:10s} {v['ms']:9.1f} ms  {v['launches']:6d} launches  {v['ms'] / v['launches']:.3f} ms avg")
+
+    dom = max(("extend", "occlude"), key=lambda k: kt[k]["ms"])
+    rays_dom = ext if dom == "extend" else sh
+    bpr = counts["extension" if dom == "extend" else "shadow"][2]
+    avg_s = kt[dom]["ms"] / 1e3 / max(1, kt[dom]["launches"])
+    bytes_per_launch = rays_dom * bpr / max(1, kt[dom]["launches"])
+    achieved = bytes_per_launch / avg_s / 1e9
+    kname = {"extend": "k_extend", "occlude": "k_occlude"}[dom]
+    traffic = load_traffic(kname)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
+                "bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(avg_s * 1e3, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(scene, W, H, args.depth, args.cpu_seconds)
+            log(f"cpu baseline: {cpu}")
+        except Exception as e:  # the GPU figure stands alone if the checker is absent
+            log(f"cpu baseline unavailable: {e}")
+
+    out = {
+        "metric": "Msamples/s (plain GPU PT+MIS)",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / K * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": WORKLOAD, "width": W, "height": H, "depth": args.depth,
+                   "triangles": scene.num_entities, "samples_per_step": W * H,
+                   "rays_per_sample": {"extension": round(ext / samples, 3), "shadow": round(sh / samples, 3),
+                                       "mis": round(mis / samples, 4)}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -250,6 +250,22 @@ class WavefrontPathIntegrator:
         check(self._lib.c2_pt_get_film(self._h, _fp(out)))
         return out
 
+    def set_profiling(self, enable: bool = True):
+        check(self._lib.c2_pt_set_profiling(self._h, int(enable)))
+
+    def kernel_times(self) -> dict:
+        k = _abi.c2_kernel_times()
+        check(self._lib.c2_pt_get_kernel_times(self._h, C.byref(k)))
+        return {name: {"ms": k.ms[i], "launches": k.launches[i]} for i, name in enumerate(_abi.KERNEL_CLASSES)}
+
+    def capture_rays(self, sample: int, bounce: int, kind: str = "shadow", n: int = 1 << 16):
+        rays = np.zeros((n, 8), np.float32)
+        ign = np.zeros(n, np.int32)
+        w = C.c_int64()
+        check(self._lib.c2_pt_capture_rays(self._h, sample, bounce, 1 if kind == "shadow" else 0, _fp(rays), _ip(ign),
+                                           n, C.byref(w)))
+        return rays[:w.value], ign[:w.value]
+
     def stats(self) -> dict:
         s = _abi.c2_pt_stats()
         check(self._lib.c2_pt_get_stats(self._h, C.byref(s)))

@@ -67,6 +67,12 @@ class c2_pt_stats(C.Structure):
                 ("mis_rays", C.c_int64), ("kernel_launches", C.c_int64)]
 
 
+class c2_kernel_times(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_int64 * 8)]
+
+
+KERNEL_CLASSES = ["generate", "extend", "shade", "occlude", "mis", "resolve", "accumulate", "nee"]
+
 P = C.c_void_p
 _i32p = C.POINTER(C.c_int32)
 _f32p = C.POINTER(C.c_float)
@@ -100,6 +106,10 @@ SIGNATURES = [
     ("c2_pt_get_film", C.c_int, [P, _f32p]),
     ("c2_pt_get_stats", C.c_int, [P, C.POINTER(c2_pt_stats)]),
     ("c2_pt_sync", C.c_int, [P]),
+    ("c2_pt_set_profiling", C.c_int, [P, C.c_int32]),
+    ("c2_pt_get_kernel_times", C.c_int, [P, C.POINTER(c2_kernel_times)]),
+    ("c2_pt_capture_rays", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, _f32p, _i32p, C.c_int64,
+                                     C.POINTER(C.c_int64)]),
     ("c2_halton_index", C.c_int, [C.c_int32] * 5 + [_i32p]),
     ("c2_halton_value", C.c_int, [C.c_int32] * 4 + [_f32p]),
 ]

@@ -90,16 +90,21 @@ struct PathBuffers {
     float4* L;       // path radiance
     int4* meta;      // pixel, halton index, dimension, flags
     float4* hit;     // t, u, v, entity bits
-    int4* nee;       // shadow slot count, own light, MIS hit light, -
-    float4* mis_p;   // surface p.xyz, pdf_bsdf
-    float4* mis_n;   // facing normal
-    float4* mis_w;   // BSDF-sampled direction
-    float4* mis_f;   // f * cos
+    float4* vtx_p;   // vertex position, uL.x
+    float4* vtx_n;   // facing shading normal, uL.y
+    float4* vtx_du;  // dpdu, (BSDF-sampled f != 0) bits
+    float4* vtx_d;   // incoming ray direction, pdf of the BSDF-sampled direction
+    int4* vtx_i;     // entity, material, attached light, front face
+    float4* mis_w;   // BSDF-sampled direction (shared by all lights)
+    float4* mis_f;   // f * cos for that direction
     float4* mis_L;   // resolved MIS term
+    int32_t* misflag;  // 1 = some light has Pdf_Li != 0 -> MIS ray traced
+    int32_t* mis_hit;  // light owning the MIS hit, -1 none
+    const int32_t* elig;  // eligible lights (Flux != 0) in light order
     float4* sh_o;    // shadow ray origin, ignore entity bits
-    float4* sh_d;    // shadow ray direction, light index bits
+    float4* sh_d;    // shadow ray direction
     float4* sh_c;    // unshadowed contribution
-    uint8_t* sh_vis; // 1 = unoccluded
+    uint8_t* sh_vis; // 0 = none / occluded, 2 = pending, 1 = unoccluded
     int32_t* queue[2];
     int32_t* sh_list;
     int32_t* mis_list;
@@ -113,7 +118,9 @@ struct PassParams {
     int32_t pix_begin, pix_count;
     int32_t max_depth;
     int32_t arg_order;
-    int32_t lmax;            // shadow slots per path
+    int32_t lmax;            // shadow slots per path (>= 1)
+    int32_t num_elig;        // eligible lights (Flux != 0)
+    int32_t nd_elig;         // eligible non-delta lights (one Get1D each per vertex)
     int32_t stride0, exp0, scale1, mult0, mult1, bscale0, bexp1;  // Halton constants
     V3 eye, look, up, right;
     float camW, camH;
@@ -192,6 +199,8 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, in
     }
 }
 
+// Vertex stage: emission, BSDF, the BSDF-sampled MIS direction (shared by all
+// lights), continuation sample and Russian roulette; the light loop runs in k_nee.
 __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, PassParams P, int q, int bounce) {
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
@@ -203,173 +212,191 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + (int)threadIdx.x;
         bool live = i < n;
-        int p = live ? queue[i] : 0;
-        int4 meta = live ? B.meta[p] : make_int4(0, 0, 0, 0);
-        int flags = meta.w & F_LIGHTPATH;
-        float4 hh = live ? B.hit[p] : make_float4(0, 0, 0, 0);
-        int ent = __float_as_int(hh.w);
-        bool active = live && ent >= 0;
         bool cont = false;
-        int nsh = 0, own = -1;
-        bool misNeeded = false;
-        V3 misF{0, 0, 0}, misW{0, 0, 0};
-        float misPdf = 0.f;
-        Inter it;
-        float4 ro = live ? B.ray_o[p] : make_float4(0, 0, 0, 0);
-        float4 rd = live ? B.ray_d[p] : make_float4(0, 0, 0, 0);
-        V3 o{ro.x, ro.y, ro.z}, d{rd.x, rd.y, rd.z};
-        float4 b4 = live ? B.beta[p] : make_float4(0, 0, 0, 0);
-        V3 beta{b4.x, b4.y, b4.z};
-        V3 beta2 = beta;
-        int32_t idx = meta.y, dim = meta.z;
-        if (active) {
-            Hit h{ent, hh.x, hh.y, hh.z};
-            it = make_inter(S, o, d, h);
-            own = it.light;
-            // emission (lightPath), :54-57
-            if (meta.w & F_LIGHTPATH) {
-                V3 Le{0, 0, 0};
-                if (it.light >= 0) {
-                    const DevLight& Ld = S.lights[it.light];
-                    V3 nact = it.front ? it.n : -it.n;
-                    Le = dot(nact, -d) < 0 ? V3{0, 0, 0} : V3{Ld.L[0], Ld.L[1], Ld.L[2]};
+        int p = live ? queue[i] : 0;
+        if (live) {
+            int4 meta = B.meta[p];
+            float4 hh = B.hit[p];
+            int ent = __float_as_int(hh.w);
+            int flags = 0;
+            int32_t idx = meta.y, dim = meta.z;
+            float4 b4 = B.beta[p];
+            V3 beta{b4.x, b4.y, b4.z};
+            V3 beta2 = beta;
+            if (ent >= 0) {
+                float4 ro = B.ray_o[p], rd = B.ray_d[p];
+                V3 o{ro.x, ro.y, ro.z}, d{rd.x, rd.y, rd.z};
+                Hit h{ent, hh.x, hh.y, hh.z};
+                Inter it = make_inter(S, o, d, h);
+                // emission when the previous bounce was specular / transmissive, :54-57
+                if (meta.w & F_LIGHTPATH) {
+                    V3 Le{0, 0, 0};
+                    if (it.light >= 0) {
+                        const DevLight& Ld = S.lights[it.light];
+                        V3 nact = it.front ? it.n : -it.n;
+                        Le = dot(nact, -d) < 0 ? V3{0, 0, 0} : V3{Ld.L[0], Ld.L[1], Ld.L[2]};
+                    }
+                    float4 L4 = B.L[p];
+                    V3 L = V3{L4.x, L4.y, L4.z} + beta * Le;
+                    B.L[p] = make_float4(L.x, L.y, L.z, 0.f);
                 }
-                float4 L4 = B.L[p];
-                V3 L = V3{L4.x, L4.y, L4.z} + beta * Le;
-                B.L[p] = make_float4(L.x, L.y, L.z, 0.f);
+                if (it.material < 0) {
+                    // transparent: SpawnRay(dir), no dimensions consumed, :63-68
+                    V3 no = spawn_origin(it.p, it.n, d);
+                    B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
+                    cont = true;
+                    flags = meta.w & F_LIGHTPATH;
+                } else {
+                    Bsdf bs = make_bsdf(S, it, (uint32_t)idx, (uint32_t)(dim + 2));
+                    V3 wo = -d;
+                    // UniformSampleAllLights: uL = Get2D(), uB = Get2D(), :179-180
+                    float uL0 = halton_value(S, hd, idx, dim), uL1 = halton_value(S, hd, idx, dim + 1);
+                    float uB0 = halton_value(S, hd, idx, dim + 2), uB1 = halton_value(S, hd, idx, dim + 3);
+                    int own = it.light;
+                    int ownElig = (own >= 0 && S.lights[own].eligible) ? 1 : 0;
+                    int nGet1D = P.nd_elig - ownElig;  // one Get1D per eligible non-delta light, :240
+                    V3 misF{0, 0, 0}, wiB{0, 0, 0};
+                    float pdfB = 0.f;
+                    int fBnz = 0;
+                    if (nGet1D > 0) {
+                        int typeB = 0;
+                        V3 fB = bsdf_sample(bs, it, wo, uB0, uB1, T_ALL & ~T_SPEC, wiB, pdfB, typeB);
+                        if (!is_zero(fB)) {
+                            fBnz = 1;
+                            misF = fB * fmax_ref(0.f, dot(it.n, wiB));
+                        }
+                    }
+                    B.vtx_p[p] = make_float4(it.p.x, it.p.y, it.p.z, uL0);
+                    B.vtx_n[p] = make_float4(it.n.x, it.n.y, it.n.z, uL1);
+                    B.vtx_du[p] = make_float4(it.dpdu.x, it.dpdu.y, it.dpdu.z, __int_as_float(fBnz));
+                    B.vtx_d[p] = make_float4(d.x, d.y, d.z, pdfB);
+                    B.vtx_i[p] = make_int4(ent, it.material, own, it.front ? 1 : 0);
+                    B.mis_w[p] = make_float4(wiB.x, wiB.y, wiB.z, 0.f);
+                    B.mis_f[p] = make_float4(misF.x, misF.y, misF.z, 0.f);
+                    B.misflag[p] = 0;
+                    B.mis_hit[p] = -1;
+                    dim += 4 + nGet1D;
+                    // continuation: sampler->Get1D() and sampler->Get2D() as arguments of one call, :79
+                    float u2x, u2y;
+                    uint32_t sdim;
+                    if (P.arg_order == C2_ARGORDER_RTL) {
+                        sdim = (uint32_t)dim;
+                        u2x = halton_value(S, hd, idx, dim);
+                        u2y = halton_value(S, hd, idx, dim + 1);
+                    } else {
+                        sdim = (uint32_t)(dim + 1);
+                        u2x = halton_value(S, hd, idx, dim + 1);
+                        u2y = halton_value(S, hd, idx, dim + 2);
+                    }
+                    // (the Get1D value only selects the BxDF; every supported material has one)
+                    dim += 3;
+                    bs.rdim = sdim;
+                    V3 wiC{0, 0, 0};
+                    float pdfC = 0.f;
+                    int typeC = 0;
+                    V3 fC = bsdf_sample(bs, it, wo, u2x, u2y, T_ALL, wiC, pdfC, typeC);
+                    flags = F_NEE;
+                    if (!(fabsf(pdfC) == 0.f || is_zero(fC))) {
+                        bool specular = (typeC & T_SPEC) != 0;
+                        float cosine = specular ? 1.0f
+                                                : fmax_ref(0.f, (typeC & T_TRANS) ? dot(-it.n, wiC) : dot(it.n, wiC));
+                        bool lp = specular || (typeC & T_TRANS);
+                        beta2 = beta * (fC * cosine / pdfC);
+                        cont = true;
+                        if (bounce > 3) {  // Russian roulette, :96-102
+                            float qq = fmax_ref(.05f, 1.f - length(beta2));
+                            float ur = halton_value(S, hd, idx, dim);
+                            dim++;
+                            if (ur < qq) cont = false;
+                            else beta2 = beta2 / (1 - qq);
+                        }
+                        if (cont) {
+                            V3 no = spawn_origin(it.p, it.n, wiC);
+                            B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
+                            B.ray_d[p] = make_float4(wiC.x, wiC.y, wiC.z, 0.f);
+                            if (lp) flags |= F_LIGHTPATH;
+                        }
+                    }
+                }
             }
-            if (it.material < 0) {
-                // transparent: SpawnRay(dir) and continue without consuming dimensions, :63-68
-                V3 no = spawn_origin(it.p, it.n, d);
-                B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
-                cont = true;
-                flags = meta.w & F_LIGHTPATH;
-            } else {
-                Bsdf bs = make_bsdf(S, it, (uint32_t)idx, (uint32_t)(dim + 2));
-                // UniformSampleAllLights, :175-186
-                float uL0 = halton_value(S, hd, idx, dim), uL1 = halton_value(S, hd, idx, dim + 1);
-                float uB0 = halton_value(S, hd, idx, dim + 2), uB1 = halton_value(S, hd, idx, dim + 3);
-                dim += 4;
-                V3 wo = -d;
-                const int sampleType = T_ALL & ~T_SPEC;
-                bool bsdfSampled = false;
-                V3 fB{0, 0, 0}, wiB{0, 0, 0};
-                float pdfB = 0.f;
-                int typeB = 0;
-                int slot0 = p * P.lmax;
-                for (int li = 0; li < S.num_lights; li++) {
-                    const DevLight& Lt = S.lights[li];
-                    if (!Lt.eligible || li == own) continue;
-                    bool delta = Lt.type == C2_LIGHT_POINT;
-                    // EsimateDirect (i): light sampling, :199-230
+            if (cont) flags |= F_CONTINUE;
+            B.meta[p] = make_int4(meta.x, idx, dim, flags);
+            B.beta2[p] = make_float4(beta2.x, beta2.y, beta2.z, 0.f);
+        }
+        int qpos = wave_append(&B.counters[q ^ 1], live && cont);
+        if (qpos >= 0) next[qpos] = p;
+    }
+}
+
+// Light loop of UniformSampleAllLights / EsimateDirect (:181-278), one thread per
+// (path, eligible light): the light sample (shadow slot p * E + j) and the test
+// whether the shared BSDF-sampled direction needs the MIS ray (Pdf_Li != 0).
+__global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
+    int n = B.counters[q];
+    const int32_t* queue = B.queue[q];
+    const int E = P.lmax;
+    const int j = blockIdx.y;
+    const int li = B.elig[j];
+    const DevLight Lt = S.lights[li];
+    const bool delta = Lt.type == C2_LIGHT_POINT;
+    int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        int i = base + (int)threadIdx.x;
+        bool live = i < n;
+        bool shadow = false, misHit = false;
+        int p = live ? queue[i] : 0;
+        int slot = p * E + j;
+        if (live) {
+            int4 meta = B.meta[p];
+            uint8_t v = 0;
+            if (meta.w & F_NEE) {
+                int4 vi = B.vtx_i[p];
+                if (vi.z != li) {
+                    float4 vp = B.vtx_p[p], vn = B.vtx_n[p], vd = B.vtx_du[p], vdir = B.vtx_d[p];
+                    Inter it;
+                    it.p = V3{vp.x, vp.y, vp.z};
+                    it.n = V3{vn.x, vn.y, vn.z};
+                    it.dpdu = V3{vd.x, vd.y, vd.z};
+                    it.dpdv = cross(it.n, it.dpdu);
+                    it.dir = V3{vdir.x, vdir.y, vdir.z};
+                    it.entity = vi.x;
+                    it.material = vi.y;
+                    it.light = vi.z;
+                    it.front = vi.w != 0;
+                    Bsdf bs = make_bsdf(S, it, (uint32_t)meta.y, 0u);
+                    V3 wo = -it.dir;
+                    // (i) light sampling, :199-230
                     V3 lightPos;
                     float pdfL;
-                    V3 Li = sample_li(S, Lt, it.p, it.n, uL0, uL1, lightPos, pdfL);
+                    V3 Li = sample_li(S, Lt, it.p, it.n, vp.w, vn.w, lightPos, pdfL);
                     V3 wi = normalize(lightPos - it.p);
                     if (pdfL != 0.f && !is_zero(Li)) {
                         V3 f = bsdf_f(bs, it, wo, wi) * fmax_ref(0.f, dot(it.n, wi));
-                        float pB = bsdf_pdf(bs, it, wo, wi, sampleType);
+                        float pB = bsdf_pdf(bs, it, wo, wi, T_ALL & ~T_SPEC);
                         if (!is_zero(f)) {
                             V3 C = delta ? f * Li / pdfL : f * Li * power_heuristic(pdfL, pB) / pdfL;
-                            // SpawnRayTo(lightPos)
-                            bool back = dot(lightPos - it.p, it.n) < 0;
+                            bool back = dot(lightPos - it.p, it.n) < 0;  // SpawnRayTo
                             V3 so = it.p + (back ? -it.n : it.n) * 1e-4f;
                             V3 sd = lightPos - so;
-                            int s = slot0 + nsh;
-                            B.sh_o[s] = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
-                            B.sh_d[s] = make_float4(sd.x, sd.y, sd.z, __int_as_float(li));
-                            B.sh_c[s] = make_float4(C.x, C.y, C.z, 0.f);
-                            B.sh_vis[s] = 0;
-                            nsh++;
+                            B.sh_o[slot] = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
+                            B.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, 0.f);
+                            B.sh_c[slot] = make_float4(C.x, C.y, C.z, 0.f);
+                            v = 2;
+                            shadow = true;
                         }
                     }
-                    // EsimateDirect (ii): BSDF sampling, :235-274 (one Get1D per non-delta light)
-                    if (!delta) {
-                        dim++;
-                        if (!bsdfSampled) {
-                            bsdfSampled = true;
-                            fB = bsdf_sample(bs, it, wo, uB0, uB1, sampleType, wiB, pdfB, typeB);
-                            if (!is_zero(fB)) {
-                                misF = fB * fmax_ref(0.f, dot(it.n, wiB));
-                                misW = wiB;
-                                misPdf = pdfB;
-                            }
-                        }
-                        if (!is_zero(fB) && !misNeeded) {
-                            if (pdf_light(S, Lt, it.p, it.n, wiB) != 0.f) misNeeded = true;
-                        }
-                    }
-                }
-                // continuation sample: sampler->Get1D(), sampler->Get2D() as arguments of one call, :79
-                float u1, u2x, u2y;
-                uint32_t sdim;
-                if (P.arg_order == C2_ARGORDER_RTL) {
-                    sdim = (uint32_t)dim;
-                    u2x = halton_value(S, hd, idx, dim);
-                    u2y = halton_value(S, hd, idx, dim + 1);
-                    u1 = halton_value(S, hd, idx, dim + 2);
-                } else {
-                    u1 = halton_value(S, hd, idx, dim);
-                    sdim = (uint32_t)(dim + 1);
-                    u2x = halton_value(S, hd, idx, dim + 1);
-                    u2y = halton_value(S, hd, idx, dim + 2);
-                }
-                (void)u1;  // one BxDF: BSDF::SampleDirection picks index 0 for every u1
-                dim += 3;
-                bs.rdim = sdim;
-                V3 wiC{0, 0, 0};
-                float pdfC = 0.f;
-                int typeC = 0;
-                V3 fC = bsdf_sample(bs, it, wo, u2x, u2y, T_ALL, wiC, pdfC, typeC);
-                flags = F_NEE;
-                if (!(fabsf(pdfC) == 0.f || is_zero(fC))) {
-                    bool specular = (typeC & T_SPEC) != 0;
-                    float cosine = specular ? 1.0f
-                                            : fmax_ref(0.f, (typeC & T_TRANS) ? dot(-it.n, wiC) : dot(it.n, wiC));
-                    bool lp = specular || (typeC & T_TRANS);
-                    V3 no = spawn_origin(it.p, it.n, wiC);
-                    beta2 = beta * (fC * cosine / pdfC);
-                    cont = true;
-                    if (bounce > 3) {  // Russian roulette, :96-102
-                        float qq = fmax_ref(.05f, 1.f - length(beta2));
-                        float ur = halton_value(S, hd, idx, dim);
-                        dim++;
-                        if (ur < qq) cont = false;
-                        else beta2 = beta2 / (1 - qq);
-                    }
-                    if (cont) {
-                        B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
-                        B.ray_d[p] = make_float4(wiC.x, wiC.y, wiC.z, 0.f);
-                        if (lp) flags |= F_LIGHTPATH;
+                    // (ii) BSDF sampling needs the MIS ray when Pdf_Li != 0, :235-257
+                    if (!delta && __float_as_int(vd.w) != 0) {
+                        float4 mw = B.mis_w[p];
+                        if (pdf_light(S, Lt, it.p, it.n, V3{mw.x, mw.y, mw.z}) != 0.f) misHit = true;
                     }
                 }
             }
+            B.sh_vis[slot] = v;
         }
-        if (live) {
-            if (cont) flags |= F_CONTINUE;
-            if (misNeeded) flags |= F_MIS;
-            B.meta[p] = make_int4(meta.x, idx, dim, flags);
-            B.beta2[p] = make_float4(beta2.x, beta2.y, beta2.z, 0.f);
-            B.nee[p] = make_int4(nsh, own, -1, 0);
-            if (misNeeded) {
-                B.mis_p[p] = make_float4(it.p.x, it.p.y, it.p.z, misPdf);
-                B.mis_n[p] = make_float4(it.n.x, it.n.y, it.n.z, 0.f);
-                B.mis_w[p] = make_float4(misW.x, misW.y, misW.z, 0.f);
-                B.mis_f[p] = make_float4(misF.x, misF.y, misF.z, 0.f);
-            }
-        }
-        // queue appends (all lanes of the wave participate)
-        int qpos = wave_append(&B.counters[q ^ 1], live && cont);
-        if (qpos >= 0) next[qpos] = p;
-        int mpos = wave_append(&B.counters[3], live && misNeeded);
-        if (mpos >= 0) B.mis_list[mpos] = p;
-        // shadow slots: append each valid slot index
-        int maxn = nsh;
-        for (int off = 32; off > 0; off >>= 1) maxn = max(maxn, __shfl_xor(maxn, off));
-        for (int j = 0; j < maxn; j++) {
-            int spos = wave_append(&B.counters[2], live && j < nsh);
-            if (spos >= 0) B.sh_list[spos] = p * P.lmax + j;
-        }
+        int spos = wave_append(&B.counters[2], shadow);
+        if (spos >= 0) B.sh_list[spos] = slot;
+        if (misHit && atomicOr(&B.misflag[p], 1) == 0) B.mis_list[atomicAdd(&B.counters[3], 1)] = p;
     }
 }
 
@@ -386,15 +413,17 @@ __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
     }
 }
 
+// The shared MIS ray: Scene::Intersect(isec.SpawnRay(wi)); only the light owning the
+// hit entity can receive f * Le * w / pdf_bsdf, :255-273
 __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B) {
     int n = B.counters[3];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[2], (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int p = B.mis_list[i];
-        float4 mp = B.mis_p[p], mn = B.mis_n[p], mw = B.mis_w[p], mf = B.mis_f[p];
+        float4 mp = B.vtx_p[p], mn = B.vtx_n[p], mw = B.mis_w[p], mf = B.mis_f[p];
         V3 sp{mp.x, mp.y, mp.z}, sn{mn.x, mn.y, mn.z}, wi{mw.x, mw.y, mw.z}, f{mf.x, mf.y, mf.z};
-        float pdfB = mp.w;
-        int own = B.nee[p].y;
+        float pdfB = B.vtx_d[p].w;
+        int own = B.vtx_i[p].z;
         V3 o = spawn_origin(sp, sn, wi);
         Hit h;
         int lh = -1;
@@ -408,7 +437,7 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B) {
                     float pdfL = pdf_light(S, Lt, sp, sn, wi);
                     if (pdfL != 0.f) {
                         float weight = power_heuristic(pdfB, pdfL);
-                        // lightIsec.Le(-wi): un-flipped normal of the hit
+                        // lightIsec.Le(-wi) with the hit's un-flipped normal
                         V3 hp, hn, hdu;
                         bool front;
                         if (e.x == C2_SHAPE_TRIANGLE) {
@@ -427,12 +456,12 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B) {
             }
         }
         B.mis_L[p] = make_float4(Lb.x, Lb.y, Lb.z, 0.f);
-        int4 ne = B.nee[p];
-        ne.z = lh;
-        B.nee[p] = ne;
+        B.mis_hit[p] = lh;
     }
 }
 
+// L += beta * sum over eligible lights in light order of EsimateDirect's result:
+// the unoccluded light sample (0 + C) plus, for the light owning the MIS hit, Lb.
 __global__ void __launch_bounds__(kBlock) k_resolve(PathBuffers B, PassParams P, int q) {
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
@@ -440,38 +469,29 @@ __global__ void __launch_bounds__(kBlock) k_resolve(PathBuffers B, PassParams P,
         int p = queue[i];
         int4 meta = B.meta[p];
         if (meta.w & F_NEE) {
-            int4 ne = B.nee[p];
-            int lh = (meta.w & F_MIS) ? ne.z : -1;
+            int lh = B.misflag[p] ? B.mis_hit[p] : -1;
             V3 Lb{0, 0, 0};
             if (lh >= 0) {
                 float4 m = B.mis_L[p];
                 Lb = V3{m.x, m.y, m.z};
             }
             V3 U{0, 0, 0};
-            bool misDone = lh < 0;
             int slot0 = p * P.lmax;
-            for (int j = 0; j < ne.x; j++) {
-                int s = slot0 + j;
-                int li = __float_as_int(B.sh_d[s].w);
-                if (!misDone && lh < li) {
-                    U = U + (V3{0, 0, 0} + Lb);
-                    misDone = true;
-                }
+            for (int j = 0; j < P.num_elig; j++) {
+                int li = B.elig[j];
                 V3 E{0, 0, 0};
                 bool any = false;
-                if (B.sh_vis[s]) {
-                    float4 c = B.sh_c[s];
+                if (B.sh_vis[slot0 + j] == 1) {
+                    float4 c = B.sh_c[slot0 + j];
                     E = E + V3{c.x, c.y, c.z};
                     any = true;
                 }
-                if (!misDone && lh == li) {
+                if (li == lh) {
                     E = E + Lb;
-                    misDone = true;
                     any = true;
                 }
                 if (any) U = U + E;
             }
-            if (!misDone) U = U + (V3{0, 0, 0} + Lb);
             float4 b4 = B.beta[p], L4 = B.L[p];
             V3 L = V3{L4.x, L4.y, L4.z} + V3{b4.x, b4.y, b4.z} * U;
             B.L[p] = make_float4(L.x, L.y, L.z, 0.f);
@@ -899,6 +919,13 @@ int c2_accel_count_visits(const c2_accel* a, const float* rays, int64_t n, int32
 
 struct c2_pt {
     std::unique_ptr<c2_accel> accel;
+    bool profiling = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
+    // ray capture request (c2_pt_capture_rays)
+    int cap_bounce = -1, cap_kind = 0;
+    int64_t cap_n = 0, cap_written = 0;
+    float* cap_rays = nullptr;
+    int32_t* cap_ignore = nullptr;
     c2_render_opts opts{};
     PassParams pp{};
     DeviceArena arena;
@@ -953,9 +980,17 @@ int c2_pt_create(const c2_scene_desc* desc, const c2_render_opts* opts, c2_pt** 
         P.bscale0 = hp.base_scales[0];
         P.mult0 = hp.mult_inverse[0];
         P.mult1 = hp.mult_inverse[1];
-        int lmax = 0;
-        for (const DevLight& L : pt->accel->scene.lights) lmax += L.eligible ? 1 : 0;
-        P.lmax = std::max(lmax, 1);
+        std::vector<int32_t> elig;
+        int ndElig = 0;
+        for (size_t l = 0; l < pt->accel->scene.lights.size(); l++) {
+            const DevLight& L = pt->accel->scene.lights[l];
+            if (!L.eligible) continue;
+            elig.push_back((int32_t)l);
+            if (L.type != C2_LIGHT_POINT) ndElig++;
+        }
+        P.num_elig = (int32_t)elig.size();
+        P.nd_elig = ndElig;
+        P.lmax = std::max<int32_t>(P.num_elig, 1);
         int64_t npix = (int64_t)opts->width * opts->height;
         int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : npix;
         // keep the per-path shadow slots within ~6 GiB
@@ -973,9 +1008,15 @@ int c2_pt_create(const c2_scene_desc* desc, const c2_render_opts* opts, c2_pt** 
         B.L = A.alloc<float4>(ch);
         B.meta = A.alloc<int4>(ch);
         B.hit = A.alloc<float4>(ch);
-        B.nee = A.alloc<int4>(ch);
-        B.mis_p = A.alloc<float4>(ch);
-        B.mis_n = A.alloc<float4>(ch);
+        B.vtx_p = A.alloc<float4>(ch);
+        B.vtx_n = A.alloc<float4>(ch);
+        B.vtx_du = A.alloc<float4>(ch);
+        B.vtx_d = A.alloc<float4>(ch);
+        B.vtx_i = A.alloc<int4>(ch);
+        B.misflag = A.alloc<int32_t>(ch);
+        B.mis_hit = A.alloc<int32_t>(ch);
+        if (elig.empty()) elig.push_back(-1);
+        B.elig = A.upload(elig);
         B.mis_w = A.alloc<float4>(ch);
         B.mis_f = A.alloc<float4>(ch);
         B.mis_L = A.alloc<float4>(ch);
@@ -1029,6 +1070,58 @@ int c2_pt_reset(c2_pt* pt) {
     });
 }
 
+}  // extern "C"
+
+namespace {
+struct KTimer {
+    c2_pt* pt;
+    int cls;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(c2_pt* p, int c, hipStream_t st) : pt(p), cls(c), s(st) {
+        if (!pt->profiling) return;
+        HIP_CHECK(hipEventCreate(&a));
+        HIP_CHECK(hipEventCreate(&b));
+        HIP_CHECK(hipEventRecord(a, s));
+    }
+    ~KTimer() {
+        if (!pt->profiling) return;
+        (void)hipEventRecord(b, s);
+        pt->events.push_back({cls, {a, b}});
+    }
+};
+
+// Copies the first rays of the current extension / shadow queue to the capture buffer.
+__global__ void k_capture(PathBuffers B, int kind, int q, int lmax, int64_t n, float* rays, int32_t* ignore,
+                          int32_t* written) {
+    int cnt = kind == 0 ? B.counters[q] : B.counters[2];
+    int64_t m = cnt < n ? cnt : n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *written = (int32_t)m;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        float4 o, d;
+        float tmax;
+        int ig = -1;
+        if (kind == 0) {
+            int p = B.queue[q][i];
+            o = B.ray_o[p];
+            d = B.ray_d[p];
+            tmax = INFINITY;
+        } else {
+            int s = B.sh_list[i];
+            o = B.sh_o[s];
+            d = B.sh_d[s];
+            tmax = 1 - 1e-5f;
+            ig = __float_as_int(o.w);
+        }
+        float* r = rays + i * 8;
+        r[0] = o.x; r[1] = o.y; r[2] = o.z; r[3] = d.x; r[4] = d.y; r[5] = d.z; r[6] = 0.f; r[7] = tmax;
+        if (ignore) ignore[i] = ig;
+    }
+}
+}  // namespace
+
+extern "C" {
+
 int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* stream) {
     return guard([&] {
         if (!pt) fail(C2_ERR_INVALID, "null renderer");
@@ -1043,24 +1136,59 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
                 P.pix_begin = (int32_t)beg;
                 P.pix_count = (int32_t)std::min<int64_t>(pt->chunk, npix - beg);
                 int g = grid_for(P.pix_count);
-                hipLaunchKernelGGL(k_generate, dim3(g), dim3(kBlock), 0, s, S, pt->B, P);
+                {
+                    KTimer kt(pt, C2_K_GENERATE, s);
+                    hipLaunchKernelGGL(k_generate, dim3(g), dim3(kBlock), 0, s, S, pt->B, P);
+                }
                 pt->launches++;
                 int q = 0;
                 if (S.num_entities > 0) {
                     for (int b = 0; b < P.max_depth; b++) {
                         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, pt->B, q ^ 1);
-                        hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBlock), 0, s, S, pt->B, q);
-                        hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q, b);
+                        if (pt->cap_bounce == b && pt->cap_kind == 0) {
+                            hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, pt->B, 0, q,
+                                               P.lmax, pt->cap_n, pt->cap_rays, pt->cap_ignore,
+                                               pt->B.counters + 6);
+                        }
+                        {
+                            KTimer kt(pt, C2_K_EXTEND, s);
+                            hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBlock), 0, s, S, pt->B, q);
+                        }
+                        {
+                            KTimer kt(pt, C2_K_SHADE, s);
+                            hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q, b);
+                        }
+                        if (P.num_elig > 0) {
+                            KTimer kt(pt, C2_K_NEE, s);
+                            hipLaunchKernelGGL(k_nee, dim3(g, P.num_elig), dim3(kBlock), 0, s, S, pt->B, P, q);
+                        }
+                        if (pt->cap_bounce == b && pt->cap_kind == 1) {
+                            hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, pt->B, 1, q,
+                                               P.lmax, pt->cap_n, pt->cap_rays, pt->cap_ignore,
+                                               pt->B.counters + 6);
+                        }
                         int gs = grid_for((int64_t)P.pix_count * P.lmax);
-                        hipLaunchKernelGGL(k_occlude, dim3(gs), dim3(kBlock), 0, s, S, pt->B);
-                        hipLaunchKernelGGL(k_mis, dim3(g), dim3(kBlock), 0, s, S, pt->B);
-                        hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kBlock), 0, s, pt->B, P, q);
-                        pt->launches += 6;
+                        {
+                            KTimer kt(pt, C2_K_OCCLUDE, s);
+                            hipLaunchKernelGGL(k_occlude, dim3(gs), dim3(kBlock), 0, s, S, pt->B);
+                        }
+                        {
+                            KTimer kt(pt, C2_K_MIS, s);
+                            hipLaunchKernelGGL(k_mis, dim3(g), dim3(kBlock), 0, s, S, pt->B);
+                        }
+                        {
+                            KTimer kt(pt, C2_K_RESOLVE, s);
+                            hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kBlock), 0, s, pt->B, P, q);
+                        }
+                        pt->launches += 7;
                         q ^= 1;
                     }
                 }
-                hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, s, pt->B, P, pt->opts.film_mode, pt->film,
-                                   pt->comp, pt->film64);
+                {
+                    KTimer kt(pt, C2_K_ACCUMULATE, s);
+                    hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, s, pt->B, P, pt->opts.film_mode,
+                                       pt->film, pt->comp, pt->film64);
+                }
                 pt->launches++;
                 HIP_CHECK(hipGetLastError());
             }
@@ -1107,6 +1235,62 @@ int c2_pt_get_film(c2_pt* pt, float* rgb) {
                     o[2] = v.z;
                 }
         }
+    });
+}
+
+int c2_pt_set_profiling(c2_pt* pt, int32_t enable) {
+    return guard([&] {
+        if (!pt) fail(C2_ERR_INVALID, "null renderer");
+        HIP_CHECK(hipStreamSynchronize(pt->stream));
+        for (auto& e : pt->events) {
+            (void)hipEventDestroy(e.second.first);
+            (void)hipEventDestroy(e.second.second);
+        }
+        pt->events.clear();
+        pt->profiling = enable != 0;
+    });
+}
+
+int c2_pt_get_kernel_times(c2_pt* pt, c2_kernel_times* out) {
+    return guard([&] {
+        if (!pt || !out) fail(C2_ERR_INVALID, "null argument");
+        HIP_CHECK(hipDeviceSynchronize());
+        c2_kernel_times r{};
+        for (auto& e : pt->events) {
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+            r.ms[e.first] += ms;
+            r.launches[e.first] += 1;
+        }
+        *out = r;
+    });
+}
+
+int c2_pt_capture_rays(c2_pt* pt, int32_t sample, int32_t bounce, int32_t kind, float* rays_out,
+                       int32_t* ignore_out, int64_t n, int64_t* written) {
+    return guard([&] {
+        if (!pt || !rays_out || !written || n <= 0 || (kind != 0 && kind != 1) || bounce < 0)
+            fail(C2_ERR_INVALID, "bad arguments");
+        DeviceArena tmp;
+        n = std::min<int64_t>(n, (int64_t)pt->chunk * (kind == 0 ? 1 : pt->pp.lmax));
+        pt->cap_rays = tmp.alloc<float>((size_t)n * 8);
+        pt->cap_ignore = ignore_out ? tmp.alloc<int32_t>(n) : nullptr;
+        pt->cap_n = n;
+        pt->cap_kind = kind;
+        pt->cap_bounce = bounce;
+        int64_t save = pt->chunk;
+        int rc = c2_pt_render(pt, sample, 1, nullptr);
+        pt->cap_bounce = -1;
+        (void)save;
+        if (rc != C2_OK) fail(rc, c2_last_error());
+        HIP_CHECK(hipStreamSynchronize(pt->stream));
+        int32_t w = 0;
+        HIP_CHECK(hipMemcpy(&w, pt->B.counters + 6, sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(rays_out, pt->cap_rays, (size_t)w * 8 * sizeof(float), hipMemcpyDeviceToHost));
+        if (ignore_out) HIP_CHECK(hipMemcpy(ignore_out, pt->cap_ignore, (size_t)w * sizeof(int32_t), hipMemcpyDeviceToHost));
+        pt->cap_rays = nullptr;
+        pt->cap_ignore = nullptr;
+        *written = w;
     });
 }
 

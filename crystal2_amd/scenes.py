@@ -146,6 +146,69 @@ def cornell(directory: str, width: int = 256, height: int = 256, spp: int = 16, 
     return SceneFiles(directory, sj, cj)
 
 
+def material_zoo(directory: str, width: int = 64, height: int = 64, spp: int = 8, depth: int = 6,
+                 threads: int = 8, accel: str = "BVH", phong: bool = True) -> SceneFiles:
+    """Every supported material and light type in one small scene (parity fixture).
+
+    Cornell-style box; ceiling area-light quad; an emissive analytic sphere (sphere
+    area light); a point light; analytic spheres of specular glass, rough glass
+    (GGX 0.2), isotropic GGX, anisotropic GGX, Blinn-Phong and mirror; a glass
+    icosphere mesh; and a material-less (transparent) triangle pair.
+    """
+    os.makedirs(directory, exist_ok=True)
+    white = {"Type": "Diffuse", "Color": [0.73, 0.73, 0.73]}
+    objects = [{"Type": "Light", "LightType": "Point", "Position": [0.5, 0.6, 0.5], "Flux": [4, 4, 4]}]
+    walls = {
+        "floor.obj": (_quad([-1, -1, -1], [2, 0, 0], [0, 0, 2], [0, 1, 0]), white),
+        "ceiling.obj": (_quad([-1, 1, -1], [2, 0, 0], [0, 0, 2], [0, -1, 0]), white),
+        "back.obj": (_quad([-1, -1, -1], [2, 0, 0], [0, 2, 0], [0, 0, 1]),
+                     {"Type": "Diffuse", "Color": [0.4, 0.5, 0.7]}),
+        "left.obj": (_quad([-1, -1, -1], [0, 0, 2], [0, 2, 0], [1, 0, 0]),
+                     {"Type": "Diffuse", "Color": [0.65, 0.05, 0.05]}),
+        "right.obj": (_quad([1, -1, -1], [0, 0, 2], [0, 2, 0], [-1, 0, 0]),
+                      {"Type": "Diffuse", "Color": [0.12, 0.45, 0.15]}),
+    }
+    for name, ((p, n, f), mat) in walls.items():
+        write_obj(os.path.join(directory, name), p, n, f)
+        objects.append(_mesh_object(name, mat))
+    lp, ln, lf = _quad([-0.3, 0.998, -0.3], [0.6, 0, 0], [0, 0, 0.6], [0, -1, 0])
+    write_obj(os.path.join(directory, "light.obj"), lp, ln, lf)
+    objects.append(_mesh_object("light.obj", white, lemit=[10, 10, 10]))
+    objects.append({"Type": "Geometry", "Shape": {"Type": "Sphere", "Position": [-0.7, 0.6, -0.6], "Radius": 0.08},
+                    "Material": {"Type": "Diffuse", "Color": [0.9, 0.9, 0.9]},
+                    "AreaLight": {"Type": "Diffused", "Lemit": [6, 4, 2], "Samples": 1}})
+    spheres = [
+        ([-0.55, -0.72, -0.35], 0.27, {"Type": "Glass", "Color": [1, 1, 1], "Roughness": [0, 0], "IOR": 1.5}),
+        ([0.0, -0.72, -0.45], 0.27, {"Type": "Glass", "Color": [0.95, 0.95, 1], "Roughness": [0.2, 0.2], "IOR": 1.5}),
+        ([0.55, -0.72, -0.35], 0.27, {"Type": "Micro", "Color": [0.9, 0.6, 0.3], "Roughness": [0.3, 0.3],
+                                      "IOR": 1.5, "NDF": "GGX"}),
+        ([-0.45, -0.2, 0.25], 0.22, {"Type": "Micro", "Color": [0.6, 0.8, 0.9], "Roughness": [0.1, 0.4],
+                                     "IOR": 1.5, "NDF": "GGX"}),
+        ([0.45, -0.2, 0.25], 0.22, {"Type": "Phong", "K": 20, "Color": [0.8, 0.8, 0.5]} if phong
+         else {"Type": "Diffuse", "Color": [0.8, 0.8, 0.5]}),
+        ([0.0, 0.3, -0.5], 0.2, {"Type": "Mirror", "Color": [0.95, 0.95, 0.95]}),
+    ]
+    for pos, r, mat in spheres:
+        objects.append({"Type": "Geometry", "Shape": {"Type": "Sphere", "Position": pos, "Radius": r},
+                        "Material": mat})
+    v, f = icosphere_mesh(2)
+    write_obj(os.path.join(directory, "glassball.obj"), v * 0.22 + np.array([0.0, -0.78, 0.35]), v, f)
+    objects.append(_mesh_object("glassball.obj", {"Type": "Glass", "Color": [1, 1, 1], "Roughness": [0, 0],
+                                                  "IOR": 1.5}))
+    cp, cn, cf = _quad([-0.9, -1.0, 0.6], [0.5, 0, 0], [0, 0.5, 0], [0, 0, 1])
+    write_obj(os.path.join(directory, "curtain.obj"), cp, cn, cf)
+    objects.append(_mesh_object("curtain.obj", None))
+    scene = {"Camera": {"Type": "Normal", "Position": [0, 0, 3.7], "LookAt": [0, 0, 0], "Up": [0, 1, 0],
+                        "FOV": 40},
+             "Scene": {"Textures": {}, "Objects": objects}}
+    sj = os.path.join(directory, "scene.json")
+    with open(sj, "w") as fh:
+        json.dump(scene, fh, indent=1)
+    cj = os.path.join(directory, "config.json")
+    write_config(cj, width, height, spp, depth, threads=threads, accel=accel)
+    return SceneFiles(directory, sj, cj)
+
+
 # ----------------------------------------------------------------------------------------------
 # icosphere + value noise (ray-cast microbenchmark)
 # ----------------------------------------------------------------------------------------------

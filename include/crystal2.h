@@ -225,6 +225,24 @@ int  c2_pt_get_stats(const c2_pt* pt, c2_pt_stats* out);
 /* Synchronise the renderer's stream; returns the first asynchronous error. */
 int  c2_pt_sync(c2_pt* pt);
 
+/* Per-kernel-class timing with HIP events recorded on the renderer's stream
+ * around every launch while profiling is enabled (adds no synchronisation). */
+enum { C2_K_GENERATE = 0, C2_K_EXTEND = 1, C2_K_SHADE = 2, C2_K_OCCLUDE = 3, C2_K_MIS = 4,
+       C2_K_RESOLVE = 5, C2_K_ACCUMULATE = 6, C2_K_NEE = 7, C2_K_COUNT = 8 };
+typedef struct c2_kernel_times {
+    double  ms[C2_K_COUNT];        /* summed kernel time per class            */
+    int64_t launches[C2_K_COUNT];  /* launches per class                      */
+} c2_kernel_times;
+int  c2_pt_set_profiling(c2_pt* pt, int32_t enable);   /* clears the record  */
+int  c2_pt_get_kernel_times(c2_pt* pt, c2_kernel_times* out);  /* synchronises */
+
+/* Renders Halton sample `sample` (adding it to the film) and copies up to n rays
+ * of one queue at bounce `bounce`: kind 0 = extension (path) rays, kind 1 =
+ * shadow rays (with the entity they ignore). rays_out: [n][8] (o, d, tmin, tmax);
+ * ignore_out may be NULL. *written = rays copied. */
+int  c2_pt_capture_rays(c2_pt* pt, int32_t sample, int32_t bounce, int32_t kind, float* rays_out,
+                        int32_t* ignore_out, int64_t n, int64_t* written);
+
 /* ---------------------------------------------------------------- Sampler (Halton)
  * Reference HaltonSampler for a W x H film: vector index of (pixel, sample) and
  * sample value per dimension (dims >= 100 use the documented deterministic
