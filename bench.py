@@ -37,6 +37,7 @@ def log(*a):
 
 
 def build_scene(n_spheres=400, level=4, seed=2024):
+    """The sphere-field scene in memory (same draws as crystal2_amd.scenes.spheres_scene)."""
     import crystal2_amd as c2
     from crystal2_amd import scenes
     d = scenes.spheres_scene_objects(n_spheres, level, seed)
@@ -51,6 +52,21 @@ def build_scene(n_spheres=400, level=4, seed=2024):
     cam = d["camera"]
     sc.set_camera(cam["Position"], cam["LookAt"], cam["Up"], cam["FOV"])
     return sc
+
+
+def scene_bounds(scene):
+    """Axis-aligned bounds of all triangles and spheres of a Scene (for the cache's encoding)."""
+    d = scene.desc()
+    lo = np.full(3, np.inf)
+    hi = np.full(3, -np.inf)
+    if d.num_triangles:
+        t = np.ctypeslib.as_array(d.triangles, shape=(d.num_triangles * 24,)).reshape(-1, 8)[:, :3]
+        lo, hi = np.minimum(lo, t.min(0)), np.maximum(hi, t.max(0))
+    if d.num_spheres:
+        s = np.ctypeslib.as_array(d.spheres, shape=(d.num_spheres * 22,)).reshape(-1, 22)
+        lo = np.minimum(lo, (s[:, :3] - s[:, 3:4]).min(0))
+        hi = np.maximum(hi, (s[:, :3] + s[:, 3:4]).max(0))
+    return lo.tolist(), hi.tolist()
 
 
 def per_ray_bytes(nodes, tris):

@@ -22,8 +22,8 @@ import numpy as np
 from . import _abi
 from ._abi import Crystal2Error, check
 
-__all__ = ["Scene", "GpuBVH", "WavefrontPathIntegrator", "Config", "Crystal2Error", "halton_index",
-           "halton_value", "load_library"]
+__all__ = ["Scene", "GpuBVH", "WavefrontPathIntegrator", "RadianceCache", "Config", "Crystal2Error",
+           "halton_index", "halton_value", "load_library"]
 
 
 def load_library():
@@ -223,7 +223,7 @@ class WavefrontPathIntegrator:
     FILMS = {"f32": _abi.C2_FILM_F32, "kahan": _abi.C2_FILM_F32_KAHAN, "f64": _abi.C2_FILM_F64}
 
     def __init__(self, scene: Scene, width: int, height: int, spp: int, max_depth: int, arg_order: str = "rtl",
-                 film: str = "f32", paths_in_flight: int = 0):
+                 film: str = "f32", paths_in_flight: int = 0, window=None):
         self._lib = load_library()
         self.width, self.height, self.spp, self.max_depth = width, height, spp, max_depth
         o = _abi.c2_render_opts()
@@ -231,6 +231,9 @@ class WavefrontPathIntegrator:
         o.arg_order = _abi.C2_ARGORDER_LTR if arg_order == "ltr" else _abi.C2_ARGORDER_RTL
         o.film_mode = self.FILMS[film]
         o.paths_in_flight = paths_in_flight
+        if window is not None:
+            o.win_x0, o.win_y0, o.win_w, o.win_h = (int(v) for v in window)
+        self.window = window
         self._h = C.c_void_p()
         d = scene.desc()
         check(self._lib.c2_pt_create(C.byref(d), C.byref(o), C.byref(self._h)))
@@ -249,6 +252,16 @@ class WavefrontPathIntegrator:
         out = np.empty((self.height, self.width, 3), np.float32)
         check(self._lib.c2_pt_get_film(self._h, _fp(out)))
         return out
+
+    def set_cache(self, cache: "RadianceCache | None", cache_bounce: int = 2, train_ratio: int = 64):
+        """Terminate camera paths at `cache_bounce` into the cache; train it online."""
+        self._cache = cache
+        check(self._lib.c2_pt_set_cache(self._h, cache._h if cache else None, cache_bounce, train_ratio))
+
+    def cache_times(self) -> dict:
+        t = _abi.c2_cache_times()
+        check(self._lib.c2_pt_get_cache_times(self._h, C.byref(t)))
+        return {"infer_ms": t.infer_ms, "train_ms": t.train_ms}
 
     def set_profiling(self, enable: bool = True):
         check(self._lib.c2_pt_set_profiling(self._h, int(enable)))
@@ -274,6 +287,58 @@ class WavefrontPathIntegrator:
     def close(self):
         if self._h:
             self._lib.c2_pt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RadianceCache:
+    """Online-trained MLP radiance cache (FP16 MFMA forward/backward, FP32 Adam)."""
+
+    PARAMS = _abi.NRC_PARAMS
+
+    def __init__(self, bbox_min, bbox_max, learning_rate=1e-2, batch_capacity=1 << 16, seed=1,
+                 beta1=0.9, beta2=0.99, epsilon=1e-8):
+        self._lib = load_library()
+        o = _abi.c2_nrc_opts()
+        o.learning_rate, o.beta1, o.beta2, o.epsilon = learning_rate, beta1, beta2, epsilon
+        o.seed, o.batch_capacity = seed, batch_capacity
+        o.bbox_min[:] = [float(v) for v in bbox_min]
+        o.bbox_max[:] = [float(v) for v in bbox_max]
+        self.opts = o
+        self._h = C.c_void_p()
+        check(self._lib.c2_nrc_create(C.byref(o), C.byref(self._h)))
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.PARAMS, np.float32)
+        check(self._lib.c2_nrc_get_params(self._h, _fp(p)))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        check(self._lib.c2_nrc_set_params(self._h, _fp(p)))
+
+    def infer(self, feats: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(feats, np.float32).reshape(-1, 16)
+        out = np.empty((len(f), 3), np.float32)
+        check(self._lib.c2_nrc_infer(self._h, _fp(f), len(f), _fp(out)))
+        return out
+
+    def train_step(self, feats: np.ndarray, targets: np.ndarray, apply: bool = True):
+        f = np.ascontiguousarray(feats, np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        g = np.empty(self.PARAMS, np.float32)
+        loss = C.c_float()
+        check(self._lib.c2_nrc_train_step(self._h, _fp(f), _fp(t), len(f), int(apply), _fp(g), C.byref(loss)))
+        return g, loss.value
+
+    def close(self):
+        if self._h:
+            self._lib.c2_nrc_destroy(self._h)
             self._h = None
 
     def __del__(self):

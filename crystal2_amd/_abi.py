@@ -59,7 +59,26 @@ class c2_accel_stats(C.Structure):
 class c2_render_opts(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
                 ("arg_order", C.c_int32), ("film_mode", C.c_int32), ("paths_in_flight", C.c_int32),
-                ("block_size", C.c_int32)]
+                ("block_size", C.c_int32), ("win_x0", C.c_int32), ("win_y0", C.c_int32), ("win_w", C.c_int32),
+                ("win_h", C.c_int32)]
+
+
+class c2_nrc_opts(C.Structure):
+    _fields_ = [("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("seed", C.c_int32), ("batch_capacity", C.c_int32), ("bbox_min", C.c_float * 3),
+                ("bbox_max", C.c_float * 3)]
+
+
+class c2_nrc_stats(C.Structure):
+    _fields_ = [("steps", C.c_int64), ("flop_forward_per_query", C.c_double)]
+
+
+class c2_cache_times(C.Structure):
+    _fields_ = [("infer_ms", C.c_double), ("train_ms", C.c_double), ("queries", C.c_int64),
+                ("train_rows", C.c_int64)]
+
+
+NRC_PARAMS = 5 * 64 * 64 + 5 * 64
 
 
 class c2_pt_stats(C.Structure):
@@ -110,6 +129,15 @@ SIGNATURES = [
     ("c2_pt_get_kernel_times", C.c_int, [P, C.POINTER(c2_kernel_times)]),
     ("c2_pt_capture_rays", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, _f32p, _i32p, C.c_int64,
                                      C.POINTER(C.c_int64)]),
+    ("c2_nrc_create", C.c_int, [C.POINTER(c2_nrc_opts), C.POINTER(P)]),
+    ("c2_nrc_destroy", None, [P]),
+    ("c2_nrc_get_params", C.c_int, [P, _f32p]),
+    ("c2_nrc_set_params", C.c_int, [P, _f32p]),
+    ("c2_nrc_infer", C.c_int, [P, _f32p, C.c_int64, _f32p]),
+    ("c2_nrc_train_step", C.c_int, [P, _f32p, _f32p, C.c_int64, C.c_int32, _f32p, _f32p]),
+    ("c2_nrc_get_stats", C.c_int, [P, C.POINTER(c2_nrc_stats)]),
+    ("c2_pt_set_cache", C.c_int, [P, P, C.c_int32, C.c_int32]),
+    ("c2_pt_get_cache_times", C.c_int, [P, C.POINTER(c2_cache_times)]),
     ("c2_halton_index", C.c_int, [C.c_int32] * 5 + [_i32p]),
     ("c2_halton_value", C.c_int, [C.c_int32] * 4 + [_f32p]),
 ]

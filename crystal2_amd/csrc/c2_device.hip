@@ -80,6 +80,7 @@ enum : int32_t {
     F_CONTINUE = 2,   // path continues after this vertex
     F_NEE = 4,        // this vertex ran UniformSampleAllLights (resolve adds beta * U)
     F_MIS = 8,        // MIS ray queued for this vertex
+    F_TRAIN = 16,     // radiance-cache training path (index in bits 8..31)
 };
 
 struct PathBuffers {
@@ -106,9 +107,18 @@ struct PathBuffers {
     float4* sh_c;    // unshadowed contribution
     uint8_t* sh_vis; // 0 = none / occluded, 2 = pending, 1 = unoccluded
     int32_t* queue[2];
+    // radiance cache
+    float4* nrc_feat;   // [path][4]: query features
+    int32_t* nrc_list;  // paths querying the cache this bounce
+    int32_t* tlist;     // training path -> path
+    float4* trec;       // [training path][depth][6]: features x4, beta, L after emission
+    uint8_t* trec_valid;
+    float4* bfeat;      // training batch features [cap][4]
+    float4* btarget;    // training batch targets [cap]
     int32_t* sh_list;
     int32_t* mis_list;
-    int32_t* counters;  // [0..1] queue counts, [2] shadow count, [3] mis count, [4..7] ray totals (int)
+    int32_t* counters;  // [0..1] queue counts, [2] shadow, [3] mis, [4] cache queries, [5] training
+                        // paths, [6] captured rays, [7] training rows
     unsigned long long* totals;  // extension, shadow, mis rays (64-bit running totals)
 };
 
@@ -124,7 +134,30 @@ struct PassParams {
     int32_t stride0, exp0, scale1, mult0, mult1, bscale0, bexp1;  // Halton constants
     V3 eye, look, up, right;
     float camW, camH;
+    int32_t win_x0, win_y0, win_w;  // pixel window: linear index i -> (x0 + i % w, y0 + i / w)
+    int32_t cache_bounce;            // radiance cache: -1 off
+    int32_t train_ratio, train_phase;
+    int32_t train_cap;               // training rows per step
+    int32_t train_paths;             // training path record capacity
 };
+
+// Radiance-cache query features: position, outgoing direction, facing normal,
+// roughness (diffuse 1, specular 0, GGX sqrt(ax ay), Phong sqrt(2 / (K + 2))), albedo.
+__device__ __forceinline__ void cache_features(const DevScene& S, const Inter& it, V3 d, float4* out) {
+    float rough = 1.f;
+    V3 alb{0.f, 0.f, 0.f};
+    if (it.material >= 0) {
+        const c2_material m = S.materials[it.material];
+        alb = V3{m.color[0], m.color[1], m.color[2]};
+        if (m.type == C2_MAT_MIRROR) rough = 0.f;
+        else if (m.type == C2_MAT_GLASS || m.type == C2_MAT_MICRO) rough = sqrtf(m.roughness[0] * m.roughness[1]);
+        else if (m.type == C2_MAT_PHONG) rough = sqrtf(2.f / (m.phong_k + 2.f));
+    }
+    out[0] = make_float4(it.p.x, it.p.y, it.p.z, -d.x);
+    out[1] = make_float4(-d.y, -d.z, it.n.x, it.n.y);
+    out[2] = make_float4(it.n.z, rough, alb.x, alb.y);
+    out[3] = make_float4(alb.z, 0.f, 0.f, 0.f);
+}
 
 __device__ __forceinline__ int32_t halton_index_dev(const PassParams& P, int x, int y, int k) {
     // HaltonSampler::GetVectorIndexForSample, HaltonSampler.cpp:251-270
@@ -157,9 +190,17 @@ __device__ __forceinline__ int32_t wave_append(int32_t* counter, bool pred) {
 
 __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, PassParams P) {
     HaltonDev hd{P.exp0, P.scale1};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.pix_count; i += gridDim.x * blockDim.x) {
-        int pixel = P.pix_begin + i;
-        int x = pixel % P.width, y = pixel / P.width;
+    int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < P.pix_count; base += stride) {
+        int i = base + (int)threadIdx.x;
+        bool live = i < P.pix_count;
+        int lin = P.pix_begin + i;
+        int x = P.win_x0 + lin % P.win_w, y = P.win_y0 + lin / P.win_w;
+        int pixel = y * P.width + x;
+        bool train = live && P.cache_bounce >= 0 && (pixel % P.train_ratio) == P.train_phase;
+        int tp = wave_append(&B.counters[5], train);
+        if (tp >= P.train_paths) train = false;  // beyond the record capacity: an ordinary path
+        if (!live) continue;
         int32_t idx = halton_index_dev(P, x, y, P.sample);
         float s0 = halton_value(S, hd, idx, 0), s1 = halton_value(S, hd, idx, 1);
         float px = (float)x + s0, py = (float)y + s1;
@@ -174,13 +215,25 @@ __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, 
         B.ray_d[i] = make_float4(d.x, d.y, d.z, 0.f);
         B.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
         B.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        B.meta[i] = make_int4(pixel, idx, 2, F_LIGHTPATH);
+        int fl = F_LIGHTPATH;
+        if (train) {
+            fl |= F_TRAIN | (tp << 8);
+            B.tlist[tp] = i;
+            for (int b = 0; b < P.max_depth; b++) B.trec_valid[(int64_t)tp * P.max_depth + b] = 0;
+        }
+        B.meta[i] = make_int4(pixel, idx, 2, fl);
         B.queue[0][i] = i;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) B.counters[0] = P.pix_count;
 }
 
+__global__ void k_chunk_reset(PathBuffers B) {
+    B.counters[5] = 0;
+    B.counters[7] = 0;
+}
+
 __global__ void k_reset(PathBuffers B, int next_q) {
+    B.counters[4] = 0;
     B.counters[next_q] = 0;
     B.counters[2] = 0;
     B.counters[3] = 0;
@@ -212,7 +265,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + (int)threadIdx.x;
         bool live = i < n;
-        bool cont = false;
+        bool cont = false, query = false;
         int p = live ? queue[i] : 0;
         if (live) {
             int4 meta = B.meta[p];
@@ -229,6 +282,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                 Hit h{ent, hh.x, hh.y, hh.z};
                 Inter it = make_inter(S, o, d, h);
                 // emission when the previous bounce was specular / transmissive, :54-57
+                float4 Lpre = B.L[p];
                 if (meta.w & F_LIGHTPATH) {
                     V3 Le{0, 0, 0};
                     if (it.light >= 0) {
@@ -236,17 +290,30 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                         V3 nact = it.front ? it.n : -it.n;
                         Le = dot(nact, -d) < 0 ? V3{0, 0, 0} : V3{Ld.L[0], Ld.L[1], Ld.L[2]};
                     }
-                    float4 L4 = B.L[p];
-                    V3 L = V3{L4.x, L4.y, L4.z} + beta * Le;
-                    B.L[p] = make_float4(L.x, L.y, L.z, 0.f);
+                    V3 L = V3{Lpre.x, Lpre.y, Lpre.z} + beta * Le;
+                    Lpre = make_float4(L.x, L.y, L.z, 0.f);
+                    B.L[p] = Lpre;
                 }
-                if (it.material < 0) {
+                const bool training = (meta.w & F_TRAIN) != 0;
+                if (it.material >= 0 && P.cache_bounce >= 0 && !training && bounce == P.cache_bounce) {
+                    // the path ends in the radiance cache: L += beta * cache(x, wo) after this kernel
+                    cache_features(S, it, d, B.nrc_feat + (int64_t)p * 4);
+                    query = true;
+                } else if (it.material < 0) {
                     // transparent: SpawnRay(dir), no dimensions consumed, :63-68
                     V3 no = spawn_origin(it.p, it.n, d);
                     B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
                     cont = true;
                     flags = meta.w & F_LIGHTPATH;
                 } else {
+                    if (training) {  // training row: features, throughput and radiance so far
+                        int64_t r = ((int64_t)(meta.w >> 8) * P.max_depth + bounce);
+                        float4* rec = B.trec + r * 6;
+                        cache_features(S, it, d, rec);
+                        rec[4] = make_float4(beta.x, beta.y, beta.z, 0.f);
+                        rec[5] = Lpre;
+                        B.trec_valid[r] = 1;
+                    }
                     Bsdf bs = make_bsdf(S, it, (uint32_t)idx, (uint32_t)(dim + 2));
                     V3 wo = -d;
                     // UniformSampleAllLights: uL = Get2D(), uB = Get2D(), :179-180
@@ -320,11 +387,37 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                 }
             }
             if (cont) flags |= F_CONTINUE;
+            flags |= meta.w & ~0xF;  // keep the training tag
             B.meta[p] = make_int4(meta.x, idx, dim, flags);
             B.beta2[p] = make_float4(beta2.x, beta2.y, beta2.z, 0.f);
         }
         int qpos = wave_append(&B.counters[q ^ 1], live && cont);
         if (qpos >= 0) next[qpos] = p;
+        int cpos = wave_append(&B.counters[4], live && query);
+        if (cpos >= 0) B.nrc_list[cpos] = p;
+    }
+}
+
+// Training rows of this pass: target = (L_final - L after the vertex's emission) / beta.
+__global__ void __launch_bounds__(kBlock) k_cache_collect(PathBuffers B, PassParams P) {
+    int nt = min(B.counters[5], P.train_paths);
+    for (int tp = blockIdx.x * blockDim.x + threadIdx.x; tp < nt; tp += gridDim.x * blockDim.x) {
+        int p = B.tlist[tp];
+        float4 Lf = B.L[p];
+        for (int b = 0; b < P.max_depth; b++) {
+            int64_t r = (int64_t)tp * P.max_depth + b;
+            if (!B.trec_valid[r]) continue;
+            const float4* rec = B.trec + r * 6;
+            float4 be = rec[4], lp = rec[5];
+            float t0 = be.x > 0.f ? (Lf.x - lp.x) / be.x : 0.f;
+            float t1 = be.y > 0.f ? (Lf.y - lp.y) / be.y : 0.f;
+            float t2 = be.z > 0.f ? (Lf.z - lp.z) / be.z : 0.f;
+            if (!(isfinite(t0) && isfinite(t1) && isfinite(t2))) continue;
+            int slot = atomicAdd(&B.counters[7], 1);
+            if (slot >= P.train_cap) continue;
+            for (int k = 0; k < 4; k++) B.bfeat[(int64_t)slot * 4 + k] = rec[k];
+            B.btarget[slot] = make_float4(fmaxf(t0, 0.f), fmaxf(t1, 0.f), fmaxf(t2, 0.f), 0.f);
+        }
     }
 }
 
@@ -937,7 +1030,19 @@ struct c2_pt {
     hipStream_t stream = nullptr;
     int64_t launches = 0;
     int64_t samples = 0;
+    // radiance cache
+    c2_nrc* nrc = nullptr;
+    DeviceArena cache_arena;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> cache_events;  // 0 infer, 1 train
+    int64_t cache_queries = 0, cache_rows = 0;
 };
+
+namespace c2 {
+void nrc_launch_infer(c2_nrc* c, const int32_t* list, const int32_t* countp, int32_t maxn, const float4* feats,
+                      float4* L, const float4* beta, hipStream_t s);
+void nrc_launch_train(c2_nrc* c, const float4* feats, const float4* targets, const int32_t* countp, int32_t maxn,
+                      hipStream_t s, bool adam);
+}  // namespace c2
 
 extern "C" {
 
@@ -972,6 +1077,22 @@ int c2_pt_create(const c2_scene_desc* desc, const c2_render_opts* opts, c2_pt** 
         V3 pr = dot(up0, P.look) * P.look;
         P.up = normalize(up0 - pr);
         P.right = normalize(cross(P.look, up0));
+        if (opts->win_w > 0) {
+            if (opts->win_x0 < 0 || opts->win_y0 < 0 || opts->win_h <= 0 || opts->win_x0 + opts->win_w > opts->width ||
+                opts->win_y0 + opts->win_h > opts->height)
+                fail(C2_ERR_INVALID, "render options: pixel window outside the film");
+            P.win_x0 = opts->win_x0;
+            P.win_y0 = opts->win_y0;
+            P.win_w = opts->win_w;
+        } else {
+            P.win_x0 = P.win_y0 = 0;
+            P.win_w = opts->width;
+        }
+        P.cache_bounce = -1;
+        P.train_ratio = 1;
+        P.train_phase = 0;
+        P.train_cap = 0;
+        P.train_paths = 0;
         HaltonParams hp = halton_params(opts->width, opts->height);
         P.stride0 = hp.stride;
         P.exp0 = hp.base_exponents[0];
@@ -991,7 +1112,7 @@ int c2_pt_create(const c2_scene_desc* desc, const c2_render_opts* opts, c2_pt** 
         P.num_elig = (int32_t)elig.size();
         P.nd_elig = ndElig;
         P.lmax = std::max<int32_t>(P.num_elig, 1);
-        int64_t npix = (int64_t)opts->width * opts->height;
+        int64_t npix = opts->win_w > 0 ? (int64_t)opts->win_w * opts->win_h : (int64_t)opts->width * opts->height;
         int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : npix;
         // keep the per-path shadow slots within ~6 GiB
         int64_t perPath = (int64_t)P.lmax * (3 * 16 + 1 + 4) + 16 * 14 + 12;
@@ -1129,13 +1250,15 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
         hipStream_t s = stream ? (hipStream_t)stream : pt->stream;
         PassParams P = pt->pp;
         DevScene S = pt->accel->scene.dev;
-        int64_t npix = (int64_t)P.width * P.height;
+        int64_t npix = pt->opts.win_w > 0 ? (int64_t)pt->opts.win_w * pt->opts.win_h : (int64_t)P.width * P.height;
         for (int k = sample_begin; k < sample_begin + sample_count; k++) {
             P.sample = k;
+            if (P.cache_bounce >= 0) P.train_phase = (int32_t)(((uint32_t)k * 2654435761u) % (uint32_t)P.train_ratio);
             for (int64_t beg = 0; beg < npix; beg += pt->chunk) {
                 P.pix_begin = (int32_t)beg;
                 P.pix_count = (int32_t)std::min<int64_t>(pt->chunk, npix - beg);
                 int g = grid_for(P.pix_count);
+                if (P.cache_bounce >= 0) hipLaunchKernelGGL(k_chunk_reset, dim3(1), dim3(1), 0, s, pt->B);
                 {
                     KTimer kt(pt, C2_K_GENERATE, s);
                     hipLaunchKernelGGL(k_generate, dim3(g), dim3(kBlock), 0, s, S, pt->B, P);
@@ -1157,6 +1280,16 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
                         {
                             KTimer kt(pt, C2_K_SHADE, s);
                             hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q, b);
+                        }
+                        if (P.cache_bounce == b) {
+                            hipEvent_t e0, e1;
+                            HIP_CHECK(hipEventCreate(&e0));
+                            HIP_CHECK(hipEventCreate(&e1));
+                            HIP_CHECK(hipEventRecord(e0, s));
+                            nrc_launch_infer(pt->nrc, pt->B.nrc_list, pt->B.counters + 4, P.pix_count, pt->B.nrc_feat,
+                                             pt->B.L, pt->B.beta, s);
+                            HIP_CHECK(hipEventRecord(e1, s));
+                            pt->cache_events.push_back({0, {e0, e1}});
                         }
                         if (P.num_elig > 0) {
                             KTimer kt(pt, C2_K_NEE, s);
@@ -1183,6 +1316,16 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
                         pt->launches += 7;
                         q ^= 1;
                     }
+                }
+                if (P.cache_bounce >= 0) {
+                    hipEvent_t e0, e1;
+                    HIP_CHECK(hipEventCreate(&e0));
+                    HIP_CHECK(hipEventCreate(&e1));
+                    HIP_CHECK(hipEventRecord(e0, s));
+                    hipLaunchKernelGGL(k_cache_collect, dim3(grid_for(P.train_paths)), dim3(kBlock), 0, s, pt->B, P);
+                    nrc_launch_train(pt->nrc, pt->B.bfeat, pt->B.btarget, pt->B.counters + 7, P.train_cap, s, true);
+                    HIP_CHECK(hipEventRecord(e1, s));
+                    pt->cache_events.push_back({1, {e0, e1}});
                 }
                 {
                     KTimer kt(pt, C2_K_ACCUMULATE, s);
@@ -1291,6 +1434,60 @@ int c2_pt_capture_rays(c2_pt* pt, int32_t sample, int32_t bounce, int32_t kind, 
         pt->cap_rays = nullptr;
         pt->cap_ignore = nullptr;
         *written = w;
+    });
+}
+
+int c2_pt_set_cache(c2_pt* pt, c2_nrc* nrc, int32_t cache_bounce, int32_t train_ratio) {
+    return guard([&] {
+        if (!pt) fail(C2_ERR_INVALID, "null renderer");
+        HIP_CHECK(hipStreamSynchronize(pt->stream));
+        if (!nrc) {
+            pt->nrc = nullptr;
+            pt->pp.cache_bounce = -1;
+            return;
+        }
+        if (cache_bounce < 0 || cache_bounce >= pt->pp.max_depth || train_ratio < 1)
+            fail(C2_ERR_INVALID, "cache: bounce must be in [0, depth) and train_ratio >= 1");
+        c2_nrc_stats st{};
+        int32_t cap = (int32_t)std::min<int64_t>(1 << 16, (int64_t)pt->chunk * pt->pp.max_depth);
+        int64_t tpaths = (int64_t)pt->chunk / train_ratio + pt->opts.height + 64;
+        PathBuffers& B = pt->B;
+        DeviceArena& A = pt->cache_arena;
+        B.nrc_feat = A.alloc<float4>((size_t)pt->chunk * 4);
+        B.nrc_list = A.alloc<int32_t>(pt->chunk);
+        B.tlist = A.alloc<int32_t>(tpaths);
+        B.trec = A.alloc<float4>((size_t)tpaths * pt->pp.max_depth * 6);
+        B.trec_valid = A.alloc<uint8_t>((size_t)tpaths * pt->pp.max_depth);
+        B.bfeat = A.alloc<float4>((size_t)cap * 4);
+        B.btarget = A.alloc<float4>(cap);
+        (void)st;
+        pt->nrc = nrc;
+        pt->pp.cache_bounce = cache_bounce;
+        pt->pp.train_ratio = train_ratio;
+        pt->pp.train_cap = cap;
+        pt->pp.train_paths = (int32_t)tpaths;
+    });
+}
+
+int c2_pt_get_cache_times(c2_pt* pt, c2_cache_times* out) {
+    return guard([&] {
+        if (!pt || !out) fail(C2_ERR_INVALID, "null argument");
+        HIP_CHECK(hipDeviceSynchronize());
+        c2_cache_times r{};
+        int passes[2] = {0, 0};
+        double ms[2] = {0, 0};
+        for (auto& e : pt->cache_events) {
+            float t = 0;
+            HIP_CHECK(hipEventElapsedTime(&t, e.second.first, e.second.second));
+            ms[e.first] += t;
+            passes[e.first]++;
+            (void)hipEventDestroy(e.second.first);
+            (void)hipEventDestroy(e.second.second);
+        }
+        pt->cache_events.clear();
+        r.infer_ms = passes[0] ? ms[0] : 0;
+        r.train_ms = passes[1] ? ms[1] : 0;
+        *out = r;
     });
 }
 

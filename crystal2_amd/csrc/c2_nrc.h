@@ -1,0 +1,39 @@
+// Radiance cache network shared definitions (device + host).
+//
+// MLP 64 -> 64 -> 64 -> 64 -> 64 -> 3 (4 hidden ReLU layers, linear output),
+// FP16 operands on v_mfma_f32_16x16x16f16 with FP32 accumulation, FP32 master
+// weights and Adam. A query is 16 raw floats: position, outgoing direction,
+// facing normal, roughness, albedo; the 64-dim input encoding is built inside
+// the kernels (per coordinate 8 frequencies x {sin, cos}, spherical direction and
+// normal, 4-bin one-blob roughness, albedo, bias).
+//
+// Feature-major formulation per wavefront: H_out^T (64 x 16 queries) =
+// W^T (64 x 64) * H_in^T (64 x 16). With the MFMA fragment layouts
+//   A: lane l holds A[l % 16][4 (l / 16) + j]
+//   B: lane l holds B[4 (l / 16) + j][l % 16]
+//   D: lane l holds D[4 (l / 16) + j][l % 16]
+// the D fragment of output tile m is exactly the B fragment of input tile m of
+// the next layer, so layers chain in registers without transposes.
+#pragma once
+#include <cstdint>
+
+namespace c2 {
+
+constexpr int kNrcIn = 64;
+constexpr int kNrcW = 64;       // hidden width
+constexpr int kNrcLayers = 5;   // 4 hidden + output
+constexpr int kNrcOut = 3;
+constexpr int kNrcFeat = 16;    // raw floats per query
+constexpr int kNrcParams = kNrcLayers * kNrcW * kNrcW + kNrcLayers * kNrcW;  // padded storage
+
+// Forward FLOP per query: 4 x (64 x 64 x 2) + 64 x 3 x 2
+constexpr double kNrcFlopFwd = 4.0 * 64 * 64 * 2 + 64.0 * 3 * 2;
+
+struct NrcDev {
+    const _Float16* wT;  // [layer][out][in]  forward A fragments
+    const _Float16* w;   // [layer][in][out]  backward A fragments
+    const float* b;      // [layer][64]
+    float bmin[3], binv[3];  // position normalisation to [0, 1]
+};
+
+}  // namespace c2

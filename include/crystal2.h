@@ -204,7 +204,9 @@ typedef struct c2_render_opts {
                                  PathTracingIntegrator.cpp:79 (RTL = GCC/MSVC)       */
     int32_t film_mode;        /* C2_FILM_*                                             */
     int32_t paths_in_flight;  /* 0 = all pixels of one sample pass at once             */
-    int32_t block_size;       /* 0 = default (256)                                     */
+    int32_t block_size;       /* reserved, 0                                           */
+    int32_t win_x0, win_y0;   /* pixel window (film coordinates, y = 0 at the bottom); */
+    int32_t win_w, win_h;     /* win_w = 0: the whole film                             */
 } c2_render_opts;
 
 typedef struct c2_pt_stats {
@@ -242,6 +244,56 @@ int  c2_pt_get_kernel_times(c2_pt* pt, c2_kernel_times* out);  /* synchronises *
  * ignore_out may be NULL. *written = rays copied. */
 int  c2_pt_capture_rays(c2_pt* pt, int32_t sample, int32_t bounce, int32_t kind, float* rays_out,
                         int32_t* ignore_out, int64_t n, int64_t* written);
+
+/* ---------------------------------------------------------------- Radiance cache (new, no reference counterpart)
+ * An online-trained MLP (64-dim input encoding, 4 hidden layers x 64 ReLU, 3
+ * outputs) with FP16 MFMA forward / backward and FP32 Adam. A query is 16 floats:
+ * position, outgoing direction, facing normal, roughness, albedo, 3 zeros; the
+ * prediction is the radiance scattered from that point along the direction.
+ * Parameters: [5 layers][64 in][64 out] weights then [5][64] biases (the output
+ * layer uses 3 of its 64 columns). */
+typedef struct c2_nrc c2_nrc;
+
+typedef struct c2_nrc_opts {
+    float learning_rate;      /* Adam step size (1e-2)                    */
+    float beta1, beta2, epsilon;
+    int32_t seed;             /* weight initialisation                    */
+    int32_t batch_capacity;   /* training rows per step (2^16)            */
+    float bbox_min[3], bbox_max[3];  /* position normalisation            */
+} c2_nrc_opts;
+
+typedef struct c2_nrc_stats {
+    int64_t steps;
+    double  flop_forward_per_query;
+} c2_nrc_stats;
+
+#define C2_NRC_PARAMS (5 * 64 * 64 + 5 * 64)
+
+int  c2_nrc_create(const c2_nrc_opts* opts, c2_nrc** out);
+void c2_nrc_destroy(c2_nrc* nrc);
+int  c2_nrc_get_params(c2_nrc* nrc, float* params);        /* C2_NRC_PARAMS floats */
+int  c2_nrc_set_params(c2_nrc* nrc, const float* params);
+/* Host convenience: features [n][16] -> radiance [n][3]. */
+int  c2_nrc_infer(c2_nrc* nrc, const float* features, int64_t n, float* out);
+/* One training step on host data: features [n][16], targets [n][3]. Writes the
+ * mean relative-L2 loss gradient (C2_NRC_PARAMS floats) and loss; apply != 0
+ * also takes the Adam step. */
+int  c2_nrc_train_step(c2_nrc* nrc, const float* features, const float* targets, int64_t n, int32_t apply,
+                       float* grad_out, float* loss_out);
+int  c2_nrc_get_stats(const c2_nrc* nrc, c2_nrc_stats* out);
+
+/* Cache mode for a renderer: camera paths end at bounce `cache_bounce` with the
+ * cache's prediction; one pixel in `train_ratio` per pass traces a full training
+ * path whose vertices become training rows (target = radiance after the vertex /
+ * throughput), and one Adam step runs per pass. nrc == NULL disables it. */
+int  c2_pt_set_cache(c2_pt* pt, c2_nrc* nrc, int32_t cache_bounce, int32_t train_ratio);
+/* Mean time per pass of the cache's kernels (inference, training step), in ms,
+ * and the rows / queries they processed (for MFMA utilisation). */
+typedef struct c2_cache_times {
+    double infer_ms, train_ms;
+    int64_t queries, train_rows;
+} c2_cache_times;
+int  c2_pt_get_cache_times(c2_pt* pt, c2_cache_times* out);
 
 /* ---------------------------------------------------------------- Sampler (Halton)
  * Reference HaltonSampler for a W x H film: vector index of (pixel, sample) and
