@@ -103,6 +103,87 @@ def cpu_baseline(scene, width, height, depth, seconds_budget=20.0):
             "sample": f"rows {y0}..{y0 + rows} of {width}x{height}, Halton sample 0 ({samples} samples, {s:.1f}s)"}
 
 
+def _crop(film, W, H, win):
+    """Window (x0, y0, w, h) in film coordinates (y up) of a top-down film."""
+    x0, y0, w, h = win
+    return film[H - y0 - h:H - y0, x0:x0 + w]
+
+
+def _relmse(x, r):
+    return float(np.mean((x.astype(np.float64) - r) ** 2 / (r.astype(np.float64) ** 2 + 1e-2)))
+
+
+def equal_time(scene, W, H, depth, seconds, judge_spp=4096, crop=256, cache_bounce=2, train_ratio=64,
+               warmup_passes=8):
+    """relMSE at equal wall time: plain PT vs radiance-cache PT, both judged against a
+    plain render of `judge_spp` samples with an FP64 film on a centre crop. The judge uses
+    Halton samples [judge_offset, judge_offset + judge_spp), the runs samples from 0."""
+    import crystal2_amd as c2
+    win = ((W - crop) // 2, (H - crop) // 2, crop, crop)
+    judge_offset = 8192
+    t0 = time.perf_counter()
+    judge = c2.WavefrontPathIntegrator(scene, W, H, judge_spp, depth, film="f64", window=win)
+    judge.render(judge_offset, judge_spp)
+    judge.sync()
+    ref = _crop(judge.film(), W, H, win)
+    judge.close()
+    log(f"judge: {judge_spp} spp FP64 on {crop}x{crop} in {time.perf_counter() - t0:.1f}s")
+
+    def run(pt, budget, checkpoints, start=0):
+        used, k, curve = 0.0, start, []
+        while used < budget:
+            a = time.perf_counter()
+            pt.render(k, 1)
+            pt.sync()
+            used += time.perf_counter() - a
+            k += 1
+            if (k - start) in checkpoints or used >= budget:
+                curve.append((round(used, 4), k - start, _relmse(_crop(pt.film(), W, H, win), ref)))
+        return curve
+
+    cps = {1, 2, 4, 8, 16, 32, 64, 128, 256, 512}
+    plain = c2.WavefrontPathIntegrator(scene, W, H, 4096, depth, film="f32")
+    plain_curve = run(plain, seconds, cps)
+    plain.close()
+    lo, hi = scene_bounds(scene)
+    cache = c2.RadianceCache(lo, hi, learning_rate=1e-2, batch_capacity=1 << 16, seed=1)
+    pt = c2.WavefrontPathIntegrator(scene, W, H, 4096, depth, film="kahan")
+    pt.set_cache(cache, cache_bounce=cache_bounce, train_ratio=train_ratio)
+    a = time.perf_counter()
+    pt.render(4096, warmup_passes)  # training passes on samples outside both runs
+    pt.sync()
+    warm = time.perf_counter() - a
+    pt.reset()
+    pt.cache_times()  # clear
+    cache_curve = run(pt, seconds - warm, cps)
+    cache_curve = [(round(t + warm, 4), n, r) for (t, n, r) in cache_curve]
+    ct = pt.cache_times()
+    st = cache.stats()
+    pt.close()
+    target = plain_curve[-1][2]
+    reach = next((t for (t, n, r) in cache_curve if r <= target), None)
+    flop_q = st["flop_forward_per_query"]
+    npass = cache_curve[-1][1]
+    infer_tflops = st["queries"] * flop_q / max(ct["infer_ms"], 1e-9) / 1e9
+    # training: forward + backward (dX) + weight gradient = 3x the forward FLOP per row
+    train_tflops = st["train_rows"] * 3 * flop_q / max(ct["train_ms"], 1e-9) / 1e9
+    return {
+        "seconds": seconds,
+        "judge": {"spp": judge_spp, "film": "f64", "window": list(win), "samples": [judge_offset,
+                                                                                  judge_offset + judge_spp]},
+        "plain": {"passes": plain_curve[-1][1], "relmse": plain_curve[-1][2], "curve": plain_curve},
+        "cache": {"passes": npass, "warmup_passes": warmup_passes, "warmup_s": round(warm, 4),
+                  "relmse": cache_curve[-1][2], "curve": cache_curve, "cache_bounce": cache_bounce,
+                  "train_ratio": train_ratio, "adam_steps": st["steps"]},
+        "cache_time_to_plain_relmse_frac": None if reach is None else round(reach / seconds, 4),
+        "mfma": {"infer_tflops": round(infer_tflops, 2), "train_tflops": round(train_tflops, 2),
+                 "infer_util_vs_fp16_peak": round(infer_tflops / 2500.0, 5),
+                 "train_util_vs_fp16_peak": round(train_tflops / 2500.0, 5),
+                 "infer_ms_per_pass": round(ct["infer_ms"] / max(1, npass), 4),
+                 "train_ms_per_pass": round(ct["train_ms"] / max(1, npass), 4)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +194,9 @@ def main():
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--spheres", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cache", action="store_true", help="skip the equal-time radiance-cache comparison")
+    ap.add_argument("--equal-seconds", type=float, default=0.0, help="equal-time budget (0: the timed run's)")
+    ap.add_argument("--judge-spp", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -178,6 +262,15 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
                 "bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(avg_s * 1e3, 4)}
 
+    pt.close()
+    eq = None
+    if not args.no_cache:
+        eq = equal_time(scene, W, H, args.depth, seconds=args.equal_seconds or elapsed, judge_spp=args.judge_spp)
+        log(f"equal time {eq['seconds']:.2f}s: plain relMSE {eq['plain']['relmse']:.4e} ({eq['plain']['passes']} "
+            f"passes), cache relMSE {eq['cache']['relmse']:.4e} ({eq['cache']['passes']} passes); "
+            f"cache reaches plain relMSE at {eq['cache_time_to_plain_relmse_frac']} of the time; "
+            f"mfma {eq['mfma']}")
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -187,7 +280,7 @@ def main():
             log(f"cpu baseline unavailable: {e}")
 
     out = {
-        "metric": "Msamples/s (plain GPU PT+MIS)",
+        "metric": "Msamples/s + relMSE at equal time: plain GPU PT vs radiance-cache PT, 1 MI355X",
         "value": round(value, 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
@@ -205,6 +298,7 @@ def main():
                                        "mis": round(mis / samples, 4)}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "relmse_equal_time": eq,
     }
     print(json.dumps(out), flush=True)
 

@@ -336,6 +336,11 @@ class RadianceCache:
         check(self._lib.c2_nrc_train_step(self._h, _fp(f), _fp(t), len(f), int(apply), _fp(g), C.byref(loss)))
         return g, loss.value
 
+    def stats(self) -> dict:
+        st = _abi.c2_nrc_stats()
+        check(self._lib.c2_nrc_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
     def close(self):
         if self._h:
             self._lib.c2_nrc_destroy(self._h)

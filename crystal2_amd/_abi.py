@@ -70,7 +70,8 @@ class c2_nrc_opts(C.Structure):
 
 
 class c2_nrc_stats(C.Structure):
-    _fields_ = [("steps", C.c_int64), ("flop_forward_per_query", C.c_double)]
+    _fields_ = [("steps", C.c_int64), ("flop_forward_per_query", C.c_double), ("queries", C.c_int64),
+                ("train_rows", C.c_int64)]
 
 
 class c2_cache_times(C.Structure):

@@ -1153,11 +1153,15 @@ int c2_pt_create(const c2_scene_desc* desc, const c2_render_opts* opts, c2_pt** 
         B.totals = A.alloc<unsigned long long>(4);
         HIP_CHECK(hipMemset(B.counters, 0, 8 * sizeof(int32_t)));
         HIP_CHECK(hipMemset(B.totals, 0, 4 * sizeof(unsigned long long)));
+        // The film always covers the whole frame: k_accumulate, c2_pt_reset and
+        // c2_pt_get_film index it by full-frame pixel (y * width + x), also when a
+        // pixel window restricts which pixels receive samples.
+        const int64_t filmPixels = (int64_t)opts->width * opts->height;
         if (opts->film_mode == C2_FILM_F64) {
-            pt->film64 = A.alloc<double4>(npix);
+            pt->film64 = A.alloc<double4>(filmPixels);
         } else {
-            pt->film = A.alloc<float4>(npix);
-            if (opts->film_mode == C2_FILM_F32_KAHAN) pt->comp = A.alloc<float4>(npix);
+            pt->film = A.alloc<float4>(filmPixels);
+            if (opts->film_mode == C2_FILM_F32_KAHAN) pt->comp = A.alloc<float4>(filmPixels);
         }
         HIP_CHECK(hipStreamCreateWithFlags(&pt->stream, hipStreamNonBlocking));
         *out = pt.release();

@@ -123,11 +123,13 @@ __device__ __forceinline__ void encode(const NrcDev& N, const float* raw, half4_
 __global__ void __launch_bounds__(kNrcBlock) k_nrc_infer(NrcDev N, const int32_t* __restrict__ list,
                                                          const int32_t* __restrict__ countp, int32_t nfixed,
                                                          const float4* __restrict__ feats, float4* __restrict__ L,
-                                                         const float4* __restrict__ beta, float4* __restrict__ out) {
+                                                         const float4* __restrict__ beta, float4* __restrict__ out,
+                                                         unsigned long long* tot) {
     __shared__ __align__(16) _Float16 sW[kNrcLayers * kNrcW * kNrcW];
     __shared__ float sB[kNrcLayers * kNrcW];
     stage_weights(N.wT, N.b, sW, sB);
     const int n = countp ? min(*countp, nfixed) : nfixed;
+    if (tot && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(tot, (unsigned long long)n);
     const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 4;
     for (int base = (blockIdx.x * 4 + wv) * 16; base < n; base += gridDim.x * 4 * 16) {
         int q = base + (l & 15);
@@ -299,8 +301,9 @@ __global__ void __launch_bounds__(kNrcBlock) k_nrc_wgrad(const int32_t* __restri
 
 // Sum the per-wavefront partials (mean over n rows) into grad [layer][in][out] + [layer][64].
 __global__ void k_nrc_reduce(const int32_t* __restrict__ countp, int32_t nfixed, const float* __restrict__ partial,
-                             int32_t maxWaves, float* grad) {
+                             int32_t maxWaves, float* grad, unsigned long long* tot) {
     const int n = countp ? min(*countp, nfixed) : nfixed;
+    if (tot && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(tot, (unsigned long long)n);
     const int used = min(maxWaves, (n + kRowsPerWave - 1) / kRowsPerWave);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kNrcLayers * kPartial) return;
@@ -376,6 +379,7 @@ struct c2_nrc {
     float* partial = nullptr;
     int32_t maxWaves = 0;
     float* loss = nullptr;
+    unsigned long long* totals = nullptr;  // inference queries, training rows
     int64_t step = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double infer_ms = 0, train_ms = 0, infer_flop = 0, train_flop = 0;
@@ -407,7 +411,7 @@ static int nrc_grid(int64_t rows) {
 void nrc_launch_infer(c2_nrc* c, const int32_t* list, const int32_t* countp, int32_t maxn, const float4* feats,
                       float4* L, const float4* beta, hipStream_t s) {
     hipLaunchKernelGGL(k_nrc_infer, dim3(nrc_grid(maxn)), dim3(kNrcBlock), 0, s, c->dev, list, countp, maxn, feats,
-                       L, beta, (float4*)nullptr);
+                       L, beta, (float4*)nullptr, c->totals);
 }
 
 void nrc_launch_train(c2_nrc* c, const float4* feats, const float4* targets, const int32_t* countp, int32_t maxn,
@@ -422,7 +426,7 @@ void nrc_launch_train(c2_nrc* c, const float4* feats, const float4* targets, con
                        c->D, rs, c->partial, c->maxWaves);
     int tot = kNrcLayers * kPartial;
     hipLaunchKernelGGL(k_nrc_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, countp, maxn, c->partial, c->maxWaves,
-                       c->grad);
+                       c->grad, c->totals + 1);
     if (adam) {
         c->step++;
         const c2_nrc_opts& o = c->opts;
@@ -458,6 +462,8 @@ int c2_nrc_create(const c2_nrc_opts* opts, c2_nrc** out) {
         c->maxWaves = (c->cap + kRowsPerWave - 1) / kRowsPerWave;
         c->partial = c->A.alloc<float>((size_t)kNrcLayers * c->maxWaves * kPartial);
         c->loss = c->A.alloc<float>(1);
+        c->totals = c->A.alloc<unsigned long long>(2);
+        NRC_CHECK(hipMemset(c->totals, 0, 2 * sizeof(unsigned long long)));
         NRC_CHECK(hipMemset(c->m1, 0, kNrcParams * sizeof(float)));
         NRC_CHECK(hipMemset(c->m2, 0, kNrcParams * sizeof(float)));
         NRC_CHECK(hipMemset(c->loss, 0, sizeof(float)));
@@ -512,7 +518,8 @@ int c2_nrc_infer(c2_nrc* c, const float* feats, int64_t n, float* out) {
         float4* dout = T.alloc<float4>(n);
         NRC_CHECK(hipMemcpy(df, feats, (size_t)n * kNrcFeat * sizeof(float), hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_nrc_infer, dim3(nrc_grid(n)), dim3(kNrcBlock), 0, 0, c->dev, (const int32_t*)nullptr,
-                           (const int32_t*)nullptr, (int32_t)n, df, (float4*)nullptr, (const float4*)nullptr, dout);
+                           (const int32_t*)nullptr, (int32_t)n, df, (float4*)nullptr, (const float4*)nullptr, dout,
+                           (unsigned long long*)nullptr);
         NRC_CHECK(hipGetLastError());
         std::vector<float4> o(n);
         NRC_CHECK(hipMemcpy(o.data(), dout, n * sizeof(float4), hipMemcpyDeviceToHost));
@@ -551,8 +558,12 @@ int c2_nrc_train_step(c2_nrc* c, const float* feats, const float* targets, int64
 int c2_nrc_get_stats(const c2_nrc* c, c2_nrc_stats* out) {
     return guard([&] {
         if (!c || !out) fail(C2_ERR_INVALID, "null argument");
+        unsigned long long t[2] = {0, 0};
+        NRC_CHECK(hipMemcpy(t, c->totals, sizeof(t), hipMemcpyDeviceToHost));
         out->steps = c->step;
         out->flop_forward_per_query = kNrcFlopFwd;
+        out->queries = (int64_t)t[0];
+        out->train_rows = (int64_t)t[1];
     });
 }
 

@@ -264,7 +264,9 @@ typedef struct c2_nrc_opts {
 
 typedef struct c2_nrc_stats {
     int64_t steps;
-    double  flop_forward_per_query;
+    double  flop_forward_per_query;   /* 33,152 */
+    int64_t queries;                  /* inference queries run in the renderer */
+    int64_t train_rows;               /* training rows consumed by Adam steps  */
 } c2_nrc_stats;
 
 #define C2_NRC_PARAMS (5 * 64 * 64 + 5 * 64)

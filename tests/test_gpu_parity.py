@@ -133,3 +133,21 @@ def test_raycast_full_size_subsample(gpu_available):
     sr[:, 7] = 1.0
     assert np.array_equal(bvh.intersect_test(sr), o.intersect_test(sr))
     assert 0.05 < (ids >= 0).mean() < 0.95
+
+
+def test_pixel_window_matches_full_frame(gpu_available):
+    """A pixel window renders exactly the full-frame pixels it covers (the film stays full size)."""
+    print("case pixel window", flush=True)
+    sc = load("cornell")
+    full = gpu_film(sc, 64, 64, 4, 5)
+    for film in ("f32", "f64"):
+        pt = c2.WavefrontPathIntegrator(sc, 64, 64, 4, 5, film=film, window=(8, 16, 24, 20))
+        pt.render()
+        pt.sync()
+        f = pt.film()
+        pt.close()
+        rows = slice(64 - 16 - 20, 64 - 16)  # film rows are top-down
+        if film == "f32":
+            assert np.array_equal(f[rows, 8:32], full[rows, 8:32])
+        else:
+            assert relmse(f[rows, 8:32], full[rows, 8:32]) < 1e-10
