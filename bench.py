@@ -83,6 +83,15 @@ def load_traffic(kernel):
     return t.get(kernel)
 
 
+def copy_calibration(mib, reps=3):
+    """A plain streaming kernel of known size (the library's k_stream_copy):
+    each launch reads exactly `mib` MiB and writes `mib` MiB."""
+    import crystal2_amd as c2
+    ms = c2.stream_copy(mib << 20, reps)
+    log(f"copy calibration: {reps} x k_stream_copy over {mib} MiB ({mib << 20} B read + written per launch), "
+        f"{ms:.3f} ms avg, {2 * (mib << 20) / ms / 1e6:.0f} GB/s")
+
+
 def cpu_baseline(scene, width, height, depth, seconds_budget=20.0):
     """Oracle (C++ port of the reference path) on a band of rows at sample 0."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
@@ -198,6 +207,9 @@ def main():
     ap.add_argument("--equal-seconds", type=float, default=0.0, help="equal-time budget (0: the timed run's)")
     ap.add_argument("--judge-spp", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain fp32 negate kernel over MIB MiB (reads MIB, writes MIB): "
+                         "the known-size reference that scales rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -263,6 +275,8 @@ def main():
                 "bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(avg_s * 1e3, 4)}
 
     pt.close()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     eq = None
     if not args.no_cache:
         eq = equal_time(scene, W, H, args.depth, seconds=args.equal_seconds or elapsed, judge_spp=args.judge_spp)

@@ -365,3 +365,12 @@ def halton_value(width: int, height: int, index: int, dim: int) -> float:
     out = C.c_float()
     check(lib.c2_halton_value(width, height, index, dim, C.byref(out)))
     return out.value
+
+
+def stream_copy(nbytes: int, reps: int = 3) -> float:
+    """Plain device copy over `nbytes` (read once, written once per launch);
+    returns the mean launch time in ms. The known-size kernel for HBM counters."""
+    lib = load_library()
+    out = C.c_double()
+    check(lib.c2_util_stream_copy(nbytes, reps, C.byref(out)))
+    return out.value

@@ -141,6 +141,7 @@ SIGNATURES = [
     ("c2_pt_get_cache_times", C.c_int, [P, C.POINTER(c2_cache_times)]),
     ("c2_halton_index", C.c_int, [C.c_int32] * 5 + [_i32p]),
     ("c2_halton_value", C.c_int, [C.c_int32] * 4 + [_f32p]),
+    ("c2_util_stream_copy", C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
 ]
 
 _lib = None

@@ -1509,3 +1509,53 @@ int c2_pt_get_stats(const c2_pt* pt, c2_pt_stats* out) {
 }
 
 }  // extern "C"
+
+// ============================================================================ HBM calibration copy
+
+namespace c2 {
+
+// Reads n4 float4 and writes n4 float4, each element exactly once.
+__global__ void __launch_bounds__(kBlock) k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                        int64_t n4) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+}  // namespace c2
+
+extern "C" int c2_util_stream_copy(int64_t bytes, int32_t reps, double* avg_ms) {
+    using namespace c2;
+    return guard([&] {
+        if (bytes <= 0 || bytes % 16 || bytes > ((int64_t)8 << 30) || reps <= 0 || !avg_ms)
+            fail(C2_ERR_INVALID, "bytes must be a positive multiple of 16 up to 8 GiB, reps > 0");
+        float4 *src = nullptr, *dst = nullptr;
+        HIP_CHECK(hipMalloc(&src, bytes));
+        if (hipMalloc(&dst, bytes) != hipSuccess) {
+            (void)hipFree(src);
+            fail(C2_ERR_DEVICE, "hipMalloc failed");
+        }
+        const int64_t n4 = bytes / 16;
+        const int grid = (int)std::min<int64_t>((n4 + kBlock - 1) / kBlock, 1 << 16);
+        hipEvent_t e0, e1;
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        hipError_t err = hipMemset(src, 0, bytes);
+        float ms = 0;
+        if (err == hipSuccess) {
+            hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kBlock), 0, nullptr, src, dst, n4);  // warm
+            (void)hipEventRecord(e0, nullptr);
+            for (int r = 0; r < reps; ++r)
+                hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kBlock), 0, nullptr, src, dst, n4);
+            (void)hipEventRecord(e1, nullptr);
+            err = hipEventSynchronize(e1);
+            if (err == hipSuccess) err = hipGetLastError();
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(src);
+        (void)hipFree(dst);
+        if (err != hipSuccess) fail(C2_ERR_DEVICE, std::string("stream copy: ") + hipGetErrorString(err));
+        *avg_ms = ms / reps;
+    });
+}

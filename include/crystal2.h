@@ -305,6 +305,15 @@ int  c2_pt_get_cache_times(c2_pt* pt, c2_cache_times* out);
 int  c2_halton_index(int32_t width, int32_t height, int32_t x, int32_t y, int32_t sample, int32_t* index);
 int  c2_halton_value(int32_t width, int32_t height, int32_t index, int32_t dim, float* value);
 
+/* ---------------------------------------------------------------- Utility
+ * No counterpart in the reference. A plain streaming copy over `bytes` bytes
+ * (read once, written once per launch; multiple of 16, at most 8 GiB) launched
+ * `reps` times after one warm-up launch, on the default stream; *avg_ms = mean
+ * launch time. It is the known-size kernel that calibrates HBM byte counters
+ * (FETCH_SIZE / WRITE_SIZE) and the copy bandwidth the renderer's kernels are
+ * compared against. Allocates and frees its own buffers. */
+int  c2_util_stream_copy(int64_t bytes, int32_t reps, double* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
