@@ -46,6 +46,7 @@ namespace c2 {
     } while (0)
 
 constexpr int kBlock = 256;
+constexpr int kMisGrid = 1024;  // blocks for the MIS-ray kernel (grid-stride; few rays)
 
 // ============================================================================ accelerator kernels
 
@@ -422,74 +423,89 @@ __global__ void __launch_bounds__(kBlock) k_cache_collect(PathBuffers B, PassPar
 }
 
 // Light loop of UniformSampleAllLights / EsimateDirect (:181-278), one thread per
-// (path, eligible light): the light sample (shadow slot p * E + j) and the test
-// whether the shared BSDF-sampled direction needs the MIS ray (Pdf_Li != 0).
+// path vertex looping over the eligible lights in light order: the vertex record
+// and its BSDF are loaded and built once, then per light j the light sample
+// (shadow slot p * E + j) and the test whether the shared BSDF-sampled direction
+// needs the MIS ray (Pdf_Li != 0).
 __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
     const int E = P.lmax;
-    const int j = blockIdx.y;
-    const int li = B.elig[j];
-    const DevLight Lt = S.lights[li];
-    const bool delta = Lt.type == C2_LIGHT_POINT;
+    const int NE = P.num_elig;
     int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + (int)threadIdx.x;
         bool live = i < n;
-        bool shadow = false, misHit = false;
         int p = live ? queue[i] : 0;
-        int slot = p * E + j;
-        if (live) {
+        bool nee = false;
+        if (live) nee = (B.meta[p].w & F_NEE) != 0;
+        Inter it{};
+        Bsdf bs{};
+        V3 wo{0, 0, 0}, misDir{0, 0, 0};
+        float uL0 = 0.f, uL1 = 0.f;
+        bool misCand = false;
+        if (nee) {
             int4 meta = B.meta[p];
+            int4 vi = B.vtx_i[p];
+            float4 vp = B.vtx_p[p], vn = B.vtx_n[p], vd = B.vtx_du[p], vdir = B.vtx_d[p];
+            it.p = V3{vp.x, vp.y, vp.z};
+            it.n = V3{vn.x, vn.y, vn.z};
+            it.dpdu = V3{vd.x, vd.y, vd.z};
+            it.dpdv = cross(it.n, it.dpdu);
+            it.dir = V3{vdir.x, vdir.y, vdir.z};
+            it.entity = vi.x;
+            it.material = vi.y;
+            it.light = vi.z;
+            it.front = vi.w != 0;
+            bs = make_bsdf(S, it, (uint32_t)meta.y, 0u);
+            wo = -it.dir;
+            uL0 = vp.w;
+            uL1 = vn.w;
+            misCand = __float_as_int(vd.w) != 0;
+            if (misCand) {
+                float4 mw = B.mis_w[p];
+                misDir = V3{mw.x, mw.y, mw.z};
+            }
+        }
+        bool misHit = false;
+        for (int j = 0; j < NE; j++) {
+            const int li = B.elig[j];
+            const DevLight Lt = S.lights[li];
+            const bool delta = Lt.type == C2_LIGHT_POINT;
+            const int slot = p * E + j;
+            bool shadow = false;
             uint8_t v = 0;
-            if (meta.w & F_NEE) {
-                int4 vi = B.vtx_i[p];
-                if (vi.z != li) {
-                    float4 vp = B.vtx_p[p], vn = B.vtx_n[p], vd = B.vtx_du[p], vdir = B.vtx_d[p];
-                    Inter it;
-                    it.p = V3{vp.x, vp.y, vp.z};
-                    it.n = V3{vn.x, vn.y, vn.z};
-                    it.dpdu = V3{vd.x, vd.y, vd.z};
-                    it.dpdv = cross(it.n, it.dpdu);
-                    it.dir = V3{vdir.x, vdir.y, vdir.z};
-                    it.entity = vi.x;
-                    it.material = vi.y;
-                    it.light = vi.z;
-                    it.front = vi.w != 0;
-                    Bsdf bs = make_bsdf(S, it, (uint32_t)meta.y, 0u);
-                    V3 wo = -it.dir;
-                    // (i) light sampling, :199-230
-                    V3 lightPos;
-                    float pdfL;
-                    V3 Li = sample_li(S, Lt, it.p, it.n, vp.w, vn.w, lightPos, pdfL);
-                    V3 wi = normalize(lightPos - it.p);
-                    if (pdfL != 0.f && !is_zero(Li)) {
-                        V3 f = bsdf_f(bs, it, wo, wi) * fmax_ref(0.f, dot(it.n, wi));
-                        float pB = bsdf_pdf(bs, it, wo, wi, T_ALL & ~T_SPEC);
-                        if (!is_zero(f)) {
-                            V3 C = delta ? f * Li / pdfL : f * Li * power_heuristic(pdfL, pB) / pdfL;
-                            bool back = dot(lightPos - it.p, it.n) < 0;  // SpawnRayTo
-                            V3 so = it.p + (back ? -it.n : it.n) * 1e-4f;
-                            V3 sd = lightPos - so;
-                            B.sh_o[slot] = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
-                            B.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, 0.f);
-                            B.sh_c[slot] = make_float4(C.x, C.y, C.z, 0.f);
-                            v = 2;
-                            shadow = true;
-                        }
-                    }
-                    // (ii) BSDF sampling needs the MIS ray when Pdf_Li != 0, :235-257
-                    if (!delta && __float_as_int(vd.w) != 0) {
-                        float4 mw = B.mis_w[p];
-                        if (pdf_light(S, Lt, it.p, it.n, V3{mw.x, mw.y, mw.z}) != 0.f) misHit = true;
+            if (nee && it.light != li) {
+                // (i) light sampling, :199-230
+                V3 lightPos;
+                float pdfL;
+                V3 Li = sample_li(S, Lt, it.p, it.n, uL0, uL1, lightPos, pdfL);
+                V3 wi = normalize(lightPos - it.p);
+                if (pdfL != 0.f && !is_zero(Li)) {
+                    V3 f = bsdf_f(bs, it, wo, wi) * fmax_ref(0.f, dot(it.n, wi));
+                    float pB = bsdf_pdf(bs, it, wo, wi, T_ALL & ~T_SPEC);
+                    if (!is_zero(f)) {
+                        V3 C = delta ? f * Li / pdfL : f * Li * power_heuristic(pdfL, pB) / pdfL;
+                        bool back = dot(lightPos - it.p, it.n) < 0;  // SpawnRayTo
+                        V3 so = it.p + (back ? -it.n : it.n) * 1e-4f;
+                        V3 sd = lightPos - so;
+                        B.sh_o[slot] = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
+                        B.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, 0.f);
+                        B.sh_c[slot] = make_float4(C.x, C.y, C.z, 0.f);
+                        v = 2;
+                        shadow = true;
                     }
                 }
+                // (ii) BSDF sampling needs the MIS ray when Pdf_Li != 0, :235-257
+                if (!delta && misCand && !misHit && pdf_light(S, Lt, it.p, it.n, misDir) != 0.f) misHit = true;
             }
-            B.sh_vis[slot] = v;
+            if (live) B.sh_vis[slot] = v;
+            int spos = wave_append(&B.counters[2], shadow);
+            if (spos >= 0) B.sh_list[spos] = slot;
         }
-        int spos = wave_append(&B.counters[2], shadow);
-        if (spos >= 0) B.sh_list[spos] = slot;
-        if (misHit && atomicOr(&B.misflag[p], 1) == 0) B.mis_list[atomicAdd(&B.counters[3], 1)] = p;
+        if (misHit) B.misflag[p] = 1;
+        int mpos = wave_append(&B.counters[3], misHit);
+        if (mpos >= 0) B.mis_list[mpos] = p;
     }
 }
 
@@ -1028,6 +1044,8 @@ struct c2_pt {
     float4* comp = nullptr;
     double4* film64 = nullptr;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // MIS rays, concurrent with the shadow rays
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int64_t launches = 0;
     int64_t samples = 0;
     // radiance cache
@@ -1035,6 +1053,21 @@ struct c2_pt {
     DeviceArena cache_arena;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> cache_events;  // 0 infer, 1 train
     int64_t cache_queries = 0, cache_rows = 0;
+
+    ~c2_pt() {
+        // drain both streams before the arenas release the buffers they use
+        if (side) (void)hipStreamSynchronize(side);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto* v : {&events, &cache_events})
+            for (auto& e : *v) {
+                (void)hipEventDestroy(e.second.first);
+                (void)hipEventDestroy(e.second.second);
+            }
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
 
 namespace c2 {
@@ -1164,20 +1197,16 @@ int c2_pt_create(const c2_scene_desc* desc, const c2_render_opts* opts, c2_pt** 
             if (opts->film_mode == C2_FILM_F32_KAHAN) pt->comp = A.alloc<float4>(filmPixels);
         }
         HIP_CHECK(hipStreamCreateWithFlags(&pt->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&pt->side, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&pt->ev_fork, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&pt->ev_join, hipEventDisableTiming));
         *out = pt.release();
         int rc2 = c2_pt_reset(*out);
         if (rc2 != C2_OK) fail(rc2, c2_last_error());
     });
 }
 
-void c2_pt_destroy(c2_pt* pt) {
-    if (!pt) return;
-    if (pt->stream) {
-        hipStreamSynchronize(pt->stream);
-        hipStreamDestroy(pt->stream);
-    }
-    delete pt;
-}
+void c2_pt_destroy(c2_pt* pt) { delete pt; }
 
 int c2_pt_reset(c2_pt* pt) {
     return guard([&] {
@@ -1297,22 +1326,30 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
                         }
                         if (P.num_elig > 0) {
                             KTimer kt(pt, C2_K_NEE, s);
-                            hipLaunchKernelGGL(k_nee, dim3(g, P.num_elig), dim3(kBlock), 0, s, S, pt->B, P, q);
+                            hipLaunchKernelGGL(k_nee, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
                         }
                         if (pt->cap_bounce == b && pt->cap_kind == 1) {
                             hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, pt->B, 1, q,
                                                P.lmax, pt->cap_n, pt->cap_rays, pt->cap_ignore,
                                                pt->B.counters + 6);
                         }
+                        // The rare MIS rays (closest hit) run on the side stream, concurrently
+                        // with the shadow rays; both only read what k_nee wrote, and k_resolve
+                        // waits for both.
+                        HIP_CHECK(hipEventRecord(pt->ev_fork, s));
+                        HIP_CHECK(hipStreamWaitEvent(pt->side, pt->ev_fork, 0));
+                        {
+                            KTimer kt(pt, C2_K_MIS, pt->side);
+                            hipLaunchKernelGGL(k_mis, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, pt->side, S,
+                                               pt->B);
+                        }
+                        HIP_CHECK(hipEventRecord(pt->ev_join, pt->side));
                         int gs = grid_for((int64_t)P.pix_count * P.lmax);
                         {
                             KTimer kt(pt, C2_K_OCCLUDE, s);
                             hipLaunchKernelGGL(k_occlude, dim3(gs), dim3(kBlock), 0, s, S, pt->B);
                         }
-                        {
-                            KTimer kt(pt, C2_K_MIS, s);
-                            hipLaunchKernelGGL(k_mis, dim3(g), dim3(kBlock), 0, s, S, pt->B);
-                        }
+                        HIP_CHECK(hipStreamWaitEvent(s, pt->ev_join, 0));
                         {
                             KTimer kt(pt, C2_K_RESOLVE, s);
                             hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kBlock), 0, s, pt->B, P, q);

@@ -240,7 +240,7 @@ __device__ __forceinline__ float fresnel(const Bsdf& b, float etaA, float etaB, 
     float sinI = sqrtf(fmax_ref(0.f, 1.f - cosI * cosI));
     float sinT = etaA / etaB * sinI;
     if (sinT >= 1.f) return 1.f;
-    if (b.fresnel == FR_SCHLICK) return 0.99f + (1.f - 0.99f) * powf_r(1.f - cosI, 5.f);
+    if (b.fresnel == FR_SCHLICK) return 0.99f + (1.f - 0.99f) * pow5_r(1.f - cosI);  // std::pow(x, 5.f)
     float cosT = sqrtf(fmax_ref(0.f, 1.f - sinT * sinT));
     float Rparl = ((etaB * cosI) - (etaA * cosT)) / ((etaB * cosI) + (etaA * cosT));
     float Rperp = ((etaA * cosI) - (etaB * cosT)) / ((etaA * cosI) + (etaB * cosT));

@@ -86,6 +86,14 @@ C2_HD float atan_r(float x) { return (float)atan((double)x); }
 C2_HD float acos_r(float x) { return (float)acos((double)x); }
 C2_HD float atan2_r(float y, float x) { return (float)atan2((double)y, (double)x); }
 C2_HD float powf_r(float x, float y) { return (float)pow((double)x, (double)y); }
+// x^5 for a float x: x^2 is exact in double, the two further products round once
+// each (relative error < 2^-51), then one rounding to float. Deterministic IEEE
+// operations, so host and device agree bit for bit.
+C2_HD float pow5_r(float x) {
+    double d = x;
+    double d2 = d * d;
+    return (float)(d2 * d2 * d);
+}
 C2_HD double powd(double x, double y) { return pow(x, y); }
 
 }  // namespace c2

@@ -41,13 +41,18 @@ inline float o_sin(float x) { return std::sin(x); }
 inline float o_cos(float x) { return std::cos(x); }
 inline float o_tan(float x) { return std::tan(x); }
 inline float o_atan(float x) { return std::atan(x); }
-inline float o_powf(float x, float y) { return std::pow(x, y); }
+inline float o_pow5(float x) { return std::pow(x, 5.f); }
 #else
 inline float o_sin(float x) { return (float)std::sin((double)x); }
 inline float o_cos(float x) { return (float)std::cos((double)x); }
 inline float o_tan(float x) { return (float)std::tan((double)x); }
 inline float o_atan(float x) { return (float)std::atan((double)x); }
-inline float o_powf(float x, float y) { return (float)std::pow((double)x, (double)y); }
+// x^5 as three double products (x^2 exact), rounded once to float: the GPU's form.
+inline float o_pow5(float x) {
+    double d = x;
+    double d2 = d * d;
+    return (float)(d2 * d2 * d);
+}
 #endif
 // std::pow(float, int) and std::pow(float, double) promote to double in both variants.
 inline double o_powd(double x, double y) { return std::pow(x, y); }
@@ -358,7 +363,7 @@ struct Fresnel {
         float sinI = std::sqrt(smax(0.f, 1.f - cosI * cosI));
         float sinT = etaA / etaB * sinI;
         if (sinT >= 1.f) return vec3(1.f);
-        if (kind == F_SCHLICK) return r0 + (vec3(1.f) - r0) * o_powf(1.f - cosI, 5.f);
+        if (kind == F_SCHLICK) return r0 + (vec3(1.f) - r0) * o_pow5(1.f - cosI);
         float cosT = std::sqrt(smax(0.f, 1.f - sinT * sinT));
         float Rparl = ((etaB * cosI) - (etaA * cosT)) / ((etaB * cosI) + (etaA * cosT));
         float Rperp = ((etaA * cosI) - (etaB * cosT)) / ((etaA * cosI) + (etaB * cosT));

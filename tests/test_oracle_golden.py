@@ -116,3 +116,17 @@ def test_permutation_tables_are_std_shuffle():
     # first table (base 5) is a permutation: the values at indices 1..4 are distinct
     vals = [c2.halton_value(256, 256, i, 2) for i in range(1, 5)]
     assert len(set(vals)) == 4
+
+
+def test_pow5_product_equals_rounded_double_pow():
+    """Schlick's x^5 is evaluated as three double products rounded once to float
+    (host and device alike); on 2^20 arguments in [0, 1] it equals pow in double
+    rounded to float."""
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.random(1 << 20).astype(np.float32),
+                        np.array([0, 1, 0.5, 1e-8, 0.999999], np.float32)])
+    d = x.astype(np.float64)
+    d2 = d * d
+    prod = (d2 * d2 * d).astype(np.float32)
+    ref = np.power(d, 5.0).astype(np.float32)
+    assert np.array_equal(prod, ref)
