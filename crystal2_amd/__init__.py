@@ -223,7 +223,7 @@ class WavefrontPathIntegrator:
     FILMS = {"f32": _abi.C2_FILM_F32, "kahan": _abi.C2_FILM_F32_KAHAN, "f64": _abi.C2_FILM_F64}
 
     def __init__(self, scene: Scene, width: int, height: int, spp: int, max_depth: int, arg_order: str = "rtl",
-                 film: str = "f32", paths_in_flight: int = 0, window=None):
+                 film: str = "f32", paths_in_flight: int = 0, window=None, queue_blocks: int = 0):
         self._lib = load_library()
         self.width, self.height, self.spp, self.max_depth = width, height, spp, max_depth
         o = _abi.c2_render_opts()
@@ -231,6 +231,7 @@ class WavefrontPathIntegrator:
         o.arg_order = _abi.C2_ARGORDER_LTR if arg_order == "ltr" else _abi.C2_ARGORDER_RTL
         o.film_mode = self.FILMS[film]
         o.paths_in_flight = paths_in_flight
+        o.queue_blocks = queue_blocks
         if window is not None:
             o.win_x0, o.win_y0, o.win_w, o.win_h = (int(v) for v in window)
         self.window = window

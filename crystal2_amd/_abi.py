@@ -59,7 +59,7 @@ class c2_accel_stats(C.Structure):
 class c2_render_opts(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
                 ("arg_order", C.c_int32), ("film_mode", C.c_int32), ("paths_in_flight", C.c_int32),
-                ("block_size", C.c_int32), ("win_x0", C.c_int32), ("win_y0", C.c_int32), ("win_w", C.c_int32),
+                ("queue_blocks", C.c_int32), ("win_x0", C.c_int32), ("win_y0", C.c_int32), ("win_w", C.c_int32),
                 ("win_h", C.c_int32)]
 
 

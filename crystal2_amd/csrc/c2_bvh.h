@@ -23,6 +23,7 @@
 // Vertices are stored unmodified so e1 = v1 - v0, e2 = v2 - v0 and the entity box
 // (component-wise min/max of v0, v1, v2) are recomputed bit-exactly in the kernel.
 #pragma once
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
@@ -40,6 +41,11 @@ struct alignas(128) Node4 {
     uint32_t pad[3];
 };
 static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
+// The kernels read a node as 8 float4: planes lo_x hi_x lo_y hi_y lo_z hi_z, then
+// child[4], then count (the near plane of axis a is float4 2a + (direction < 0)).
+static_assert(offsetof(Node4, hi_x) == 16 && offsetof(Node4, lo_y) == 32 && offsetof(Node4, hi_z) == 80 &&
+                  offsetof(Node4, child) == 96 && offsetof(Node4, count) == 112,
+              "Node4 plane layout");
 
 struct Box3 {
     float lo[3], hi[3];

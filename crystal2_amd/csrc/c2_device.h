@@ -62,29 +62,25 @@ C2_HD RayPre make_ray(V3 o, V3 d) {
     r.neg[2] = r.invD.z < 0;
     return r;
 }
-// RayBoxTest(ray, inv, invD, box, tMin, tMax), Geometry.h:136-157
+// RayBoxTest(ray, inv, invD, box, tMin, tMax), Geometry.h:136-157. The reference
+// updates tMin = std::max(tMin, a) and tMax = std::min(tMax, b) axis by axis and
+// returns false as soon as tMax < tMin. Here tMin and tMax are never NaN, so
+// std::max(tMin, a) (= tMin < a ? a : tMin) and std::min(tMax, b) equal IEEE
+// maxNum / minNum, which also ignore a NaN a or b; and later axes only shrink the
+// interval, so the early returns do not change the outcome: one test after the
+// three axes gives the same result and, when it passes, the same interval (up to
+// the sign of a zero, which no comparison sees). Ray tmin / tmax must not be NaN.
 C2_HD bool slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy, float hz,
                                      float& tMin, float& tMax) {
-    float a, b;
-    a = (lx - r.o.x) * r.invD.x;
-    b = (hx - r.o.x) * r.invD.x;
-    if (r.neg[0]) { float t = a; a = b; b = t; }
-    tMin = fmax_ref(tMin, a);
-    tMax = fmin_ref(tMax, b);
-    if (tMax < tMin) return false;
-    a = (ly - r.o.y) * r.invD.y;
-    b = (hy - r.o.y) * r.invD.y;
-    if (r.neg[1]) { float t = a; a = b; b = t; }
-    tMin = fmax_ref(tMin, a);
-    tMax = fmin_ref(tMax, b);
-    if (tMax < tMin) return false;
-    a = (lz - r.o.z) * r.invD.z;
-    b = (hz - r.o.z) * r.invD.z;
-    if (r.neg[2]) { float t = a; a = b; b = t; }
-    tMin = fmax_ref(tMin, a);
-    tMax = fmin_ref(tMax, b);
-    if (tMax < tMin) return false;
-    return true;
+    float ax = (lx - r.o.x) * r.invD.x, bx = (hx - r.o.x) * r.invD.x;
+    float ay = (ly - r.o.y) * r.invD.y, by = (hy - r.o.y) * r.invD.y;
+    float az = (lz - r.o.z) * r.invD.z, bz = (hz - r.o.z) * r.invD.z;
+    if (r.neg[0]) { float t = ax; ax = bx; bx = t; }
+    if (r.neg[1]) { float t = ay; ay = by; by = t; }
+    if (r.neg[2]) { float t = az; az = bz; bz = t; }
+    tMin = fmaxf(fmaxf(fmaxf(tMin, ax), ay), az);
+    tMax = fminf(fminf(fminf(tMax, bx), by), bz);
+    return !(tMax < tMin);
 }
 
 // ------------------------------------------------------------------------- triangle
@@ -185,46 +181,106 @@ __device__ __forceinline__ void load_prim(const DevScene& S, int slot, float4& a
 
 // Closest hit (ANY = false) or any hit (ANY = true). Closest hit: tmin = 0 and
 // tmax = ray tmax (the reference passes 0, +inf). Returns true on a hit.
+//
+// The traversal stack: the top K entries of each lane live in LDS (a [K][kTravBlock]
+// slab per block, slot-major, so the lanes of a wavefront touch consecutive words
+// and no two lanes share a bank beyond the wave64 halves); entries deeper than K
+// move to a private overflow array, which only deep traversals touch. The stack
+// never holds more than kStackSize entries (at most 3 pushes per level of a tree
+// at most kMaxBvh4Depth deep).
+constexpr int kTravBlock = 256;  // block size of every kernel that traverses
+#ifndef C2_TRAV_K_ANY
+#define C2_TRAV_K_ANY 16
+#endif
+#ifndef C2_TRAV_K_CLOSEST
+#define C2_TRAV_K_CLOSEST 8
+#endif
+template <bool ANY>
+struct TravK {
+    static constexpr int value = ANY ? C2_TRAV_K_ANY : C2_TRAV_K_CLOSEST;
+};
+// LDS backing of the stack for one block; declare as `__shared__ TravLds<ANY> lds;`.
+template <bool ANY>
+struct TravLds {
+    int32_t idx[TravK<ANY>::value * kTravBlock];
+    float t[ANY ? 1 : TravK<ANY>::value * kTravBlock];
+};
+
+template <bool ANY>
+struct TravStack {
+    static constexpr int K = TravK<ANY>::value;
+    static_assert((K & (K - 1)) == 0 && K > 0 && K <= kStackSize, "LDS stack depth must be a power of two");
+    int32_t* li;
+    float* lt;
+    int lane;
+    int sp;
+    int32_t oi[kStackSize - K];
+    float ot[ANY ? 1 : kStackSize - K];
+    __device__ __forceinline__ TravStack(TravLds<ANY>& L) : li(L.idx), lt(L.t), lane((int)threadIdx.x), sp(0) {}
+    __device__ __forceinline__ void push(int32_t e, float t) {
+        int off = (sp & (K - 1)) * kTravBlock + lane;
+        if (sp >= K) {  // evict the oldest resident entry (same ring slot) to the overflow array
+            oi[sp - K] = li[off];
+            if (!ANY) ot[sp - K] = lt[off];
+        }
+        li[off] = e;
+        if (!ANY) lt[off] = t;
+        sp++;
+    }
+    __device__ __forceinline__ void pop(int32_t& e, float& t) {
+        sp--;
+        int off = (sp & (K - 1)) * kTravBlock + lane;
+        e = li[off];
+        t = ANY ? 0.f : lt[off];
+        if (sp >= K) {  // refill the freed ring slot with the entry K below the new top
+            li[off] = oi[sp - K];
+            if (!ANY) lt[off] = ot[sp - K];
+        }
+    }
+};
+
 template <bool ANY>
 __device__ __forceinline__ bool traverse(const DevScene& S, V3 o, V3 d, float tmin, float tmax, int ignore,
-                                         Hit& hit) {
+                                         Hit& hit, TravLds<ANY>& lds) {
     RayPre r = make_ray(o, d);
-    int32_t stack[kStackSize];
-    float stackT[kStackSize];
-    int sp = 0;
+    const int negX = r.neg[0] ? 1 : 0, negY = r.neg[1] ? 1 : 0, negZ = r.neg[2] ? 1 : 0;
+    TravStack<ANY> st(lds);
     int32_t cur = 0;
     float best = INFINITY;
     int32_t bestE = -1;
     float bestU = 0.f, bestV = 0.f;
     while (true) {
         if (cur >= 0) {
-            const Node4* nd = S.nodes + cur;
-            float4 lx = *reinterpret_cast<const float4*>(nd->lo_x);
-            float4 hx = *reinterpret_cast<const float4*>(nd->hi_x);
-            float4 ly = *reinterpret_cast<const float4*>(nd->lo_y);
-            float4 hy = *reinterpret_cast<const float4*>(nd->hi_y);
-            float4 lz = *reinterpret_cast<const float4*>(nd->lo_z);
-            float4 hz = *reinterpret_cast<const float4*>(nd->hi_z);
-            int4 ch = *reinterpret_cast<const int4*>(nd->child);
-            uint32_t cnt = nd->count;
+            // The same slab arithmetic for the four children at once. The near plane
+            // of an axis is the child's hi plane when the direction is negative: it is
+            // picked by address (float4 index 2 * axis + neg), which is the reference's
+            // swap of the two plane distances. Empty slots hold inverted boxes
+            // (+inf / -inf), whose interval is empty for every direction.
+            const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
+            float4 nx = nf[0 + negX], fx = nf[1 - negX];
+            float4 ny = nf[2 + negY], fy = nf[3 - negY];
+            float4 nz = nf[4 + negZ], fz = nf[5 - negZ];
+            int4 ch = *reinterpret_cast<const int4*>(nf + 6);
+            uint32_t cnt = *reinterpret_cast<const uint32_t*>(nf + 7);
             float lim = ANY ? tmax : fmin_ref(tmax, best);
             float k0 = INFINITY, k1 = INFINITY, k2 = INFINITY, k3 = INFINITY;
             int32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
-#define C2_CHILD(I, LXC, HXC, LYC, HYC, LZC, HZC, CH, KOUT, EOUT)                                     \
+#define C2_CHILD(I, C, CH, KOUT, EOUT)                                                                \
     {                                                                                                \
-        uint32_t c8 = (cnt >> (8 * I)) & 0xffu;                                                       \
-        if (CH >= 0 || c8 != 0) {                                                                    \
-            float t0 = tmin, t1 = lim;                                                               \
-            if (slab(r, LXC, LYC, LZC, HXC, HYC, HZC, t0, t1)) {                                     \
-                KOUT = t0;                                                                           \
-                EOUT = CH >= 0 ? CH : ~((int32_t)((uint32_t)(~CH) | ((c8 - 1u) << kSlotBits)));      \
-            }                                                                                        \
+        float tn = fmaxf(fmaxf(fmaxf(tmin, (nx.C - r.o.x) * r.invD.x), (ny.C - r.o.y) * r.invD.y),      \
+                         (nz.C - r.o.z) * r.invD.z);                                                 \
+        float tf = fminf(fminf(fminf(lim, (fx.C - r.o.x) * r.invD.x), (fy.C - r.o.y) * r.invD.y),       \
+                         (fz.C - r.o.z) * r.invD.z);                                                 \
+        if (!(tf < tn)) {                                                                            \
+            uint32_t c8 = (cnt >> (8 * I)) & 0xffu;                                                  \
+            KOUT = tn;                                                                               \
+            EOUT = CH >= 0 ? CH : ~((int32_t)((uint32_t)(~CH) | ((c8 - 1u) << kSlotBits)));          \
         }                                                                                            \
     }
-            C2_CHILD(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x, k0, e0)
-            C2_CHILD(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y, k1, e1)
-            C2_CHILD(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z, k2, e2)
-            C2_CHILD(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w, k3, e3)
+            C2_CHILD(0, x, ch.x, k0, e0)
+            C2_CHILD(1, y, ch.y, k1, e1)
+            C2_CHILD(2, z, ch.z, k2, e2)
+            C2_CHILD(3, w, ch.w, k3, e3)
 #undef C2_CHILD
             // sort the four (key, entry) pairs ascending; misses carry +inf keys
 #define C2_CSWAP(KA, EA, KB, EB)                       \
@@ -238,9 +294,9 @@ __device__ __forceinline__ bool traverse(const DevScene& S, V3 o, V3 d, float tm
             C2_CSWAP(k1, e1, k3, e3)
             C2_CSWAP(k1, e1, k2, e2)
 #undef C2_CSWAP
-            if (k3 != INFINITY) { stack[sp] = e3; stackT[sp] = k3; sp++; }
-            if (k2 != INFINITY) { stack[sp] = e2; stackT[sp] = k2; sp++; }
-            if (k1 != INFINITY) { stack[sp] = e1; stackT[sp] = k1; sp++; }
+            if (k3 != INFINITY) st.push(e3, k3);
+            if (k2 != INFINITY) st.push(e2, k2);
+            if (k1 != INFINITY) st.push(e1, k1);
             if (k0 != INFINITY) {
                 cur = e0;
                 continue;
@@ -304,7 +360,7 @@ __device__ __forceinline__ bool traverse(const DevScene& S, V3 o, V3 d, float tm
         // pop; a closest-hit entry whose entry distance is already beyond the best hit
         // cannot contain a closer one (its slab interval starts after `best`).
         while (true) {
-            if (sp == 0) {
+            if (st.sp == 0) {
                 if (ANY) return false;
                 hit.entity = bestE;
                 hit.t = best;
@@ -312,9 +368,11 @@ __device__ __forceinline__ bool traverse(const DevScene& S, V3 o, V3 d, float tm
                 hit.v = bestV;
                 return bestE >= 0;
             }
-            sp--;
-            if (!ANY && stackT[sp] > best) continue;
-            cur = stack[sp];
+            int32_t e;
+            float et;
+            st.pop(e, et);
+            if (!ANY && et > best) continue;
+            cur = e;
             break;
         }
     }

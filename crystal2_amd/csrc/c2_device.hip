@@ -46,17 +46,23 @@ namespace c2 {
     } while (0)
 
 constexpr int kBlock = 256;
+static_assert(kBlock == kTravBlock, "traversal kernels index their LDS stack by threadIdx.x");
 constexpr int kMisGrid = 1024;  // blocks for the MIS-ray kernel (grid-stride; few rays)
+// Default grid cap of the per-bounce queue kernels. Their live counts are only known
+// on the device, so they are launched for the upper bound with grid-stride loops;
+// the cap keeps a shrinking queue from paying for tens of thousands of empty blocks.
+constexpr int kQueueBlocksDefault = 8192;
 
 // ============================================================================ accelerator kernels
 
 __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, const float* __restrict__ rays, int64_t n,
                                                       int32_t* __restrict__ prim, float* __restrict__ tout) {
+    __shared__ TravLds<false> lds;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
         float4 a = r4[0], b = r4[1];
         Hit h;
-        bool hit = traverse<false>(S, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h);
+        bool hit = traverse<false>(S, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h, lds);
         prim[i] = hit ? h.entity : -1;
         tout[i] = hit ? h.t : INFINITY;
     }
@@ -65,11 +71,12 @@ __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, const float* _
 __global__ void __launch_bounds__(kBlock) k_intersect_test(DevScene S, const float* __restrict__ rays,
                                                            const int32_t* __restrict__ ignore, int64_t n,
                                                            uint8_t* __restrict__ out) {
+    __shared__ TravLds<true> lds;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
         float4 a = r4[0], b = r4[1];
         Hit h;
-        bool hit = traverse<true>(S, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ignore ? ignore[i] : -1, h);
+        bool hit = traverse<true>(S, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ignore ? ignore[i] : -1, h, lds);
         out[i] = hit ? 1 : 0;
     }
 }
@@ -241,6 +248,7 @@ __global__ void k_reset(PathBuffers B, int next_q) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, int q) {
+    __shared__ TravLds<false> lds;
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[0], (unsigned long long)n);
@@ -248,7 +256,7 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, in
         int p = queue[i];
         float4 o = B.ray_o[p], d = B.ray_d[p];
         Hit h;
-        bool hit = traverse<false>(S, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY, -1, h);
+        bool hit = traverse<false>(S, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY, -1, h, lds);
         B.hit[p] = make_float4(h.t, h.u, h.v, __int_as_float(hit ? h.entity : -1));
     }
 }
@@ -510,6 +518,7 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
 }
 
 __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
+    __shared__ TravLds<true> lds;
     int n = B.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[1], (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -517,7 +526,7 @@ __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
         float4 o = B.sh_o[s], d = B.sh_d[s];
         Hit h;
         bool occluded =
-            traverse<true>(S, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f, __float_as_int(o.w), h);
+            traverse<true>(S, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f, __float_as_int(o.w), h, lds);
         B.sh_vis[s] = occluded ? 0 : 1;
     }
 }
@@ -525,6 +534,7 @@ __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
 // The shared MIS ray: Scene::Intersect(isec.SpawnRay(wi)); only the light owning the
 // hit entity can receive f * Le * w / pdf_bsdf, :255-273
 __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B) {
+    __shared__ TravLds<false> lds;
     int n = B.counters[3];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[2], (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -537,7 +547,7 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B) {
         Hit h;
         int lh = -1;
         V3 Lb{0, 0, 0};
-        if (traverse<false>(S, o, wi, 0.f, INFINITY, -1, h)) {
+        if (traverse<false>(S, o, wi, 0.f, INFINITY, -1, h, lds)) {
             int4 e = S.entities[h.entity];
             int li = e.w;
             if (li >= 0 && li != own) {
@@ -1088,6 +1098,8 @@ int c2_pt_create(const c2_scene_desc* desc, const c2_render_opts* opts, c2_pt** 
             fail(C2_ERR_INVALID, "render options: bad arg_order");
         if (opts->film_mode < C2_FILM_F32 || opts->film_mode > C2_FILM_F64)
             fail(C2_ERR_INVALID, "render options: bad film_mode");
+        if (opts->queue_blocks < 0 || opts->paths_in_flight < 0)
+            fail(C2_ERR_INVALID, "render options: negative queue_blocks / paths_in_flight");
         std::unique_ptr<c2_pt> pt(new c2_pt());
         pt->opts = *opts;
         c2_accel* a = nullptr;
@@ -1290,7 +1302,8 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
             for (int64_t beg = 0; beg < npix; beg += pt->chunk) {
                 P.pix_begin = (int32_t)beg;
                 P.pix_count = (int32_t)std::min<int64_t>(pt->chunk, npix - beg);
-                int g = grid_for(P.pix_count);
+                const int qcap = pt->opts.queue_blocks > 0 ? pt->opts.queue_blocks : kQueueBlocksDefault;
+                int g = std::min(grid_for(P.pix_count), qcap);
                 if (P.cache_bounce >= 0) hipLaunchKernelGGL(k_chunk_reset, dim3(1), dim3(1), 0, s, pt->B);
                 {
                     KTimer kt(pt, C2_K_GENERATE, s);
@@ -1344,7 +1357,7 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
                                                pt->B);
                         }
                         HIP_CHECK(hipEventRecord(pt->ev_join, pt->side));
-                        int gs = grid_for((int64_t)P.pix_count * P.lmax);
+                        int gs = std::min(grid_for((int64_t)P.pix_count * P.lmax), qcap);
                         {
                             KTimer kt(pt, C2_K_OCCLUDE, s);
                             hipLaunchKernelGGL(k_occlude, dim3(gs), dim3(kBlock), 0, s, S, pt->B);

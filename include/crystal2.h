@@ -204,7 +204,8 @@ typedef struct c2_render_opts {
                                  PathTracingIntegrator.cpp:79 (RTL = GCC/MSVC)       */
     int32_t film_mode;        /* C2_FILM_*                                             */
     int32_t paths_in_flight;  /* 0 = all pixels of one sample pass at once             */
-    int32_t block_size;       /* reserved, 0                                           */
+    int32_t queue_blocks;     /* grid cap (blocks) of the per-bounce queue kernels,
+                                 which loop grid-stride; 0 = the library default     */
     int32_t win_x0, win_y0;   /* pixel window (film coordinates, y = 0 at the bottom); */
     int32_t win_w, win_h;     /* win_w = 0: the whole film                             */
 } c2_render_opts;
