@@ -38,20 +38,8 @@ def log(*a):
 
 def build_scene(n_spheres=400, level=4, seed=2024):
     """The sphere-field scene in memory (same draws as crystal2_amd.scenes.spheres_scene)."""
-    import crystal2_amd as c2
     from crystal2_amd import scenes
-    d = scenes.spheres_scene_objects(n_spheres, level, seed)
-    sc = c2.Scene.create()
-    for (p, n, f, mat, le) in d["meshes"]:
-        if mat is None:
-            m = -1
-        else:
-            mt = {"Diffuse": 0, "Mirror": 1, "Glass": 2, "Micro": 3, "Phong": 4}[mat["Type"]]
-            m = sc.add_material(mt, mat["Color"], mat.get("Roughness", (0, 0)), mat.get("IOR", 1.0), mat.get("K", 0))
-        sc.add_mesh(p, n, f, m, lemit=le)
-    cam = d["camera"]
-    sc.set_camera(cam["Position"], cam["LookAt"], cam["Up"], cam["FOV"])
-    return sc
+    return scenes.build_spheres_scene(n_spheres, level, seed)
 
 
 def scene_bounds(scene):

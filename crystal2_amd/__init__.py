@@ -297,6 +297,68 @@ class WavefrontPathIntegrator:
             pass
 
 
+class BatchRenderer:
+    """Many scenes rendered together (no counterpart in the reference, which renders
+    one scene per run): sample pass k traces every pixel of every scene with
+    spp > k in shared launches; ``render(use_graph=True)`` replays the whole render
+    as one captured hipGraph. Each film equals the scene's own
+    ``WavefrontPathIntegrator(scene, w, h, spp, depth).render()`` bit for bit.
+
+    ``items``: sequence of (Scene, width, height, spp).
+    """
+
+    def __init__(self, items, max_depth: int, arg_order: str = "rtl", film: str = "f32",
+                 paths_in_flight: int = 0, queue_blocks: int = 0):
+        self._lib = load_library()
+        self.items = [(sc, int(w), int(h), int(spp)) for sc, w, h, spp in items]
+        self._descs = [sc.desc() for sc, _, _, _ in self.items]  # kept alive for the call
+        arr = (_abi.c2_batch_item * len(self.items))()
+        for i, ((sc, w, h, spp), d) in enumerate(zip(self.items, self._descs)):
+            arr[i].scene = C.cast(C.pointer(d), C.c_void_p)
+            arr[i].width, arr[i].height, arr[i].spp = w, h, spp
+        o = _abi.c2_render_opts()
+        o.max_depth = max_depth
+        o.arg_order = _abi.C2_ARGORDER_LTR if arg_order == "ltr" else _abi.C2_ARGORDER_RTL
+        o.film_mode = WavefrontPathIntegrator.FILMS[film]
+        o.paths_in_flight = paths_in_flight
+        o.queue_blocks = queue_blocks
+        self._h = C.c_void_p()
+        check(self._lib.c2_batch_create(arr, len(self.items), C.byref(o), C.byref(self._h)))
+
+    def autotune(self) -> dict:
+        t = _abi.c2_batch_tuning()
+        check(self._lib.c2_batch_autotune(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def render(self, use_graph: bool = True):
+        check(self._lib.c2_batch_render(self._h, int(use_graph)))
+
+    def sync(self):
+        check(self._lib.c2_batch_sync(self._h))
+
+    def film(self, i: int) -> np.ndarray:
+        _, w, h, _ = self.items[i]
+        out = np.empty((h, w, 3), np.float32)
+        check(self._lib.c2_batch_get_film(self._h, i, _fp(out)))
+        return out
+
+    def stats(self) -> dict:
+        s = _abi.c2_batch_stats()
+        check(self._lib.c2_batch_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def close(self):
+        if self._h:
+            self._lib.c2_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class RadianceCache:
     """Online-trained MLP radiance cache (FP16 MFMA forward/backward, FP32 Adam)."""
 

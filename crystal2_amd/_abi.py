@@ -87,6 +87,21 @@ class c2_pt_stats(C.Structure):
                 ("mis_rays", C.c_int64), ("kernel_launches", C.c_int64)]
 
 
+class c2_batch_item(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32)]
+
+
+class c2_batch_tuning(C.Structure):
+    _fields_ = [("paths_in_flight", C.c_int32), ("queue_blocks", C.c_int32), ("pass_ms", C.c_double),
+                ("candidates", C.c_int32)]
+
+
+class c2_batch_stats(C.Structure):
+    _fields_ = [("samples", C.c_int64), ("samples_per_render", C.c_int64), ("extension_rays", C.c_int64),
+                ("shadow_rays", C.c_int64), ("mis_rays", C.c_int64), ("passes", C.c_int32),
+                ("paths_in_flight", C.c_int32), ("queue_blocks", C.c_int32), ("graph_nodes", C.c_int64)]
+
+
 class c2_kernel_times(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_int64 * 8)]
 
@@ -142,6 +157,13 @@ SIGNATURES = [
     ("c2_halton_index", C.c_int, [C.c_int32] * 5 + [_i32p]),
     ("c2_halton_value", C.c_int, [C.c_int32] * 4 + [_f32p]),
     ("c2_util_stream_copy", C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    ("c2_batch_create", C.c_int, [C.POINTER(c2_batch_item), C.c_int32, C.POINTER(c2_render_opts), C.POINTER(P)]),
+    ("c2_batch_destroy", None, [P]),
+    ("c2_batch_autotune", C.c_int, [P, C.POINTER(c2_batch_tuning)]),
+    ("c2_batch_render", C.c_int, [P, C.c_int32]),
+    ("c2_batch_sync", C.c_int, [P]),
+    ("c2_batch_get_film", C.c_int, [P, C.c_int32, _f32p]),
+    ("c2_batch_get_stats", C.c_int, [P, C.POINTER(c2_batch_stats)]),
 ]
 
 _lib = None

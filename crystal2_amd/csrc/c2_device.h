@@ -239,13 +239,17 @@ struct TravStack {
     }
 };
 
+// root: Node4 index where the traversal starts (a batch keeps every scene's tree in
+// one node array); a negative root is a scene without entities (always a miss).
 template <bool ANY>
-__device__ __forceinline__ bool traverse(const DevScene& S, V3 o, V3 d, float tmin, float tmax, int ignore,
-                                         Hit& hit, TravLds<ANY>& lds) {
+__device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, V3 d, float tmin, float tmax,
+                                         int ignore, Hit& hit, TravLds<ANY>& lds) {
+    hit.entity = -1;
+    if (root < 0) return false;
     RayPre r = make_ray(o, d);
     const int negX = r.neg[0] ? 1 : 0, negY = r.neg[1] ? 1 : 0, negZ = r.neg[2] ? 1 : 0;
     TravStack<ANY> st(lds);
-    int32_t cur = 0;
+    int32_t cur = root;
     float best = INFINITY;
     int32_t bestE = -1;
     float bestU = 0.f, bestV = 0.f;

@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, const float* _
         const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
         float4 a = r4[0], b = r4[1];
         Hit h;
-        bool hit = traverse<false>(S, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h, lds);
+        bool hit = traverse<false>(S, 0, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h, lds);
         prim[i] = hit ? h.entity : -1;
         tout[i] = hit ? h.t : INFINITY;
     }
@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect_test(DevScene S, const flo
         const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
         float4 a = r4[0], b = r4[1];
         Hit h;
-        bool hit = traverse<true>(S, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ignore ? ignore[i] : -1, h, lds);
+        bool hit = traverse<true>(S, 0, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ignore ? ignore[i] : -1, h, lds);
         out[i] = hit ? 1 : 0;
     }
 }
@@ -97,7 +97,8 @@ struct PathBuffers {
     float4* beta;    // beta at the current vertex
     float4* beta2;   // beta after the continuation (applied by resolve)
     float4* L;       // path radiance
-    int4* meta;      // pixel, halton index, dimension, flags
+    int4* meta;      // film pixel (all views' films concatenated), halton index, dimension, flags
+    int32_t* vid;    // view (scene of a batch) of the path
     float4* hit;     // t, u, v, entity bits
     float4* vtx_p;   // vertex position, uL.x
     float4* vtx_n;   // facing shading normal, uL.y
@@ -130,19 +131,31 @@ struct PathBuffers {
     unsigned long long* totals;  // extension, shadow, mis rays (64-bit running totals)
 };
 
-struct PassParams {
+// One rendered image: a scene with its camera, film and sampler. A single renderer
+// has one view; a batch has one per scene, all sharing one concatenated scene.
+struct View {
     int32_t width, height;
-    int32_t sample;          // Halton sample number k
-    int32_t pix_begin, pix_count;
-    int32_t max_depth;
-    int32_t arg_order;
-    int32_t lmax;            // shadow slots per path (>= 1)
-    int32_t num_elig;        // eligible lights (Flux != 0)
+    int32_t root;            // Node4 index of the scene's BVH root, -1 = no entities
+    int32_t elig_begin;      // eligible lights (Flux != 0): elig[elig_begin, + num_elig)
+    int32_t num_elig;
     int32_t nd_elig;         // eligible non-delta lights (one Get1D each per vertex)
     int32_t stride0, exp0, scale1, mult0, mult1, bscale0, bexp1;  // Halton constants
+    int32_t win_x0, win_y0, win_w;  // pixel window: linear index i -> (x0 + i % w, y0 + i / w)
+    int32_t pad_;
+    int64_t film_off;        // first pixel of this view's film in the concatenated film
     V3 eye, look, up, right;
     float camW, camH;
-    int32_t win_x0, win_y0, win_w;  // pixel window: linear index i -> (x0 + i % w, y0 + i / w)
+};
+
+struct PassParams {
+    int32_t sample;          // Halton sample number k
+    int32_t pix_begin, pix_count;  // linear pass indices of this chunk
+    int32_t max_depth;
+    int32_t arg_order;
+    int32_t lmax;            // shadow slots per path: the largest eligible-light count (>= 1)
+    const View* views;
+    const int2* pv;          // views of this pass: (view, first linear index), ascending
+    int32_t npv;
     int32_t cache_bounce;            // radiance cache: -1 off
     int32_t train_ratio, train_phase;
     int32_t train_cap;               // training rows per step
@@ -167,7 +180,7 @@ __device__ __forceinline__ void cache_features(const DevScene& S, const Inter& i
     out[3] = make_float4(alb.z, 0.f, 0.f, 0.f);
 }
 
-__device__ __forceinline__ int32_t halton_index_dev(const PassParams& P, int x, int y, int k) {
+__device__ __forceinline__ int32_t halton_index_dev(const View& P, int x, int y, int k) {
     // HaltonSampler::GetVectorIndexForSample, HaltonSampler.cpp:251-270
     int64_t off = 0;
     if (P.stride0 > 1) {
@@ -197,29 +210,39 @@ __device__ __forceinline__ int32_t wave_append(int32_t* counter, bool pred) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, PassParams P) {
-    HaltonDev hd{P.exp0, P.scale1};
     int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < P.pix_count; base += stride) {
         int i = base + (int)threadIdx.x;
         bool live = i < P.pix_count;
         int lin = P.pix_begin + i;
-        int x = P.win_x0 + lin % P.win_w, y = P.win_y0 + lin / P.win_w;
-        int pixel = y * P.width + x;
+        // the view owning linear index lin: last entry with first index <= lin
+        int lo = 0, hi = P.npv - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (P.pv[mid].y <= lin) lo = mid;
+            else hi = mid - 1;
+        }
+        const int v = P.pv[lo].x;
+        const View& V = P.views[v];
+        lin -= P.pv[lo].y;
+        int x = V.win_x0 + lin % V.win_w, y = V.win_y0 + lin / V.win_w;
+        int pixel = y * V.width + x;
         bool train = live && P.cache_bounce >= 0 && (pixel % P.train_ratio) == P.train_phase;
         int tp = wave_append(&B.counters[5], train);
         if (tp >= P.train_paths) train = false;  // beyond the record capacity: an ordinary path
         if (!live) continue;
-        int32_t idx = halton_index_dev(P, x, y, P.sample);
+        HaltonDev hd{V.exp0, V.scale1};
+        int32_t idx = halton_index_dev(V, x, y, P.sample);
         float s0 = halton_value(S, hd, idx, 0), s1 = halton_value(S, hd, idx, 1);
         float px = (float)x + s0, py = (float)y + s1;
-        px = px / P.width;
-        py = py / P.height;
+        px = px / V.width;
+        py = py / V.height;
         // PerspCamera::GenerateRay
         px = px * 2 - 1;
         py = py * 2 - 1;
-        px *= P.camW, py *= P.camH;
-        V3 d = normalize(P.right * px + P.up * py + P.look * 1.0f);
-        B.ray_o[i] = make_float4(P.eye.x, P.eye.y, P.eye.z, 0.f);
+        px *= V.camW, py *= V.camH;
+        V3 d = normalize(V.right * px + V.up * py + V.look * 1.0f);
+        B.ray_o[i] = make_float4(V.eye.x, V.eye.y, V.eye.z, 0.f);
         B.ray_d[i] = make_float4(d.x, d.y, d.z, 0.f);
         B.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
         B.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -229,7 +252,8 @@ __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, 
             B.tlist[tp] = i;
             for (int b = 0; b < P.max_depth; b++) B.trec_valid[(int64_t)tp * P.max_depth + b] = 0;
         }
-        B.meta[i] = make_int4(pixel, idx, 2, fl);
+        B.meta[i] = make_int4((int32_t)(V.film_off + pixel), idx, 2, fl);
+        B.vid[i] = v;
         B.queue[0][i] = i;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) B.counters[0] = P.pix_count;
@@ -247,7 +271,7 @@ __global__ void k_reset(PathBuffers B, int next_q) {
     B.counters[3] = 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, int q) {
+__global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, PassParams P, int q) {
     __shared__ TravLds<false> lds;
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
@@ -255,8 +279,9 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, in
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int p = queue[i];
         float4 o = B.ray_o[p], d = B.ray_d[p];
+        const int root = P.views[B.vid[p]].root;
         Hit h;
-        bool hit = traverse<false>(S, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY, -1, h, lds);
+        bool hit = traverse<false>(S, root, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY, -1, h, lds);
         B.hit[p] = make_float4(h.t, h.u, h.v, __int_as_float(hit ? h.entity : -1));
     }
 }
@@ -267,7 +292,6 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
     int32_t* next = B.queue[q ^ 1];
-    HaltonDev hd{P.exp0, P.scale1};
     // grid-stride loop over block-uniform bases: every lane of a wavefront runs the
     // same iterations, as wave_append's ballot requires
     int stride = gridDim.x * blockDim.x;
@@ -278,6 +302,8 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
         int p = live ? queue[i] : 0;
         if (live) {
             int4 meta = B.meta[p];
+            const View& V = P.views[B.vid[p]];
+            const HaltonDev hd{V.exp0, V.scale1};
             float4 hh = B.hit[p];
             int ent = __float_as_int(hh.w);
             int flags = 0;
@@ -330,7 +356,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                     float uB0 = halton_value(S, hd, idx, dim + 2), uB1 = halton_value(S, hd, idx, dim + 3);
                     int own = it.light;
                     int ownElig = (own >= 0 && S.lights[own].eligible) ? 1 : 0;
-                    int nGet1D = P.nd_elig - ownElig;  // one Get1D per eligible non-delta light, :240
+                    int nGet1D = V.nd_elig - ownElig;  // one Get1D per eligible non-delta light, :240
                     V3 misF{0, 0, 0}, wiB{0, 0, 0};
                     float pdfB = 0.f;
                     int fBnz = 0;
@@ -439,14 +465,20 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
     const int E = P.lmax;
-    const int NE = P.num_elig;
     int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + (int)threadIdx.x;
         bool live = i < n;
         int p = live ? queue[i] : 0;
         bool nee = false;
-        if (live) nee = (B.meta[p].w & F_NEE) != 0;
+        int eb = 0, ne = 0, root = -1;
+        if (live) {
+            nee = (B.meta[p].w & F_NEE) != 0;
+            const View& V = P.views[B.vid[p]];
+            eb = V.elig_begin;
+            ne = V.num_elig;
+            root = V.root;
+        }
         Inter it{};
         Bsdf bs{};
         V3 wo{0, 0, 0}, misDir{0, 0, 0};
@@ -476,14 +508,17 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
             }
         }
         bool misHit = false;
-        for (int j = 0; j < NE; j++) {
-            const int li = B.elig[j];
-            const DevLight Lt = S.lights[li];
-            const bool delta = Lt.type == C2_LIGHT_POINT;
+        // wave-uniform trip count (wave_append below needs every lane); lanes whose
+        // view has fewer eligible lights idle through the tail
+        for (int j = 0; j < E; j++) {
+            const bool lj = j < ne;
+            const int li = lj ? B.elig[eb + j] : -1;
+            const bool delta = lj && S.lights[li].type == C2_LIGHT_POINT;
             const int slot = p * E + j;
             bool shadow = false;
             uint8_t v = 0;
-            if (nee && it.light != li) {
+            if (nee && lj && it.light != li) {
+                const DevLight Lt = S.lights[li];
                 // (i) light sampling, :199-230
                 V3 lightPos;
                 float pdfL;
@@ -498,7 +533,7 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
                         V3 so = it.p + (back ? -it.n : it.n) * 1e-4f;
                         V3 sd = lightPos - so;
                         B.sh_o[slot] = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
-                        B.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, 0.f);
+                        B.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, __int_as_float(root));
                         B.sh_c[slot] = make_float4(C.x, C.y, C.z, 0.f);
                         v = 2;
                         shadow = true;
@@ -507,7 +542,7 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
                 // (ii) BSDF sampling needs the MIS ray when Pdf_Li != 0, :235-257
                 if (!delta && misCand && !misHit && pdf_light(S, Lt, it.p, it.n, misDir) != 0.f) misHit = true;
             }
-            if (live) B.sh_vis[slot] = v;
+            if (live && lj) B.sh_vis[slot] = v;
             int spos = wave_append(&B.counters[2], shadow);
             if (spos >= 0) B.sh_list[spos] = slot;
         }
@@ -525,15 +560,15 @@ __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
         int s = B.sh_list[i];
         float4 o = B.sh_o[s], d = B.sh_d[s];
         Hit h;
-        bool occluded =
-            traverse<true>(S, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f, __float_as_int(o.w), h, lds);
+        bool occluded = traverse<true>(S, __float_as_int(d.w), V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f,
+                                       __float_as_int(o.w), h, lds);
         B.sh_vis[s] = occluded ? 0 : 1;
     }
 }
 
 // The shared MIS ray: Scene::Intersect(isec.SpawnRay(wi)); only the light owning the
 // hit entity can receive f * Le * w / pdf_bsdf, :255-273
-__global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B) {
+__global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassParams P) {
     __shared__ TravLds<false> lds;
     int n = B.counters[3];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[2], (unsigned long long)n);
@@ -547,7 +582,7 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B) {
         Hit h;
         int lh = -1;
         V3 Lb{0, 0, 0};
-        if (traverse<false>(S, o, wi, 0.f, INFINITY, -1, h, lds)) {
+        if (traverse<false>(S, P.views[B.vid[p]].root, o, wi, 0.f, INFINITY, -1, h, lds)) {
             int4 e = S.entities[h.entity];
             int li = e.w;
             if (li >= 0 && li != own) {
@@ -596,8 +631,9 @@ __global__ void __launch_bounds__(kBlock) k_resolve(PathBuffers B, PassParams P,
             }
             V3 U{0, 0, 0};
             int slot0 = p * P.lmax;
-            for (int j = 0; j < P.num_elig; j++) {
-                int li = B.elig[j];
+            const View& V = P.views[B.vid[p]];
+            for (int j = 0; j < V.num_elig; j++) {
+                int li = B.elig[V.elig_begin + j];
                 V3 E{0, 0, 0};
                 bool any = false;
                 if (B.sh_vis[slot0 + j] == 1) {
@@ -683,7 +719,7 @@ struct DeviceArena {
         return p;
     }
     ~DeviceArena() {
-        for (void* p : ptrs) hipFree(p);
+        for (void* p : ptrs) (void)hipFree(p);
     }
 };
 
@@ -732,17 +768,55 @@ struct SceneGPU {
     std::vector<DevLight> lights;
     double build_seconds = 0;
 
-    void build(const c2_scene_desc* d) {
-        validate_desc(d);
+    // Start of each scene in the concatenated arrays (a single scene has offsets 0).
+    struct Offsets {
+        int32_t node, slot, ent, tri, sph, mat, light;
+    };
+    std::vector<Offsets> offs;
+    std::vector<Node4> nodes;  // every scene's tree; scene s is rooted at offs[s].node
+    std::vector<c2_material> mats;
+
+    void build(const c2_scene_desc* d) { build(&d, 1); }
+
+    // Concatenates scenes: each keeps its own BVH (built exactly as when alone) and
+    // its indices are shifted by the arrays before it, so traversal from its root
+    // visits the same nodes and primitives in the same order as a lone build.
+    void build(const c2_scene_desc* const* descs, int n) {
+        for (int k = 0; k < n; k++) validate_desc(descs[k]);
+        for (int k = 0; k < n; k++) {
+            Offsets o{(int32_t)nodes.size(), (int32_t)(prims.size() / 3), (int32_t)ents.size(),
+                      (int32_t)(tri_pos.size() / 9), (int32_t)(spheres.size() / 22), (int32_t)mats.size(),
+                      (int32_t)lights.size()};
+            offs.push_back(o);
+            append_scene(descs[k], o, k == 0);
+            if ((int64_t)ents.size() >= (1ll << kSlotBits) || (int64_t)prims.size() / 3 >= (1ll << kSlotBits))
+                fail(C2_ERR_UNSUPPORTED, "too many entities in the scene set");
+        }
+        dev.nodes = arena.upload(nodes);
+        dev.prims = arena.upload(prims);
+        dev.entities = arena.upload(ents);
+        dev.tri_pos = arena.upload(tri_pos);
+        dev.tri_shade = arena.upload(tri_shade);
+        dev.spheres = arena.upload(spheres);
+        dev.materials = arena.upload(mats);
+        dev.lights = arena.upload(lights);
+        dev.perm = arena.upload(halton_perm_data());
+        dev.perm_off = arena.upload(halton_perm_offsets());
+        dev.num_lights = (int32_t)lights.size();
+        dev.num_entities = (int32_t)ents.size();
+    }
+
+    // One scene in kernel layout, appended at offsets o. keep_bvh keeps the host
+    // trees of the first scene (ray-visit counting and accelerator statistics).
+    void append_scene(const c2_scene_desc* d, const Offsets& o, bool keep_bvh) {
         int nt = d->num_triangles, ns = d->num_spheres, ne = d->num_entities;
-        tri_pos.resize((size_t)nt * 9);
-        tri_shade.resize((size_t)nt * 12);
+        std::vector<float> tp((size_t)nt * 9), ts((size_t)nt * 12);
         for (int t = 0; t < nt; t++) {
             const float* v = d->triangles + (size_t)t * 24;
             for (int k = 0; k < 3; k++) {
                 for (int c = 0; c < 3; c++) {
-                    tri_pos[(size_t)t * 9 + k * 3 + c] = v[k * 8 + c];
-                    tri_shade[(size_t)t * 12 + k * 3 + c] = v[k * 8 + 3 + c];
+                    tp[(size_t)t * 9 + k * 3 + c] = v[k * 8 + c];
+                    ts[(size_t)t * 12 + k * 3 + c] = v[k * 8 + 3 + c];
                 }
             }
             // Triangle::calculateDerivative, Triangle.cpp:128-148
@@ -756,67 +830,73 @@ struct SceneGPU {
                 V3 a = p0 - p2, b = p1 - p2;
                 du = normalize(V3{i00 * a.x + i10 * b.x, i00 * a.y + i10 * b.y, i00 * a.z + i10 * b.z});
             }
-            tri_shade[(size_t)t * 12 + 9] = du.x;
-            tri_shade[(size_t)t * 12 + 10] = du.y;
-            tri_shade[(size_t)t * 12 + 11] = du.z;
+            ts[(size_t)t * 12 + 9] = du.x;
+            ts[(size_t)t * 12 + 10] = du.y;
+            ts[(size_t)t * 12 + 11] = du.z;
         }
-        spheres.assign(d->spheres, d->spheres + (size_t)ns * 22);
+        std::vector<float> sp3(d->spheres, d->spheres + (size_t)ns * 22);
         std::vector<Box3> boxes(ne);
-        ents.resize(ne);
+        std::vector<int4> le(ne);
         for (int e = 0; e < ne; e++) {
-            int k = d->entity_shape_kind[e], s = d->entity_shape_index[e];
-            ents[e] = make_int4(k, s, d->entity_material[e], d->entity_light[e]);
+            int k = d->entity_shape_kind[e], sh = d->entity_shape_index[e];
+            le[e] = make_int4(k, sh, d->entity_material[e], d->entity_light[e]);
             Box3& b = boxes[e];
             if (k == C2_SHAPE_TRIANGLE) {
-                const float* p = &tri_pos[(size_t)s * 9];
+                const float* p = &tp[(size_t)sh * 9];
                 for (int c = 0; c < 3; c++) {
                     b.lo[c] = fmin_ref(fmin_ref(p[c], p[3 + c]), p[6 + c]);
                     b.hi[c] = fmax_ref(fmax_ref(p[c], p[3 + c]), p[6 + c]);
                 }
             } else {
-                const float* sp = &spheres[(size_t)s * 22];
+                const float* q = &sp3[(size_t)sh * 22];
                 for (int c = 0; c < 3; c++) {
-                    b.lo[c] = sp[c] - sp[3];
-                    b.hi[c] = sp[c] + sp[3];
+                    b.lo[c] = q[c] - q[3];
+                    b.hi[c] = q[c] + q[3];
                 }
             }
         }
+        BuiltBvh local;
+        BuiltBvh& bv = keep_bvh ? bvh : local;
         auto t0 = std::chrono::steady_clock::now();
-        build_bvh(boxes, bvh);
-        build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        prims.resize(bvh.order.size() * 3);
-        for (size_t slot = 0; slot < bvh.order.size(); slot++) {
-            int e = bvh.order[slot];
-            int k = ents[e].x, s = ents[e].y;
+        build_bvh(boxes, bv);
+        build_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // nodes: inner children and leaf slots shifted; empty slots (count 0) stay -1
+        for (const Node4& nd0 : bv.nodes) {
+            Node4 nd = nd0;
+            for (int c = 0; c < 4; c++) {
+                uint32_t c8 = (nd.count >> (8 * c)) & 0xffu;
+                if (nd.child[c] >= 0) nd.child[c] += o.node;
+                else if (c8 != 0) nd.child[c] = ~(~nd.child[c] + o.slot);
+            }
+            nodes.push_back(nd);
+        }
+        for (size_t slot = 0; slot < bv.order.size(); slot++) {
+            int e = bv.order[slot];
+            int k = le[e].x, sh = le[e].y;
             if (k == C2_SHAPE_TRIANGLE) {
-                const float* p = &tri_pos[(size_t)s * 9];
-                prims[slot * 3 + 0] = make_float4(p[0], p[1], p[2], p[3]);
-                prims[slot * 3 + 1] = make_float4(p[4], p[5], p[6], p[7]);
-                float4 c;
-                c.x = p[8];
-                c.y = __int_as_float_host(e);
-                c.z = __int_as_float_host(C2_SHAPE_TRIANGLE);
-                c.w = 0.f;
-                prims[slot * 3 + 2] = c;
+                const float* p = &tp[(size_t)sh * 9];
+                prims.push_back(make_float4(p[0], p[1], p[2], p[3]));
+                prims.push_back(make_float4(p[4], p[5], p[6], p[7]));
+                prims.push_back(make_float4(p[8], __int_as_float_host(e + o.ent), __int_as_float_host(C2_SHAPE_TRIANGLE),
+                                            0.f));
             } else {
-                const float* sp = &spheres[(size_t)s * 22];
-                prims[slot * 3 + 0] = make_float4(sp[0], sp[1], sp[2], sp[3]);
-                float4 b;
-                b.x = __int_as_float_host(s);
-                b.y = b.z = b.w = 0.f;
-                prims[slot * 3 + 1] = b;
-                float4 c;
-                c.x = 0.f;
-                c.y = __int_as_float_host(e);
-                c.z = __int_as_float_host(C2_SHAPE_SPHERE);
-                c.w = 0.f;
-                prims[slot * 3 + 2] = c;
+                const float* q = &sp3[(size_t)sh * 22];
+                prims.push_back(make_float4(q[0], q[1], q[2], q[3]));
+                prims.push_back(make_float4(__int_as_float_host(sh + o.sph), 0.f, 0.f, 0.f));
+                prims.push_back(make_float4(0.f, __int_as_float_host(e + o.ent), __int_as_float_host(C2_SHAPE_SPHERE),
+                                            0.f));
             }
         }
-        lights.resize(d->num_lights);
+        for (int e = 0; e < ne; e++) {
+            int4 x = le[e];
+            x.y += x.x == C2_SHAPE_TRIANGLE ? o.tri : o.sph;
+            if (x.z >= 0) x.z += o.mat;
+            if (x.w >= 0) x.w += o.light;
+            ents.push_back(x);
+        }
         for (int l = 0; l < d->num_lights; l++) {
             const c2_light& L = d->lights[l];
-            DevLight& D = lights[l];
+            DevLight D;
             std::memset(&D, 0, sizeof(D));
             D.type = L.type;
             D.entity = L.entity;
@@ -825,35 +905,29 @@ struct SceneGPU {
                 D.pos[c] = L.position[c];
             }
             if (L.type == C2_LIGHT_AREA) {
-                D.shape_kind = ents[L.entity].x;
-                D.shape = ents[L.entity].y;
+                D.shape_kind = le[L.entity].x;
+                D.shape = le[L.entity].y;
                 if (D.shape_kind == C2_SHAPE_TRIANGLE) {  // Triangle::SurfaceArea
-                    const float* p = &tri_pos[(size_t)D.shape * 9];
+                    const float* p = &tp[(size_t)D.shape * 9];
                     V3 v0{p[0], p[1], p[2]}, v1{p[3], p[4], p[5]}, v2{p[6], p[7], p[8]};
                     D.area = length(cross(v1 - v0, v2 - v0)) / 2.f;
                 } else {  // Sphere::SurfaceArea
-                    float r = spheres[(size_t)D.shape * 22 + 3];
+                    float r = sp3[(size_t)D.shape * 22 + 3];
                     D.area = 4 * kPi * r * r;
                 }
                 V3 flux = V3{D.L[0], D.L[1], D.L[2]} * kPi * D.area;
                 D.eligible = !is_zero(flux);
+                D.entity += o.ent;
+                D.shape += D.shape_kind == C2_SHAPE_TRIANGLE ? o.tri : o.sph;
             } else {
                 D.eligible = !(D.L[0] == 0.f && D.L[1] == 0.f && D.L[2] == 0.f);
             }
+            lights.push_back(D);
         }
-        dev.nodes = arena.upload(bvh.nodes);
-        dev.prims = arena.upload(prims);
-        dev.entities = arena.upload(ents);
-        dev.tri_pos = arena.upload(tri_pos);
-        dev.tri_shade = arena.upload(tri_shade);
-        dev.spheres = arena.upload(spheres);
-        std::vector<c2_material> mats(d->materials, d->materials + d->num_materials);
-        dev.materials = arena.upload(mats);
-        dev.lights = arena.upload(lights);
-        dev.perm = arena.upload(halton_perm_data());
-        dev.perm_off = arena.upload(halton_perm_offsets());
-        dev.num_lights = d->num_lights;
-        dev.num_entities = ne;
+        tri_pos.insert(tri_pos.end(), tp.begin(), tp.end());
+        tri_shade.insert(tri_shade.end(), ts.begin(), ts.end());
+        spheres.insert(spheres.end(), sp3.begin(), sp3.end());
+        mats.insert(mats.end(), d->materials, d->materials + d->num_materials);
     }
 
     static float __int_as_float_host(int32_t i) {
@@ -938,7 +1012,7 @@ int c2_accel_build(const c2_scene_desc* desc, c2_accel** out) {
         std::unique_ptr<c2_accel> a(new c2_accel());
         a->scene.build(desc);
         a->stats.num_nodes_bvh2 = (int64_t)a->scene.bvh.bvh2.size();
-        a->stats.num_nodes_bvh4 = (int64_t)a->scene.bvh.nodes.size();
+        a->stats.num_nodes_bvh4 = (int64_t)a->scene.nodes.size();
         a->stats.max_depth_bvh4 = a->scene.bvh.depth4;
         a->stats.leaf_size = kLeafSize;
         a->stats.build_seconds = a->scene.build_seconds;
@@ -1049,7 +1123,17 @@ struct c2_pt {
     PassParams pp{};
     DeviceArena arena;
     PathBuffers B{};
-    int32_t chunk = 0;
+    int32_t chunk = 0;      // paths in flight per launch (at most chunk_cap)
+    int32_t chunk_cap = 0;  // paths the buffers hold
+    // views (one per scene of a batch) and the views taking part in each sample pass
+    std::vector<View> views;
+    struct PassTab {
+        int32_t off, npv;  // entries pv[off, off + npv)
+        int64_t npix;      // linear pass indices
+    };
+    std::vector<PassTab> passes;  // a single renderer uses passes[0] for every sample
+    int2* pv = nullptr;
+    int64_t film_pixels = 0;
     float4* film = nullptr;
     float4* comp = nullptr;
     double4* film64 = nullptr;
@@ -1089,129 +1173,202 @@ void nrc_launch_train(c2_nrc* c, const float4* feats, const float4* targets, con
 
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+// One image of a renderer: film size, Halton sample count (pass participation) and
+// the pixel window that receives samples (win_w = 0: the whole film).
+struct ViewSpec {
+    int32_t width, height, spp;
+    int32_t win_x0, win_y0, win_w, win_h;
+};
+
+void check_opts(const c2_render_opts* opts) {
+    if (opts->max_depth < 0) fail(C2_ERR_INVALID, "render options: negative depth");
+    if (opts->arg_order != C2_ARGORDER_RTL && opts->arg_order != C2_ARGORDER_LTR)
+        fail(C2_ERR_INVALID, "render options: bad arg_order");
+    if (opts->film_mode < C2_FILM_F32 || opts->film_mode > C2_FILM_F64)
+        fail(C2_ERR_INVALID, "render options: bad film_mode");
+    if (opts->queue_blocks < 0 || opts->paths_in_flight < 0)
+        fail(C2_ERR_INVALID, "render options: negative queue_blocks / paths_in_flight");
+}
+
+// Builds a renderer over n scenes (one view each) sharing one concatenated scene,
+// path state sized for the largest sample pass, and one film per view.
+void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, int n, const c2_render_opts* opts,
+              bool per_sample_passes) {
+    check_opts(opts);
+    for (int k = 0; k < n; k++) {
+        const ViewSpec& v = vs[k];
+        if (v.width <= 0 || v.height <= 0 || v.spp <= 0) fail(C2_ERR_INVALID, "render options: non-positive size");
+        if (v.win_w > 0 && (v.win_x0 < 0 || v.win_y0 < 0 || v.win_h <= 0 || v.win_x0 + v.win_w > v.width ||
+                            v.win_y0 + v.win_h > v.height))
+            fail(C2_ERR_INVALID, "render options: pixel window outside the film");
+    }
+    pt->opts = *opts;
+    std::unique_ptr<c2_accel> acc(new c2_accel());
+    acc->scene.build(descs, n);
+    pt->accel = std::move(acc);
+    const SceneGPU& SG = pt->accel->scene;
+    PassParams& P = pt->pp;
+    P.max_depth = opts->max_depth;
+    P.arg_order = opts->arg_order;
+    P.cache_bounce = -1;
+    P.train_ratio = 1;
+    P.train_phase = 0;
+    P.train_cap = 0;
+    P.train_paths = 0;
+    std::vector<int32_t> elig;
+    int64_t film_off = 0;
+    int32_t maxElig = 0;
+    for (int k = 0; k < n; k++) {
+        const ViewSpec& v = vs[k];
+        View V{};
+        V.width = v.width;
+        V.height = v.height;
+        V.root = descs[k]->num_entities > 0 ? SG.offs[k].node : -1;
+        // camera (PerspCamera ctor: tan evaluated on the host with the float libm call)
+        const c2_camera& c = descs[k]->camera;
+        float aspect = (float)v.width / v.height;
+        V.camH = 1.0f * std::tan(c.fov / 2);
+        V.camW = V.camH * aspect;
+        V.eye = V3{c.position[0], c.position[1], c.position[2]};
+        V3 at{c.look_at[0], c.look_at[1], c.look_at[2]}, up0{c.up[0], c.up[1], c.up[2]};
+        V.look = normalize(at - V.eye);
+        V3 pr = dot(up0, V.look) * V.look;
+        V.up = normalize(up0 - pr);
+        V.right = normalize(cross(V.look, up0));
+        V.win_x0 = v.win_w > 0 ? v.win_x0 : 0;
+        V.win_y0 = v.win_w > 0 ? v.win_y0 : 0;
+        V.win_w = v.win_w > 0 ? v.win_w : v.width;
+        HaltonParams hp = halton_params(v.width, v.height);
+        V.stride0 = hp.stride;
+        V.exp0 = hp.base_exponents[0];
+        V.bexp1 = hp.base_exponents[1];
+        V.scale1 = hp.base_scales[1];
+        V.bscale0 = hp.base_scales[0];
+        V.mult0 = hp.mult_inverse[0];
+        V.mult1 = hp.mult_inverse[1];
+        V.elig_begin = (int32_t)elig.size();
+        int32_t lb = SG.offs[k].light, le = lb + descs[k]->num_lights, nd = 0;
+        for (int32_t l = lb; l < le; l++) {
+            const DevLight& L = SG.lights[l];
+            if (!L.eligible) continue;
+            elig.push_back(l);
+            if (L.type != C2_LIGHT_POINT) nd++;
+        }
+        V.num_elig = (int32_t)elig.size() - V.elig_begin;
+        V.nd_elig = nd;
+        maxElig = std::max(maxElig, V.num_elig);
+        V.film_off = film_off;
+        film_off += (int64_t)v.width * v.height;
+        pt->views.push_back(V);
+    }
+    if (film_off >= (1ll << 31)) fail(C2_ERR_UNSUPPORTED, "films exceed 2^31 pixels");
+    pt->film_pixels = film_off;
+    P.lmax = std::max<int32_t>(maxElig, 1);
+    // sample passes: which views take part (spp > k) and where their pixels start
+    std::vector<int2> pv;
+    auto view_pixels = [&](int k) {
+        const ViewSpec& v = vs[k];
+        return v.win_w > 0 ? (int64_t)v.win_w * v.win_h : (int64_t)v.width * v.height;
+    };
+    int32_t npasses = 1;
+    if (per_sample_passes)
+        for (int k = 0; k < n; k++) npasses = std::max(npasses, vs[k].spp);
+    for (int32_t k = 0; k < npasses; k++) {
+        c2_pt::PassTab t{(int32_t)pv.size(), 0, 0};
+        for (int v = 0; v < n; v++) {
+            if (per_sample_passes && vs[v].spp <= k) continue;
+            pv.push_back(make_int2(v, (int32_t)t.npix));
+            t.npix += view_pixels(v);
+            t.npv++;
+        }
+        if (t.npix >= (1ll << 31)) fail(C2_ERR_UNSUPPORTED, "a sample pass exceeds 2^31 paths");
+        pt->passes.push_back(t);
+    }
+    int64_t maxPass = 0;
+    for (auto& t : pt->passes) maxPass = std::max(maxPass, t.npix);
+    int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : maxPass;
+    // keep the per-path shadow slots within ~6 GiB
+    int64_t perPath = (int64_t)P.lmax * (3 * 16 + 1 + 4) + 16 * 14 + 16;
+    int64_t maxChunk = std::max<int64_t>(4096, ((int64_t)6 << 30) / perPath);
+    chunk = std::max<int64_t>(1, std::min(std::min(chunk, maxPass), maxChunk));
+    pt->chunk = pt->chunk_cap = (int32_t)chunk;
+    PathBuffers& B = pt->B;
+    DeviceArena& A = pt->arena;
+    View* dviews = A.upload(pt->views);
+    pt->pv = A.upload(pv);
+    P.views = dviews;
+    size_t ch = (size_t)chunk, slots = (size_t)chunk * P.lmax;
+    B.ray_o = A.alloc<float4>(ch);
+    B.ray_d = A.alloc<float4>(ch);
+    B.beta = A.alloc<float4>(ch);
+    B.beta2 = A.alloc<float4>(ch);
+    B.L = A.alloc<float4>(ch);
+    B.meta = A.alloc<int4>(ch);
+    B.vid = A.alloc<int32_t>(ch);
+    B.hit = A.alloc<float4>(ch);
+    B.vtx_p = A.alloc<float4>(ch);
+    B.vtx_n = A.alloc<float4>(ch);
+    B.vtx_du = A.alloc<float4>(ch);
+    B.vtx_d = A.alloc<float4>(ch);
+    B.vtx_i = A.alloc<int4>(ch);
+    B.misflag = A.alloc<int32_t>(ch);
+    B.mis_hit = A.alloc<int32_t>(ch);
+    if (elig.empty()) elig.push_back(-1);
+    B.elig = A.upload(elig);
+    B.mis_w = A.alloc<float4>(ch);
+    B.mis_f = A.alloc<float4>(ch);
+    B.mis_L = A.alloc<float4>(ch);
+    B.sh_o = A.alloc<float4>(slots);
+    B.sh_d = A.alloc<float4>(slots);
+    B.sh_c = A.alloc<float4>(slots);
+    B.sh_vis = A.alloc<uint8_t>(slots);
+    B.queue[0] = A.alloc<int32_t>(ch);
+    B.queue[1] = A.alloc<int32_t>(ch);
+    B.sh_list = A.alloc<int32_t>(slots);
+    B.mis_list = A.alloc<int32_t>(ch);
+    B.counters = A.alloc<int32_t>(8);
+    B.totals = A.alloc<unsigned long long>(4);
+    HIP_CHECK(hipMemset(B.counters, 0, 8 * sizeof(int32_t)));
+    HIP_CHECK(hipMemset(B.totals, 0, 4 * sizeof(unsigned long long)));
+    // Every view's film covers its whole frame: k_accumulate, the reset and the film
+    // read-back index it by full-frame pixel (y * width + x), also when a pixel
+    // window restricts which pixels receive samples.
+    if (opts->film_mode == C2_FILM_F64) {
+        pt->film64 = A.alloc<double4>(film_off);
+    } else {
+        pt->film = A.alloc<float4>(film_off);
+        if (opts->film_mode == C2_FILM_F32_KAHAN) pt->comp = A.alloc<float4>(film_off);
+    }
+    HIP_CHECK(hipStreamCreateWithFlags(&pt->stream, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamCreateWithFlags(&pt->side, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreateWithFlags(&pt->ev_fork, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&pt->ev_join, hipEventDisableTiming));
+}
+
+// Zeroes every film with kernels (also valid inside a captured graph).
+void enqueue_film_reset(c2_pt* pt, hipStream_t s) {
+    int64_t npix = pt->film_pixels;
+    if (pt->film64) {
+        hipLaunchKernelGGL(k_zero_d4, dim3(grid_for(npix)), dim3(kBlock), 0, s, pt->film64, npix);
+    } else {
+        hipLaunchKernelGGL(k_zero4, dim3(grid_for(npix)), dim3(kBlock), 0, s, pt->film, npix);
+        if (pt->comp) hipLaunchKernelGGL(k_zero4, dim3(grid_for(npix)), dim3(kBlock), 0, s, pt->comp, npix);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+extern "C" {
+
 int c2_pt_create(const c2_scene_desc* desc, const c2_render_opts* opts, c2_pt** out) {
     return guard([&] {
         if (!opts || !out) fail(C2_ERR_INVALID, "null argument");
-        if (opts->width <= 0 || opts->height <= 0 || opts->spp <= 0 || opts->max_depth < 0)
-            fail(C2_ERR_INVALID, "render options: non-positive size");
-        if (opts->arg_order != C2_ARGORDER_RTL && opts->arg_order != C2_ARGORDER_LTR)
-            fail(C2_ERR_INVALID, "render options: bad arg_order");
-        if (opts->film_mode < C2_FILM_F32 || opts->film_mode > C2_FILM_F64)
-            fail(C2_ERR_INVALID, "render options: bad film_mode");
-        if (opts->queue_blocks < 0 || opts->paths_in_flight < 0)
-            fail(C2_ERR_INVALID, "render options: negative queue_blocks / paths_in_flight");
         std::unique_ptr<c2_pt> pt(new c2_pt());
-        pt->opts = *opts;
-        c2_accel* a = nullptr;
-        int rc = c2_accel_build(desc, &a);
-        if (rc != C2_OK) fail(rc, c2_last_error());
-        pt->accel.reset(a);
-        // camera (PerspCamera ctor: tan evaluated on the host with the float libm call)
-        const c2_camera& c = desc->camera;
-        PassParams& P = pt->pp;
-        P.width = opts->width;
-        P.height = opts->height;
-        P.max_depth = opts->max_depth;
-        P.arg_order = opts->arg_order;
-        float aspect = (float)opts->width / opts->height;
-        P.camH = 1.0f * std::tan(c.fov / 2);
-        P.camW = P.camH * aspect;
-        P.eye = V3{c.position[0], c.position[1], c.position[2]};
-        V3 at{c.look_at[0], c.look_at[1], c.look_at[2]}, up0{c.up[0], c.up[1], c.up[2]};
-        P.look = normalize(at - P.eye);
-        V3 pr = dot(up0, P.look) * P.look;
-        P.up = normalize(up0 - pr);
-        P.right = normalize(cross(P.look, up0));
-        if (opts->win_w > 0) {
-            if (opts->win_x0 < 0 || opts->win_y0 < 0 || opts->win_h <= 0 || opts->win_x0 + opts->win_w > opts->width ||
-                opts->win_y0 + opts->win_h > opts->height)
-                fail(C2_ERR_INVALID, "render options: pixel window outside the film");
-            P.win_x0 = opts->win_x0;
-            P.win_y0 = opts->win_y0;
-            P.win_w = opts->win_w;
-        } else {
-            P.win_x0 = P.win_y0 = 0;
-            P.win_w = opts->width;
-        }
-        P.cache_bounce = -1;
-        P.train_ratio = 1;
-        P.train_phase = 0;
-        P.train_cap = 0;
-        P.train_paths = 0;
-        HaltonParams hp = halton_params(opts->width, opts->height);
-        P.stride0 = hp.stride;
-        P.exp0 = hp.base_exponents[0];
-        P.bexp1 = hp.base_exponents[1];
-        P.scale1 = hp.base_scales[1];
-        P.bscale0 = hp.base_scales[0];
-        P.mult0 = hp.mult_inverse[0];
-        P.mult1 = hp.mult_inverse[1];
-        std::vector<int32_t> elig;
-        int ndElig = 0;
-        for (size_t l = 0; l < pt->accel->scene.lights.size(); l++) {
-            const DevLight& L = pt->accel->scene.lights[l];
-            if (!L.eligible) continue;
-            elig.push_back((int32_t)l);
-            if (L.type != C2_LIGHT_POINT) ndElig++;
-        }
-        P.num_elig = (int32_t)elig.size();
-        P.nd_elig = ndElig;
-        P.lmax = std::max<int32_t>(P.num_elig, 1);
-        int64_t npix = opts->win_w > 0 ? (int64_t)opts->win_w * opts->win_h : (int64_t)opts->width * opts->height;
-        int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : npix;
-        // keep the per-path shadow slots within ~6 GiB
-        int64_t perPath = (int64_t)P.lmax * (3 * 16 + 1 + 4) + 16 * 14 + 12;
-        int64_t maxChunk = std::max<int64_t>(4096, ((int64_t)6 << 30) / perPath);
-        chunk = std::min(std::min(chunk, npix), maxChunk);
-        pt->chunk = (int32_t)chunk;
-        PathBuffers& B = pt->B;
-        DeviceArena& A = pt->arena;
-        size_t ch = (size_t)chunk, slots = (size_t)chunk * P.lmax;
-        B.ray_o = A.alloc<float4>(ch);
-        B.ray_d = A.alloc<float4>(ch);
-        B.beta = A.alloc<float4>(ch);
-        B.beta2 = A.alloc<float4>(ch);
-        B.L = A.alloc<float4>(ch);
-        B.meta = A.alloc<int4>(ch);
-        B.hit = A.alloc<float4>(ch);
-        B.vtx_p = A.alloc<float4>(ch);
-        B.vtx_n = A.alloc<float4>(ch);
-        B.vtx_du = A.alloc<float4>(ch);
-        B.vtx_d = A.alloc<float4>(ch);
-        B.vtx_i = A.alloc<int4>(ch);
-        B.misflag = A.alloc<int32_t>(ch);
-        B.mis_hit = A.alloc<int32_t>(ch);
-        if (elig.empty()) elig.push_back(-1);
-        B.elig = A.upload(elig);
-        B.mis_w = A.alloc<float4>(ch);
-        B.mis_f = A.alloc<float4>(ch);
-        B.mis_L = A.alloc<float4>(ch);
-        B.sh_o = A.alloc<float4>(slots);
-        B.sh_d = A.alloc<float4>(slots);
-        B.sh_c = A.alloc<float4>(slots);
-        B.sh_vis = A.alloc<uint8_t>(slots);
-        B.queue[0] = A.alloc<int32_t>(ch);
-        B.queue[1] = A.alloc<int32_t>(ch);
-        B.sh_list = A.alloc<int32_t>(slots);
-        B.mis_list = A.alloc<int32_t>(ch);
-        B.counters = A.alloc<int32_t>(8);
-        B.totals = A.alloc<unsigned long long>(4);
-        HIP_CHECK(hipMemset(B.counters, 0, 8 * sizeof(int32_t)));
-        HIP_CHECK(hipMemset(B.totals, 0, 4 * sizeof(unsigned long long)));
-        // The film always covers the whole frame: k_accumulate, c2_pt_reset and
-        // c2_pt_get_film index it by full-frame pixel (y * width + x), also when a
-        // pixel window restricts which pixels receive samples.
-        const int64_t filmPixels = (int64_t)opts->width * opts->height;
-        if (opts->film_mode == C2_FILM_F64) {
-            pt->film64 = A.alloc<double4>(filmPixels);
-        } else {
-            pt->film = A.alloc<float4>(filmPixels);
-            if (opts->film_mode == C2_FILM_F32_KAHAN) pt->comp = A.alloc<float4>(filmPixels);
-        }
-        HIP_CHECK(hipStreamCreateWithFlags(&pt->stream, hipStreamNonBlocking));
-        HIP_CHECK(hipStreamCreateWithFlags(&pt->side, hipStreamNonBlocking));
-        HIP_CHECK(hipEventCreateWithFlags(&pt->ev_fork, hipEventDisableTiming));
-        HIP_CHECK(hipEventCreateWithFlags(&pt->ev_join, hipEventDisableTiming));
+        ViewSpec v{opts->width, opts->height, opts->spp, opts->win_x0, opts->win_y0, opts->win_w, opts->win_h};
+        pt_build(pt.get(), &desc, &v, 1, opts, false);
         *out = pt.release();
         int rc2 = c2_pt_reset(*out);
         if (rc2 != C2_OK) fail(rc2, c2_last_error());
@@ -1223,14 +1380,7 @@ void c2_pt_destroy(c2_pt* pt) { delete pt; }
 int c2_pt_reset(c2_pt* pt) {
     return guard([&] {
         if (!pt) fail(C2_ERR_INVALID, "null renderer");
-        int64_t npix = (int64_t)pt->opts.width * pt->opts.height;
-        if (pt->film64) {
-            hipLaunchKernelGGL(k_zero_d4, dim3(grid_for(npix)), dim3(kBlock), 0, pt->stream, pt->film64, npix);
-        } else {
-            hipLaunchKernelGGL(k_zero4, dim3(grid_for(npix)), dim3(kBlock), 0, pt->stream, pt->film, npix);
-            if (pt->comp) hipLaunchKernelGGL(k_zero4, dim3(grid_for(npix)), dim3(kBlock), 0, pt->stream, pt->comp, npix);
-        }
-        HIP_CHECK(hipGetLastError());
+        enqueue_film_reset(pt, pt->stream);
         HIP_CHECK(hipStreamSynchronize(pt->stream));
         pt->samples = 0;
     });
@@ -1288,15 +1438,20 @@ __global__ void k_capture(PathBuffers B, int kind, int q, int lmax, int64_t n, f
 
 extern "C" {
 
-int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* stream) {
-    return guard([&] {
-        if (!pt) fail(C2_ERR_INVALID, "null renderer");
-        if (sample_begin < 0 || sample_count < 0) fail(C2_ERR_INVALID, "negative sample range");
-        hipStream_t s = stream ? (hipStream_t)stream : pt->stream;
+}  // extern "C"
+
+namespace {
+// Enqueues Halton sample passes [sample_begin, + sample_count) on stream s. A single
+// renderer uses one pass table for every sample; a batch uses table k for sample k
+// (the views with spp > k).
+void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipStream_t s, bool per_sample_passes) {
         PassParams P = pt->pp;
         DevScene S = pt->accel->scene.dev;
-        int64_t npix = pt->opts.win_w > 0 ? (int64_t)pt->opts.win_w * pt->opts.win_h : (int64_t)P.width * P.height;
         for (int k = sample_begin; k < sample_begin + sample_count; k++) {
+            const c2_pt::PassTab& tab = pt->passes[per_sample_passes ? k : 0];
+            const int64_t npix = tab.npix;
+            P.pv = pt->pv + tab.off;
+            P.npv = tab.npv;
             P.sample = k;
             if (P.cache_bounce >= 0) P.train_phase = (int32_t)(((uint32_t)k * 2654435761u) % (uint32_t)P.train_ratio);
             for (int64_t beg = 0; beg < npix; beg += pt->chunk) {
@@ -1321,7 +1476,7 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
                         }
                         {
                             KTimer kt(pt, C2_K_EXTEND, s);
-                            hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBlock), 0, s, S, pt->B, q);
+                            hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
                         }
                         {
                             KTimer kt(pt, C2_K_SHADE, s);
@@ -1337,7 +1492,7 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
                             HIP_CHECK(hipEventRecord(e1, s));
                             pt->cache_events.push_back({0, {e0, e1}});
                         }
-                        if (P.num_elig > 0) {
+                        {
                             KTimer kt(pt, C2_K_NEE, s);
                             hipLaunchKernelGGL(k_nee, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
                         }
@@ -1354,7 +1509,7 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
                         {
                             KTimer kt(pt, C2_K_MIS, pt->side);
                             hipLaunchKernelGGL(k_mis, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, pt->side, S,
-                                               pt->B);
+                                               pt->B, P);
                         }
                         HIP_CHECK(hipEventRecord(pt->ev_join, pt->side));
                         int gs = std::min(grid_for((int64_t)P.pix_count * P.lmax), qcap);
@@ -1391,6 +1546,49 @@ int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* st
             }
             pt->samples += npix;
         }
+}
+
+// Film of view v as FrameBuffer::GetImageDataRGB32F returns it: color / weight,
+// rows flipped (y = 0 at the top).
+void read_film(c2_pt* pt, int v, float* rgb) {
+    HIP_CHECK(hipDeviceSynchronize());
+    const View& V = pt->views[v];
+    int W = V.width, H = V.height;
+    int64_t npix = (int64_t)W * H;
+    if (pt->film64) {
+        std::vector<double4> f(npix);
+        HIP_CHECK(hipMemcpy(f.data(), pt->film64 + V.film_off, npix * sizeof(double4), hipMemcpyDeviceToHost));
+        for (int y = 0; y < H; y++)
+            for (int x = 0; x < W; x++) {
+                const double4& c = f[(size_t)y * W + x];
+                float* o = rgb + ((size_t)(H - y - 1) * W + x) * 3;
+                o[0] = (float)(c.x / c.w);
+                o[1] = (float)(c.y / c.w);
+                o[2] = (float)(c.z / c.w);
+            }
+    } else {
+        std::vector<float4> f(npix);
+        HIP_CHECK(hipMemcpy(f.data(), pt->film + V.film_off, npix * sizeof(float4), hipMemcpyDeviceToHost));
+        for (int y = 0; y < H; y++)
+            for (int x = 0; x < W; x++) {
+                const float4& c = f[(size_t)y * W + x];
+                V3 q = V3{c.x, c.y, c.z} / c.w;
+                float* o = rgb + ((size_t)(H - y - 1) * W + x) * 3;
+                o[0] = q.x;
+                o[1] = q.y;
+                o[2] = q.z;
+            }
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* stream) {
+    return guard([&] {
+        if (!pt) fail(C2_ERR_INVALID, "null renderer");
+        if (sample_begin < 0 || sample_count < 0) fail(C2_ERR_INVALID, "negative sample range");
+        enqueue_passes(pt, sample_begin, sample_count, stream ? (hipStream_t)stream : pt->stream, false);
     });
 }
 
@@ -1405,33 +1603,7 @@ int c2_pt_sync(c2_pt* pt) {
 int c2_pt_get_film(c2_pt* pt, float* rgb) {
     return guard([&] {
         if (!pt || !rgb) fail(C2_ERR_INVALID, "null argument");
-        HIP_CHECK(hipDeviceSynchronize());
-        int W = pt->opts.width, H = pt->opts.height;
-        int64_t npix = (int64_t)W * H;
-        if (pt->film64) {
-            std::vector<double4> f(npix);
-            HIP_CHECK(hipMemcpy(f.data(), pt->film64, npix * sizeof(double4), hipMemcpyDeviceToHost));
-            for (int y = 0; y < H; y++)
-                for (int x = 0; x < W; x++) {
-                    const double4& c = f[(size_t)y * W + x];
-                    float* o = rgb + ((size_t)(H - y - 1) * W + x) * 3;
-                    o[0] = (float)(c.x / c.w);
-                    o[1] = (float)(c.y / c.w);
-                    o[2] = (float)(c.z / c.w);
-                }
-        } else {
-            std::vector<float4> f(npix);
-            HIP_CHECK(hipMemcpy(f.data(), pt->film, npix * sizeof(float4), hipMemcpyDeviceToHost));
-            for (int y = 0; y < H; y++)
-                for (int x = 0; x < W; x++) {
-                    const float4& c = f[(size_t)y * W + x];
-                    V3 v = V3{c.x, c.y, c.z} / c.w;  // GetImageDataRGB32F: color / weight, rows flipped
-                    float* o = rgb + ((size_t)(H - y - 1) * W + x) * 3;
-                    o[0] = v.x;
-                    o[1] = v.y;
-                    o[2] = v.z;
-                }
-        }
+        read_film(pt, 0, rgb);
     });
 }
 
@@ -1504,7 +1676,7 @@ int c2_pt_set_cache(c2_pt* pt, c2_nrc* nrc, int32_t cache_bounce, int32_t train_
             fail(C2_ERR_INVALID, "cache: bounce must be in [0, depth) and train_ratio >= 1");
         c2_nrc_stats st{};
         int32_t cap = (int32_t)std::min<int64_t>(1 << 16, (int64_t)pt->chunk * pt->pp.max_depth);
-        int64_t tpaths = (int64_t)pt->chunk / train_ratio + pt->opts.height + 64;
+        int64_t tpaths = (int64_t)pt->chunk / train_ratio + pt->views[0].height + 64;
         PathBuffers& B = pt->B;
         DeviceArena& A = pt->cache_arena;
         B.nrc_feat = A.alloc<float4>((size_t)pt->chunk * 4);
@@ -1609,3 +1781,179 @@ extern "C" int c2_util_stream_copy(int64_t bytes, int32_t reps, double* avg_ms) 
         *avg_ms = ms / reps;
     });
 }
+
+// ============================================================================ batched scenes
+
+// Many scenes rendered together: every scene is one view of a single renderer over
+// the concatenated scene set, and sample pass k runs the wavefront over the pixels
+// of every view with spp > k, so small scenes share launches instead of each
+// paying for its own. The full render (film reset + every pass) can be captured
+// once into a hipGraph and replayed with one launch.
+struct c2_batch {
+    std::unique_ptr<c2_pt> pt;
+    int32_t max_spp = 0;
+    int64_t samples_per_render = 0;  // sum over scenes of width * height * spp
+    int64_t renders = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~c2_batch() {
+        if (pt && pt->stream) (void)hipStreamSynchronize(pt->stream);
+        drop_graph();
+    }
+};
+
+extern "C" {
+
+int c2_batch_create(const c2_batch_item* items, int32_t n, const c2_render_opts* opts, c2_batch** out) {
+    return guard([&] {
+        if (!items || n <= 0 || !opts || !out) fail(C2_ERR_INVALID, "bad arguments");
+        std::vector<const c2_scene_desc*> descs(n);
+        std::vector<ViewSpec> vs(n);
+        std::unique_ptr<c2_batch> b(new c2_batch());
+        for (int k = 0; k < n; k++) {
+            if (!items[k].scene) fail(C2_ERR_INVALID, "null scene in batch");
+            descs[k] = items[k].scene;
+            vs[k] = ViewSpec{items[k].width, items[k].height, items[k].spp, 0, 0, 0, 0};
+            b->max_spp = std::max(b->max_spp, items[k].spp);
+            b->samples_per_render += (int64_t)items[k].width * items[k].height * std::max(0, items[k].spp);
+        }
+        b->pt.reset(new c2_pt());
+        pt_build(b->pt.get(), descs.data(), vs.data(), n, opts, true);
+        enqueue_film_reset(b->pt.get(), b->pt->stream);
+        HIP_CHECK(hipStreamSynchronize(b->pt->stream));
+        *out = b.release();
+    });
+}
+
+void c2_batch_destroy(c2_batch* b) { delete b; }
+
+int c2_batch_autotune(c2_batch* b, c2_batch_tuning* out) {
+    return guard([&] {
+        if (!b) fail(C2_ERR_INVALID, "null batch");
+        c2_pt* pt = b->pt.get();
+        hipStream_t s = pt->stream;
+        std::vector<int32_t> chunks;
+        for (int32_t c = pt->chunk_cap; c >= (1 << 16) && chunks.size() < 3; c /= 2) chunks.push_back(c);
+        if (chunks.empty()) chunks.push_back(pt->chunk_cap);
+        const int32_t qbs[] = {4096, 8192, 16384};
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        bool saveProf = pt->profiling;
+        pt->profiling = false;
+        float best = INFINITY;
+        int32_t bestC = pt->chunk, bestQ = pt->opts.queue_blocks;
+        int tried = 0;
+        enqueue_passes(pt, 0, 1, s, true);  // warm-up: code objects, caches
+        for (int32_t c : chunks)
+            for (int32_t q : qbs) {
+                pt->chunk = c;
+                pt->opts.queue_blocks = q;
+                (void)hipEventRecord(e0, s);
+                enqueue_passes(pt, 0, 1, s, true);
+                (void)hipEventRecord(e1, s);
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                tried++;
+                if (ms < best) {
+                    best = ms;
+                    bestC = c;
+                    bestQ = q;
+                }
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        pt->profiling = saveProf;
+        pt->chunk = bestC;
+        pt->opts.queue_blocks = bestQ;
+        pt->samples = 0;
+        b->drop_graph();  // a captured graph holds the old launch shapes
+        enqueue_film_reset(pt, s);
+        HIP_CHECK(hipStreamSynchronize(s));
+        if (out) {
+            out->paths_in_flight = bestC;
+            out->queue_blocks = bestQ;
+            out->pass_ms = best;
+            out->candidates = tried;
+        }
+    });
+}
+
+int c2_batch_render(c2_batch* b, int32_t use_graph) {
+    return guard([&] {
+        if (!b) fail(C2_ERR_INVALID, "null batch");
+        c2_pt* pt = b->pt.get();
+        hipStream_t s = pt->stream;
+        if (!use_graph) {
+            enqueue_film_reset(pt, s);
+            enqueue_passes(pt, 0, b->max_spp, s, true);
+        } else {
+            if (!b->exec) {
+                bool saveProf = pt->profiling;
+                pt->profiling = false;  // no timing events inside the graph
+                HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+                try {
+                    enqueue_film_reset(pt, s);
+                    enqueue_passes(pt, 0, b->max_spp, s, true);
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(s, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    pt->profiling = saveProf;
+                    throw;
+                }
+                pt->profiling = saveProf;
+                HIP_CHECK(hipStreamEndCapture(s, &b->graph));
+                HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            }
+            HIP_CHECK(hipGraphLaunch(b->exec, s));
+        }
+        b->renders++;
+    });
+}
+
+int c2_batch_sync(c2_batch* b) {
+    return guard([&] {
+        if (!b) fail(C2_ERR_INVALID, "null batch");
+        HIP_CHECK(hipStreamSynchronize(b->pt->stream));
+    });
+}
+
+int c2_batch_get_film(c2_batch* b, int32_t item, float* rgb) {
+    return guard([&] {
+        if (!b || !rgb) fail(C2_ERR_INVALID, "null argument");
+        if (item < 0 || item >= (int32_t)b->pt->views.size()) fail(C2_ERR_INVALID, "item out of range");
+        read_film(b->pt.get(), item, rgb);
+    });
+}
+
+int c2_batch_get_stats(const c2_batch* b, c2_batch_stats* out) {
+    return guard([&] {
+        if (!b || !out) fail(C2_ERR_INVALID, "null argument");
+        unsigned long long t[4] = {0, 0, 0, 0};
+        HIP_CHECK(hipMemcpy(t, b->pt->B.totals, sizeof(t), hipMemcpyDeviceToHost));
+        out->samples = b->renders * b->samples_per_render;
+        out->samples_per_render = b->samples_per_render;
+        out->extension_rays = (int64_t)t[0];
+        out->shadow_rays = (int64_t)t[1];
+        out->mis_rays = (int64_t)t[2];
+        out->passes = b->max_spp;
+        out->paths_in_flight = b->pt->chunk;
+        out->queue_blocks = b->pt->opts.queue_blocks;
+        out->graph_nodes = 0;
+        if (b->graph) {
+            size_t nn = 0;
+            HIP_CHECK(hipGraphGetNodes(b->graph, nullptr, &nn));
+            out->graph_nodes = (int64_t)nn;
+        }
+    });
+}
+
+}  // extern "C"

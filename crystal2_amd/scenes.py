@@ -387,3 +387,46 @@ def spheres_scene(directory: str, width: int = 1920, height: int = 1080, spp: in
     cj = os.path.join(directory, "config.json")
     write_config(cj, width, height, spp, depth, threads=threads)
     return SceneFiles(directory, sj, cj)
+
+
+def build_spheres_scene(n_spheres: int = 400, level: int = 4, seed: int = 2024, extent: float = 12.0):
+    """The sphere-field scene as an in-memory Scene (same draws as spheres_scene)."""
+    import crystal2_amd as c2
+    d = spheres_scene_objects(n_spheres, level, seed, extent)
+    sc = c2.Scene.create()
+    for (p, n, f, mat, le) in d["meshes"]:
+        if mat is None:
+            m = -1
+        else:
+            mt = {"Diffuse": 0, "Mirror": 1, "Glass": 2, "Micro": 3, "Phong": 4}[mat["Type"]]
+            m = sc.add_material(mt, mat["Color"], mat.get("Roughness", (0, 0)), mat.get("IOR", 1.0), mat.get("K", 0))
+        sc.add_mesh(p, n, f, m, lemit=le)
+    cam = d["camera"]
+    sc.set_camera(cam["Position"], cam["LookAt"], cam["Up"], cam["FOV"])
+    return sc
+
+
+def batch_specs(n: int = 64, seed: int = 6400):
+    """The batched-scenes workload: n sphere-field scenes with triangle counts
+    log-uniform in [1k, 200k], square resolutions uniform in {128, 256, 512} and
+    spp uniform in {4, 16, 64}. Spheres are icospheres of 320, 1280 or 5120
+    triangles (level 2 below 10k triangles, 3 below 50k, else 4). Returns a list of
+    dicts (n_spheres, level, seed, width, height, spp, triangles)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        target = float(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        level = 2 if target < 1e4 else (3 if target < 5e4 else 4)
+        per = 20 * 4 ** level
+        ns = max(1, int(round((target - 10) / per)))
+        res = int(rng.choice([128, 256, 512]))
+        spp = int(rng.choice([4, 16, 64]))
+        out.append({"n_spheres": ns, "level": level, "seed": seed + 1 + i, "width": res, "height": res,
+                    "spp": spp, "triangles": ns * per + 10})
+    return out
+
+
+def build_batch(specs):
+    """[(Scene, width, height, spp)] for batch_specs entries."""
+    return [(build_spheres_scene(s["n_spheres"], s["level"], s["seed"]), s["width"], s["height"], s["spp"])
+            for s in specs]

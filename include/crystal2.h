@@ -298,6 +298,43 @@ typedef struct c2_cache_times {
 } c2_cache_times;
 int  c2_pt_get_cache_times(c2_pt* pt, c2_cache_times* out);
 
+/* ---------------------------------------------------------------- Batched scenes
+ * No counterpart in the reference, which renders one scene per process
+ * (src/Core/RayTracer.cpp:11-22). Many scenes, each with its own film size and
+ * sample count, render together: sample pass k traces every pixel of every scene
+ * with spp > k in shared launches. Each scene's film is bit-identical to its own
+ * c2_pt_render(0, spp) with the same options. The whole render can be captured
+ * once into a hipGraph and replayed with one launch (use_graph = 1).
+ * opts: max_depth, arg_order, film_mode, paths_in_flight, queue_blocks apply to
+ * every scene; width / height / spp / the pixel window come from the items. */
+typedef struct c2_batch c2_batch;
+typedef struct c2_batch_item {
+    const c2_scene_desc* scene;
+    int32_t width, height, spp;
+} c2_batch_item;
+typedef struct c2_batch_tuning {
+    int32_t paths_in_flight;  /* chosen paths per launch                         */
+    int32_t queue_blocks;     /* chosen grid cap of the queue kernels            */
+    double pass_ms;           /* sample pass 0 with the chosen values            */
+    int32_t candidates;       /* configurations timed                            */
+} c2_batch_tuning;
+typedef struct c2_batch_stats {
+    int64_t samples, samples_per_render;
+    int64_t extension_rays, shadow_rays, mis_rays;
+    int32_t passes, paths_in_flight, queue_blocks;
+    int64_t graph_nodes;      /* nodes of the captured graph (0 before capture)  */
+} c2_batch_stats;
+int  c2_batch_create(const c2_batch_item* items, int32_t n, const c2_render_opts* opts, c2_batch** out);
+void c2_batch_destroy(c2_batch* batch);
+/* Times sample pass 0 over a small grid of paths-in-flight x queue_blocks values,
+ * keeps the fastest (drops a captured graph) and clears the films. */
+int  c2_batch_autotune(c2_batch* batch, c2_batch_tuning* out);
+/* Clears the films and renders every sample of every scene; asynchronous. */
+int  c2_batch_render(c2_batch* batch, int32_t use_graph);
+int  c2_batch_sync(c2_batch* batch);
+int  c2_batch_get_film(c2_batch* batch, int32_t item, float* rgb);  /* item's W*H*3, rows top-down */
+int  c2_batch_get_stats(const c2_batch* batch, c2_batch_stats* out);
+
 /* ---------------------------------------------------------------- Sampler (Halton)
  * Reference HaltonSampler for a W x H film: vector index of (pixel, sample) and
  * sample value per dimension (dims >= 100 use the documented deterministic
