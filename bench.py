@@ -181,6 +181,58 @@ def equal_time(scene, W, H, depth, seconds, judge_spp=4096, crop=256, cache_boun
     }
 
 
+def batch_bench(n=64, seed=6400, depth=8):
+    """Batched scenes: n sphere-field scenes (triangles log-uniform in [1k, 200k],
+    resolution in {128, 256, 512}^2, spp in {4, 16, 64}) rendered (a) one scene at a
+    time with its own launches and (b) as one batch replayed from a captured hipGraph
+    after the pass-0 autotuner. Both are timed wall-clock after a warm-up render;
+    every batch film must equal its single-scene film bit for bit."""
+    import crystal2_amd as c2
+    from crystal2_amd import scenes
+    t0 = time.time()
+    specs = scenes.batch_specs(n, seed)
+    items = scenes.build_batch(specs)
+    log(f"batch: {n} scenes ({sum(s['triangles'] for s in specs)} triangles) built in {time.time() - t0:.1f}s")
+    samples = sum(w * h * spp for _, w, h, spp in items)
+    # (a) per-scene launches: each scene its own renderer, rendered in turn
+    pts = [c2.WavefrontPathIntegrator(sc, w, h, spp, depth, arg_order="rtl") for sc, w, h, spp in items]
+    for pt in pts:  # warm-up
+        pt.render(0, 1)
+    for pt in pts:
+        pt.sync()
+        pt.reset()
+    a = time.perf_counter()
+    for pt in pts:
+        pt.render()
+    for pt in pts:
+        pt.sync()
+    single_s = time.perf_counter() - a
+    ref = [pt.film() for pt in pts]
+    for pt in pts:
+        pt.close()
+    # (b) one batch, autotuned, replayed from a captured graph
+    b = c2.BatchRenderer(items, depth, arg_order="rtl")
+    tune = b.autotune()
+    b.render(True)  # capture + first replay (warm-up)
+    b.sync()
+    a = time.perf_counter()
+    b.render(True)
+    b.sync()
+    batch_s = time.perf_counter() - a
+    exact = sum(int(np.array_equal(b.film(i), r)) for i, r in enumerate(ref))
+    st = b.stats()
+    b.close()
+    out = {"scenes": n, "triangles": int(sum(s["triangles"] for s in specs)), "samples": int(samples),
+           "depth": depth, "per_scene_s": round(single_s, 4), "batch_graph_s": round(batch_s, 4),
+           "per_scene_msamples_s": round(samples / single_s / 1e6, 2),
+           "batch_graph_msamples_s": round(samples / batch_s / 1e6, 2),
+           "speedup": round(single_s / batch_s, 3), "films_bit_identical": f"{exact}/{n}",
+           "autotune": {"paths_in_flight": tune["paths_in_flight"], "queue_blocks": tune["queue_blocks"],
+                        "candidates": tune["candidates"], "pass0_ms": round(tune["pass_ms"], 3)},
+           "graph_nodes": st["graph_nodes"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +244,8 @@ def main():
     ap.add_argument("--spheres", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-cache", action="store_true", help="skip the equal-time radiance-cache comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-render comparison")
+    ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--equal-seconds", type=float, default=0.0, help="equal-time budget (0: the timed run's)")
     ap.add_argument("--judge-spp", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -273,6 +327,11 @@ def main():
             f"cache reaches plain relMSE at {eq['cache_time_to_plain_relmse_frac']} of the time; "
             f"mfma {eq['mfma']}")
 
+    bat = None
+    if not args.no_batch:
+        bat = batch_bench(args.batch_scenes, depth=args.depth)
+        log(f"batch: {bat}")
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -301,6 +360,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "relmse_equal_time": eq,
+        "batch": bat,
     }
     print(json.dumps(out), flush=True)
 
