@@ -5,7 +5,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libcrystal2.so")
+# CRYSTAL2_LIB selects another build of the library (for example a variant under test).
+LIB_PATH = os.environ.get("CRYSTAL2_LIB") or os.path.join(HERE, "_lib", "libcrystal2.so")
 
 C2_OK = 0
 ERRORS = {-1: "C2_ERR_INVALID", -2: "C2_ERR_PARSE", -3: "C2_ERR_IO", -4: "C2_ERR_DEVICE",

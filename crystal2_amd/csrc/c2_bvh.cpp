@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <stdexcept>
 #include <thread>
 
 #include "c2_host.h"
@@ -46,10 +47,11 @@ struct Builder {
     std::vector<float> cen;  // [n][3]
     std::vector<int32_t>& order;
     std::vector<Bvh2Node>& nodes;
+    int leafSize;
     int maxDepth = 0;
 
-    Builder(const std::vector<Box3>& b, std::vector<int32_t>& o, std::vector<Bvh2Node>& n)
-        : boxes(b), order(o), nodes(n) {}
+    Builder(const std::vector<Box3>& b, std::vector<int32_t>& o, std::vector<Bvh2Node>& n, int leaf)
+        : boxes(b), order(o), nodes(n), leafSize(leaf) {}
 
     int ceil_log2(int64_t x) {
         int r = 0;
@@ -70,13 +72,13 @@ struct Builder {
         }
         nodes[idx].box = bb;
         int count = end - begin;
-        if (count <= kLeafSize) {
+        if (count <= leafSize) {
             nodes[idx].left = begin;
             nodes[idx].right = -count;
             return;
         }
         int mid = -1, axis = 0;
-        bool forceMedian = depth + 1 + ceil_log2((count + kLeafSize - 1) / kLeafSize) >= kMaxBvh2Depth;
+        bool forceMedian = depth + 1 + ceil_log2((count + leafSize - 1) / leafSize) >= kMaxBvh2Depth;
         float ext[3];
         for (int k = 0; k < 3; k++) ext[k] = cb.hi[k] - cb.lo[k];
         axis = (ext[0] >= ext[1] && ext[0] >= ext[2]) ? 0 : (ext[1] >= ext[2] ? 1 : 2);
@@ -219,18 +221,19 @@ struct Collapser {
 
 }  // namespace
 
-void build_bvh(const std::vector<Box3>& boxes, BuiltBvh& out) {
+void build_bvh(const std::vector<Box3>& boxes, BuiltBvh& out, int leaf_size) {
+    if (leaf_size < 1 || leaf_size > kLeafSize) throw std::invalid_argument("BVH leaf size out of range");
     size_t n = boxes.size();
     out.order.resize(n);
     std::iota(out.order.begin(), out.order.end(), 0);
     out.bvh2.clear();
     out.nodes.clear();
     if (n == 0) return;
-    Builder b(boxes, out.order, out.bvh2);
+    Builder b(boxes, out.order, out.bvh2, leaf_size);
     b.cen.resize(n * 3);
     for (size_t e = 0; e < n; e++)
         for (int k = 0; k < 3; k++) b.cen[e * 3 + k] = 0.5f * boxes[e].lo[k] + 0.5f * boxes[e].hi[k];
-    out.bvh2.reserve(2 * n / kLeafSize + 4);
+    out.bvh2.reserve(2 * n / leaf_size + 4);
     out.bvh2.push_back(Bvh2Node{});
     b.build(0, 0, (int)n, 0);
     out.depth2 = b.maxDepth;

@@ -29,7 +29,8 @@
 
 namespace c2 {
 
-constexpr int kLeafSize = 4;
+constexpr int kLeafSize = 4;        // largest leaf (4-bit count field); the ray-cast accelerator's leaves
+constexpr int kLeafSizeRender = 2;  // leaves of the renderer's trees (fewer primitive tests per ray)
 constexpr int kStackSize = 64;      // per-lane traversal stack (entries)
 constexpr int kMaxBvh4Depth = 21;   // 3 * depth + 1 <= kStackSize
 constexpr int kMaxBvh2Depth = 42;
@@ -65,6 +66,6 @@ struct BuiltBvh {
 };
 
 // Builds both trees from per-entity boxes. Throws c2::Error on depth overflow.
-void build_bvh(const std::vector<Box3>& boxes, BuiltBvh& out);
+void build_bvh(const std::vector<Box3>& boxes, BuiltBvh& out, int leaf_size = kLeafSize);
 
 }  // namespace c2

@@ -195,21 +195,70 @@ __device__ __forceinline__ int32_t halton_index_dev(const View& P, int x, int y,
     return (int32_t)(off + (int64_t)(k * P.stride0));
 }
 
-// wave-aggregated append: one atomic per wavefront
-__device__ __forceinline__ int32_t wave_append(int32_t* counter, bool pred) {
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return -1;
-    int lane = __lane_id();
-    int leader = __ffsll((long long)mask) - 1;
-    int32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (int32_t)__popcll(mask));
-    base = __shfl(base, leader);
+// Block-aggregated appends: one atomic per block of kBlock threads instead of one per
+// wavefront, because every block of a launch appends to the same counter and the
+// atomics on one address serialise in L2. Every thread of the block must make the
+// call (block-uniform control flow); each call starts with a barrier, so one LDS
+// scratch area can serve consecutive calls.
+constexpr int kWaves = kBlock / 64;
+constexpr int kAppendLights = 32;  // lights per light-major append group
+
+// Predicate append: returns this thread's list position, or -1 when !pred.
+// sm: kWaves + 1 ints of LDS.
+__device__ __forceinline__ int32_t block_append(int32_t* counter, bool pred, int32_t* sm) {
+    const int lane = __lane_id(), w = (int)threadIdx.x >> 6;
+    const uint64_t m = __ballot(pred);
+    __syncthreads();
+    if (lane == 0) sm[w] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int k = 0; k < kWaves; k++) {
+            int c = sm[k];
+            sm[k] = tot;
+            tot += c;
+        }
+        sm[kWaves] = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
     if (!pred) return -1;
-    uint64_t lower = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-    return base + (int32_t)__popcll(lower);
+    const uint64_t lower = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    return sm[kWaves] + sm[w] + __popcll(m & lower);
+}
+
+// Appends list entries p * E + jbase + j for the set bits j < nb of `bits`, in
+// light-major order within the block (all of the block's light-jbase rays, then the
+// next light's, ...). sm: kWaves * kAppendLights + 1 ints of LDS.
+__device__ __forceinline__ void block_append_lights(int32_t* counter, uint32_t bits, int nb, int32_t* sm,
+                                                    int32_t* list, int p, int E, int jbase) {
+    const int lane = __lane_id(), w = (int)threadIdx.x >> 6;
+    __syncthreads();
+    for (int j = 0; j < nb; j++) {
+        uint64_t m = __ballot((bits >> j) & 1u);
+        if (lane == 0) sm[w * kAppendLights + j] = __popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int j = 0; j < nb; j++)
+            for (int k = 0; k < kWaves; k++) {
+                int c = sm[k * kAppendLights + j];
+                sm[k * kAppendLights + j] = tot;
+                tot += c;
+            }
+        sm[kWaves * kAppendLights] = tot ? atomicAdd(counter, tot) : 0;
+    }
+    __syncthreads();
+    const uint64_t lower = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    const int32_t base = sm[kWaves * kAppendLights];
+    for (int j = 0; j < nb; j++) {
+        uint64_t m = __ballot((bits >> j) & 1u);
+        if ((bits >> j) & 1u) list[base + sm[w * kAppendLights + j] + __popcll(m & lower)] = p * E + jbase + j;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, PassParams P) {
+    __shared__ int32_t sm[kWaves + 1];
     int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < P.pix_count; base += stride) {
         int i = base + (int)threadIdx.x;
@@ -228,7 +277,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, 
         int x = V.win_x0 + lin % V.win_w, y = V.win_y0 + lin / V.win_w;
         int pixel = y * V.width + x;
         bool train = live && P.cache_bounce >= 0 && (pixel % P.train_ratio) == P.train_phase;
-        int tp = wave_append(&B.counters[5], train);
+        int tp = P.cache_bounce >= 0 ? block_append(&B.counters[5], train, sm) : -1;  // uniform condition
         if (tp >= P.train_paths) train = false;  // beyond the record capacity: an ordinary path
         if (!live) continue;
         HaltonDev hd{V.exp0, V.scale1};
@@ -289,11 +338,12 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, Pa
 // Vertex stage: emission, BSDF, the BSDF-sampled MIS direction (shared by all
 // lights), continuation sample and Russian roulette; the light loop runs in k_nee.
 __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, PassParams P, int q, int bounce) {
+    __shared__ int32_t sm[kWaves + 1];
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
     int32_t* next = B.queue[q ^ 1];
     // grid-stride loop over block-uniform bases: every lane of a wavefront runs the
-    // same iterations, as wave_append's ballot requires
+    // same iterations, as block_append's barriers and ballots require
     int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + (int)threadIdx.x;
@@ -426,10 +476,12 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
             B.meta[p] = make_int4(meta.x, idx, dim, flags);
             B.beta2[p] = make_float4(beta2.x, beta2.y, beta2.z, 0.f);
         }
-        int qpos = wave_append(&B.counters[q ^ 1], live && cont);
+        int qpos = block_append(&B.counters[q ^ 1], live && cont, sm);
         if (qpos >= 0) next[qpos] = p;
-        int cpos = wave_append(&B.counters[4], live && query);
-        if (cpos >= 0) B.nrc_list[cpos] = p;
+        if (P.cache_bounce >= 0) {  // uniform over the launch
+            int cpos = block_append(&B.counters[4], live && query, sm);
+            if (cpos >= 0) B.nrc_list[cpos] = p;
+        }
     }
 }
 
@@ -462,6 +514,7 @@ __global__ void __launch_bounds__(kBlock) k_cache_collect(PathBuffers B, PassPar
 // (shadow slot p * E + j) and the test whether the shared BSDF-sampled direction
 // needs the MIS ray (Pdf_Li != 0).
 __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
+    __shared__ int32_t sm[kWaves * kAppendLights + 1];
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
     const int E = P.lmax;
@@ -508,7 +561,8 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
             }
         }
         bool misHit = false;
-        // wave-uniform trip count (wave_append below needs every lane); lanes whose
+        uint32_t shbits = 0;
+        // block-uniform trip count (the appends below need every thread); lanes whose
         // view has fewer eligible lights idle through the tail
         for (int j = 0; j < E; j++) {
             const bool lj = j < ne;
@@ -543,11 +597,15 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
                 if (!delta && misCand && !misHit && pdf_light(S, Lt, it.p, it.n, misDir) != 0.f) misHit = true;
             }
             if (live && lj) B.sh_vis[slot] = v;
-            int spos = wave_append(&B.counters[2], shadow);
-            if (spos >= 0) B.sh_list[spos] = slot;
+            shbits |= shadow ? (1u << (j % kAppendLights)) : 0u;
+            if (j % kAppendLights == kAppendLights - 1 || j == E - 1) {
+                block_append_lights(&B.counters[2], shbits, j % kAppendLights + 1, sm, B.sh_list, p, E,
+                                    j - j % kAppendLights);
+                shbits = 0;
+            }
         }
         if (misHit) B.misflag[p] = 1;
-        int mpos = wave_append(&B.counters[3], misHit);
+        int mpos = block_append(&B.counters[3], misHit, sm);
         if (mpos >= 0) B.mis_list[mpos] = p;
     }
 }
@@ -776,6 +834,8 @@ struct SceneGPU {
     std::vector<Node4> nodes;  // every scene's tree; scene s is rooted at offs[s].node
     std::vector<c2_material> mats;
 
+    int leaf_size = kLeafSize;  // binary leaves before the 4-wide collapse
+
     void build(const c2_scene_desc* d) { build(&d, 1); }
 
     // Concatenates scenes: each keeps its own BVH (built exactly as when alone) and
@@ -858,7 +918,7 @@ struct SceneGPU {
         BuiltBvh local;
         BuiltBvh& bv = keep_bvh ? bvh : local;
         auto t0 = std::chrono::steady_clock::now();
-        build_bvh(boxes, bv);
+        build_bvh(boxes, bv, leaf_size);
         build_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         // nodes: inner children and leaf slots shifted; empty slots (count 0) stay -1
         for (const Node4& nd0 : bv.nodes) {
@@ -1207,6 +1267,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     }
     pt->opts = *opts;
     std::unique_ptr<c2_accel> acc(new c2_accel());
+    acc->scene.leaf_size = kLeafSizeRender;
     acc->scene.build(descs, n);
     pt->accel = std::move(acc);
     const SceneGPU& SG = pt->accel->scene;
