@@ -318,9 +318,12 @@ __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, 
                 float t1 = tmin, t2 = ANY ? tmax : fmin_ref(tmax, best);
                 if (kind == C2_SHAPE_TRIANGLE) {
                     V3 v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
-                    float lxx = fmin_ref(fmin_ref(v0.x, v1.x), v2.x), lyy = fmin_ref(fmin_ref(v0.y, v1.y), v2.y);
-                    float lzz = fmin_ref(fmin_ref(v0.z, v1.z), v2.z), hxx = fmax_ref(fmax_ref(v0.x, v1.x), v2.x);
-                    float hyy = fmax_ref(fmax_ref(v0.y, v1.y), v2.y), hzz = fmax_ref(fmax_ref(v0.z, v1.z), v2.z);
+                    // entity box (component-wise min / max of finite vertices: the IEEE
+                    // forms equal std::min / std::max up to the sign of a zero, which
+                    // no comparison of the slab test sees)
+                    float lxx = fminf(fminf(v0.x, v1.x), v2.x), lyy = fminf(fminf(v0.y, v1.y), v2.y);
+                    float lzz = fminf(fminf(v0.z, v1.z), v2.z), hxx = fmaxf(fmaxf(v0.x, v1.x), v2.x);
+                    float hyy = fmaxf(fmaxf(v0.y, v1.y), v2.y), hzz = fmaxf(fmaxf(v0.z, v1.z), v2.z);
                     if (!slab(r, lxx, lyy, lzz, hxx, hyy, hzz, t1, t2)) continue;
                     if (ANY) {
                         if (ent == ignore) continue;

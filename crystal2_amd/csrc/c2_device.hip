@@ -140,8 +140,7 @@ struct View {
     int32_t num_elig;
     int32_t nd_elig;         // eligible non-delta lights (one Get1D each per vertex)
     int32_t stride0, exp0, scale1, mult0, mult1, bscale0, bexp1;  // Halton constants
-    int32_t win_x0, win_y0, win_w;  // pixel window: linear index i -> (x0 + i % w, y0 + i / w)
-    int32_t pad_;
+    int32_t win_x0, win_y0, win_w, win_h;  // pixel window receiving samples
     int64_t film_off;        // first pixel of this view's film in the concatenated film
     V3 eye, look, up, right;
     float camW, camH;
@@ -274,7 +273,15 @@ __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, 
         const int v = P.pv[lo].x;
         const View& V = P.views[v];
         lin -= P.pv[lo].y;
-        int x = V.win_x0 + lin % V.win_w, y = V.win_y0 + lin / V.win_w;
+        // Pixel order: bands of 8 rows walked column by column, so the 64 paths of a
+        // wavefront start in one 8 x 8 tile (coherent primary and shadow rays). A
+        // bijection onto the window for any size; the last band holds the leftover rows.
+        const int winH = V.win_h;
+        const int fb = winH >> 3;
+        const int band = lin / (V.win_w * 8);
+        const int hb = band < fb ? 8 : winH - fb * 8;
+        const int idxb = lin - band * V.win_w * 8;
+        int x = V.win_x0 + idxb / hb, y = V.win_y0 + band * 8 + idxb % hb;
         int pixel = y * V.width + x;
         bool train = live && P.cache_bounce >= 0 && (pixel % P.train_ratio) == P.train_phase;
         int tp = P.cache_bounce >= 0 ? block_append(&B.counters[5], train, sm) : -1;  // uniform condition
@@ -851,6 +858,7 @@ struct SceneGPU {
             append_scene(descs[k], o, k == 0);
             if ((int64_t)ents.size() >= (1ll << kSlotBits) || (int64_t)prims.size() / 3 >= (1ll << kSlotBits))
                 fail(C2_ERR_UNSUPPORTED, "too many entities in the scene set");
+            if ((int64_t)nodes.size() >= 0x7fffffffll) fail(C2_ERR_UNSUPPORTED, "too many BVH nodes");
         }
         dev.nodes = arena.upload(nodes);
         dev.prims = arena.upload(prims);
@@ -1302,6 +1310,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
         V.win_x0 = v.win_w > 0 ? v.win_x0 : 0;
         V.win_y0 = v.win_w > 0 ? v.win_y0 : 0;
         V.win_w = v.win_w > 0 ? v.win_w : v.width;
+        V.win_h = v.win_w > 0 ? v.win_h : v.height;
         HaltonParams hp = halton_params(v.width, v.height);
         V.stride0 = hp.stride;
         V.exp0 = hp.base_exponents[0];
