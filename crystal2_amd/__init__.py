@@ -218,12 +218,17 @@ class WavefrontPathIntegrator:
     ``SampleDirection(sampler->Get1D(), sampler->Get2D(), ...)``
     (PathTracingIntegrator.cpp:79): "rtl" (GCC, MSVC) or "ltr" (clang).
     ``film``: "f32" (reference FrameBuffer), "kahan" or "f64".
+    ``integrator``: "path" (PathTracingIntegrator) or "direct" (DirectLightingIntegrator,
+    DirectLightingIntegrator.cpp:22-156, on the same kernels).
     """
 
     FILMS = {"f32": _abi.C2_FILM_F32, "kahan": _abi.C2_FILM_F32_KAHAN, "f64": _abi.C2_FILM_F64}
+    INTEGRATORS = {"path": _abi.C2_INTEGRATOR_PATH, "direct": _abi.C2_INTEGRATOR_DIRECT,
+                   "PathTracer": _abi.C2_INTEGRATOR_PATH, "DirectLight": _abi.C2_INTEGRATOR_DIRECT}
 
     def __init__(self, scene: Scene, width: int, height: int, spp: int, max_depth: int, arg_order: str = "rtl",
-                 film: str = "f32", paths_in_flight: int = 0, window=None, queue_blocks: int = 0):
+                 film: str = "f32", paths_in_flight: int = 0, window=None, queue_blocks: int = 0,
+                 integrator: str = "path"):
         self._lib = load_library()
         self.width, self.height, self.spp, self.max_depth = width, height, spp, max_depth
         o = _abi.c2_render_opts()
@@ -232,6 +237,7 @@ class WavefrontPathIntegrator:
         o.film_mode = self.FILMS[film]
         o.paths_in_flight = paths_in_flight
         o.queue_blocks = queue_blocks
+        o.integrator = self.INTEGRATORS[integrator]
         if window is not None:
             o.win_x0, o.win_y0, o.win_w, o.win_h = (int(v) for v in window)
         self.window = window
@@ -308,7 +314,7 @@ class BatchRenderer:
     """
 
     def __init__(self, items, max_depth: int, arg_order: str = "rtl", film: str = "f32",
-                 paths_in_flight: int = 0, queue_blocks: int = 0):
+                 paths_in_flight: int = 0, queue_blocks: int = 0, integrator: str = "path"):
         self._lib = load_library()
         self.items = [(sc, int(w), int(h), int(spp)) for sc, w, h, spp in items]
         self._descs = [sc.desc() for sc, _, _, _ in self.items]  # kept alive for the call
@@ -322,6 +328,7 @@ class BatchRenderer:
         o.film_mode = WavefrontPathIntegrator.FILMS[film]
         o.paths_in_flight = paths_in_flight
         o.queue_blocks = queue_blocks
+        o.integrator = WavefrontPathIntegrator.INTEGRATORS[integrator]
         self._h = C.c_void_p()
         check(self._lib.c2_batch_create(arr, len(self.items), C.byref(o), C.byref(self._h)))
 

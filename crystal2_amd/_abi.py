@@ -18,6 +18,7 @@ C2_NDF_GGX, C2_NDF_BECKMANN = 0, 1
 C2_LIGHT_AREA, C2_LIGHT_POINT = 0, 1
 C2_ARGORDER_RTL, C2_ARGORDER_LTR = 0, 1
 C2_FILM_F32, C2_FILM_F32_KAHAN, C2_FILM_F64 = 0, 1, 2
+C2_INTEGRATOR_PATH, C2_INTEGRATOR_DIRECT = 0, 1
 
 
 class c2_material(C.Structure):
@@ -61,7 +62,7 @@ class c2_render_opts(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
                 ("arg_order", C.c_int32), ("film_mode", C.c_int32), ("paths_in_flight", C.c_int32),
                 ("queue_blocks", C.c_int32), ("win_x0", C.c_int32), ("win_y0", C.c_int32), ("win_w", C.c_int32),
-                ("win_h", C.c_int32)]
+                ("win_h", C.c_int32), ("integrator", C.c_int32)]
 
 
 class c2_nrc_opts(C.Structure):

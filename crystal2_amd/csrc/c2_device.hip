@@ -48,6 +48,8 @@ namespace c2 {
 constexpr int kBlock = 256;
 static_assert(kBlock == kTravBlock, "traversal kernels index their LDS stack by threadIdx.x");
 constexpr int kMisGrid = 1024;  // blocks for the MIS-ray kernel (grid-stride; few rays)
+// Direct lighting: wavefront steps per path (transparent surfaces crossed + 1).
+constexpr int kDlChainLimit = 8;
 // Default grid cap of the per-bounce queue kernels. Their live counts are only known
 // on the device, so they are launched for the upper bound with grid-stride loops;
 // the cap keeps a shrinking queue from paying for tens of thousands of empty blocks.
@@ -152,6 +154,7 @@ struct PassParams {
     int32_t max_depth;
     int32_t arg_order;
     int32_t lmax;            // shadow slots per path: the largest eligible-light count (>= 1)
+    int32_t integrator;      // C2_INTEGRATOR_PATH or C2_INTEGRATOR_DIRECT
     const View* views;
     const int2* pv;          // views of this pass: (view, first linear index), ascending
     int32_t npv;
@@ -392,10 +395,14 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                     cache_features(S, it, d, B.nrc_feat + (int64_t)p * 4);
                     query = true;
                 } else if (it.material < 0) {
-                    // transparent: SpawnRay(dir), no dimensions consumed, :63-68
-                    V3 no = spawn_origin(it.p, it.n, d);
-                    B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
-                    cont = true;
+                    // transparent: SpawnRay(dir), no dimensions consumed, :63-68. The
+                    // direct-lighting integrator ends the path at a transparent emitter
+                    // (DirectLightingIntegrator.cpp:46-53).
+                    if (P.integrator == C2_INTEGRATOR_PATH || it.light < 0) {
+                        V3 no = spawn_origin(it.p, it.n, d);
+                        B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
+                        cont = true;
+                    }
                     flags = meta.w & F_LIGHTPATH;
                 } else {
                     if (training) {  // training row: features, throughput and radiance so far
@@ -411,17 +418,31 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                     // UniformSampleAllLights: uL = Get2D(), uB = Get2D(), :179-180
                     float uL0 = halton_value(S, hd, idx, dim), uL1 = halton_value(S, hd, idx, dim + 1);
                     float uB0 = halton_value(S, hd, idx, dim + 2), uB1 = halton_value(S, hd, idx, dim + 3);
+                    const bool direct = P.integrator == C2_INTEGRATOR_DIRECT;
                     int own = it.light;
-                    int ownElig = (own >= 0 && S.lights[own].eligible) ? 1 : 0;
+                    // the path tracer skips the hit's own light (:181-184), direct lighting
+                    // does not (DirectLightingIntegrator.cpp:68-71)
+                    int ownElig = (!direct && own >= 0 && S.lights[own].eligible) ? 1 : 0;
                     int nGet1D = V.nd_elig - ownElig;  // one Get1D per eligible non-delta light, :240
                     V3 misF{0, 0, 0}, wiB{0, 0, 0};
                     float pdfB = 0.f;
                     int fBnz = 0;
                     if (nGet1D > 0) {
-                        int typeB = 0;
-                        V3 fB = bsdf_sample(bs, it, wo, uB0, uB1, T_ALL & ~T_SPEC, wiB, pdfB, typeB);
+                        int typeB = 0, stB = kSampleOk;
+                        V3 wiFail{0, 0, 0};
+                        V3 fB = bsdf_sample(bs, it, wo, uB0, uB1, T_ALL & ~T_SPEC, wiB, pdfB, typeB, &stB, &wiFail);
                         if (!is_zero(fB)) {
                             fBnz = 1;
+                            misF = fB * fmax_ref(0.f, dot(it.n, wiB));
+                        } else if (direct && stB == kSampleZero && pdfB == 0.f) {
+                            // Direct lighting keeps going after a zero sample
+                            // (DirectLightingIntegrator.cpp:120-150): the BxDF's local-frame
+                            // wi is traced as it is, and f * Li * w / pdf is 0 / 0 when the
+                            // ray reaches the light. With a nonzero pdf the term is 0, and
+                            // with an unwritten pdf the reference reads an uninitialised
+                            // value: both add nothing and are not traced.
+                            fBnz = 1;
+                            wiB = wiFail;
                             misF = fB * fmax_ref(0.f, dot(it.n, wiB));
                         }
                     }
@@ -435,45 +456,48 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                     B.misflag[p] = 0;
                     B.mis_hit[p] = -1;
                     dim += 4 + nGet1D;
-                    // continuation: sampler->Get1D() and sampler->Get2D() as arguments of one call, :79
-                    float u2x, u2y;
-                    uint32_t sdim;
-                    if (P.arg_order == C2_ARGORDER_RTL) {
-                        sdim = (uint32_t)dim;
-                        u2x = halton_value(S, hd, idx, dim);
-                        u2y = halton_value(S, hd, idx, dim + 1);
-                    } else {
-                        sdim = (uint32_t)(dim + 1);
-                        u2x = halton_value(S, hd, idx, dim + 1);
-                        u2y = halton_value(S, hd, idx, dim + 2);
-                    }
-                    // (the Get1D value only selects the BxDF; every supported material has one)
-                    dim += 3;
-                    bs.rdim = sdim;
-                    V3 wiC{0, 0, 0};
-                    float pdfC = 0.f;
-                    int typeC = 0;
-                    V3 fC = bsdf_sample(bs, it, wo, u2x, u2y, T_ALL, wiC, pdfC, typeC);
                     flags = F_NEE;
-                    if (!(fabsf(pdfC) == 0.f || is_zero(fC))) {
-                        bool specular = (typeC & T_SPEC) != 0;
-                        float cosine = specular ? 1.0f
-                                                : fmax_ref(0.f, (typeC & T_TRANS) ? dot(-it.n, wiC) : dot(it.n, wiC));
-                        bool lp = specular || (typeC & T_TRANS);
-                        beta2 = beta * (fC * cosine / pdfC);
-                        cont = true;
-                        if (bounce > 3) {  // Russian roulette, :96-102
-                            float qq = fmax_ref(.05f, 1.f - length(beta2));
-                            float ur = halton_value(S, hd, idx, dim);
-                            dim++;
-                            if (ur < qq) cont = false;
-                            else beta2 = beta2 / (1 - qq);
+                    if (!direct) {
+                        // the path tracer continues: the direct-lighting path ends here
+                        // continuation: sampler->Get1D() and sampler->Get2D() as arguments of one call, :79
+                        float u2x, u2y;
+                        uint32_t sdim;
+                        if (P.arg_order == C2_ARGORDER_RTL) {
+                            sdim = (uint32_t)dim;
+                            u2x = halton_value(S, hd, idx, dim);
+                            u2y = halton_value(S, hd, idx, dim + 1);
+                        } else {
+                            sdim = (uint32_t)(dim + 1);
+                            u2x = halton_value(S, hd, idx, dim + 1);
+                            u2y = halton_value(S, hd, idx, dim + 2);
                         }
-                        if (cont) {
-                            V3 no = spawn_origin(it.p, it.n, wiC);
-                            B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
-                            B.ray_d[p] = make_float4(wiC.x, wiC.y, wiC.z, 0.f);
-                            if (lp) flags |= F_LIGHTPATH;
+                        // (the Get1D value only selects the BxDF; every supported material has one)
+                        dim += 3;
+                        bs.rdim = sdim;
+                        V3 wiC{0, 0, 0};
+                        float pdfC = 0.f;
+                        int typeC = 0;
+                        V3 fC = bsdf_sample(bs, it, wo, u2x, u2y, T_ALL, wiC, pdfC, typeC);
+                        if (!(fabsf(pdfC) == 0.f || is_zero(fC))) {
+                            bool specular = (typeC & T_SPEC) != 0;
+                            float cosine = specular ? 1.0f
+                                                    : fmax_ref(0.f, (typeC & T_TRANS) ? dot(-it.n, wiC) : dot(it.n, wiC));
+                            bool lp = specular || (typeC & T_TRANS);
+                            beta2 = beta * (fC * cosine / pdfC);
+                            cont = true;
+                            if (bounce > 3) {  // Russian roulette, :96-102
+                                float qq = fmax_ref(.05f, 1.f - length(beta2));
+                                float ur = halton_value(S, hd, idx, dim);
+                                dim++;
+                                if (ur < qq) cont = false;
+                                else beta2 = beta2 / (1 - qq);
+                            }
+                            if (cont) {
+                                V3 no = spawn_origin(it.p, it.n, wiC);
+                                B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
+                                B.ray_d[p] = make_float4(wiC.x, wiC.y, wiC.z, 0.f);
+                                if (lp) flags |= F_LIGHTPATH;
+                            }
                         }
                     }
                 }
@@ -578,7 +602,7 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
             const int slot = p * E + j;
             bool shadow = false;
             uint8_t v = 0;
-            if (nee && lj && it.light != li) {
+            if (nee && lj && (P.integrator == C2_INTEGRATOR_DIRECT || it.light != li)) {
                 const DevLight Lt = S.lights[li];
                 // (i) light sampling, :199-230
                 V3 lightPos;
@@ -650,7 +674,7 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassP
         if (traverse<false>(S, P.views[B.vid[p]].root, o, wi, 0.f, INFINITY, -1, h, lds)) {
             int4 e = S.entities[h.entity];
             int li = e.w;
-            if (li >= 0 && li != own) {
+            if (li >= 0 && (P.integrator == C2_INTEGRATOR_DIRECT || li != own)) {
                 const DevLight& Lt = S.lights[li];
                 if (Lt.eligible && Lt.type == C2_LIGHT_AREA) {
                     float pdfL = pdf_light(S, Lt, sp, sn, wi);
@@ -1259,6 +1283,8 @@ void check_opts(const c2_render_opts* opts) {
         fail(C2_ERR_INVALID, "render options: bad film_mode");
     if (opts->queue_blocks < 0 || opts->paths_in_flight < 0)
         fail(C2_ERR_INVALID, "render options: negative queue_blocks / paths_in_flight");
+    if (opts->integrator != C2_INTEGRATOR_PATH && opts->integrator != C2_INTEGRATOR_DIRECT)
+        fail(C2_ERR_INVALID, "render options: bad integrator");
 }
 
 // Builds a renderer over n scenes (one view each) sharing one concatenated scene,
@@ -1282,6 +1308,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     PassParams& P = pt->pp;
     P.max_depth = opts->max_depth;
     P.arg_order = opts->arg_order;
+    P.integrator = opts->integrator;
     P.cache_bounce = -1;
     P.train_ratio = 1;
     P.train_phase = 0;
@@ -1537,7 +1564,13 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                 pt->launches++;
                 int q = 0;
                 if (S.num_entities > 0) {
-                    for (int b = 0; b < P.max_depth; b++) {
+                    // Direct lighting: one vertex, reached through at most kDlChainLimit - 1
+                    // transparent surfaces (the reference recurses through them without
+                    // consuming depth); depth 0 renders black, as level == maxDepth at
+                    // level 0 does.
+                    const int steps = P.integrator == C2_INTEGRATOR_DIRECT ? (P.max_depth > 0 ? kDlChainLimit : 0)
+                                                                          : P.max_depth;
+                    for (int b = 0; b < steps; b++) {
                         hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, pt->B, q ^ 1);
                         if (pt->cap_bounce == b && pt->cap_kind == 0) {
                             hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, pt->B, 0, q,
@@ -1742,6 +1775,7 @@ int c2_pt_set_cache(c2_pt* pt, c2_nrc* nrc, int32_t cache_bounce, int32_t train_
             pt->pp.cache_bounce = -1;
             return;
         }
+        if (pt->pp.integrator != C2_INTEGRATOR_PATH) fail(C2_ERR_INVALID, "cache: path tracer only");
         if (cache_bounce < 0 || cache_bounce >= pt->pp.max_depth || train_ratio < 1)
             fail(C2_ERR_INVALID, "cache: bounce must be in [0, depth) and train_ratio >= 1");
         c2_nrc_stats st{};

@@ -543,15 +543,34 @@ __device__ __forceinline__ float bsdf_pdf(const Bsdf& b, const Inter& it, V3 woW
 }
 // BSDF::SampleDirection with one BxDF, BSDF.cpp:64-130. Returns f; sets wi (world),
 // pdf and sampled type; f = 0 when the BxDF does not match or sampling fails.
+// status (optional): kSampleOk; kSampleNoMatch (no BxDF matches `flags`: the reference
+// returns before writing wi, pdf and the sampled type); kSampleNoPdf (the BxDF returned
+// before writing wi and pdf); kSampleZero (the BxDF wrote wi, in the local frame, and
+// pdf, but f or pdf is 0: *wiFail gets that local wi).
+enum : int { kSampleOk = 0, kSampleNoMatch = 1, kSampleNoPdf = 2, kSampleZero = 3 };
 __device__ __forceinline__ V3 bsdf_sample(const Bsdf& b, const Inter& it, V3 woW, float sx, float sy, int flags,
-                                          V3& wiW, float& pdf, int& sampled) {
-    if (b.kind == BK_NONE || !matches(b.type, flags)) return V3{0, 0, 0};
+                                          V3& wiW, float& pdf, int& sampled, int* status = nullptr,
+                                          V3* wiFail = nullptr) {
+    if (b.kind == BK_NONE || !matches(b.type, flags)) {
+        if (status) *status = kSampleNoMatch;
+        return V3{0, 0, 0};
+    }
     sampled = b.type;
     V3 wo = clamp_unit(to_local(it, woW));
     V3 wi{0, 0, 0};
-    pdf = 0.f;
+    pdf = -1.f;  // "not written" (a pdf is never negative)
     V3 L = bxdf_sample(b, sx, sy, wo, wi, pdf, sampled);
-    if (is_zero(L) || pdf == 0.f) return V3{0, 0, 0};
+    if (pdf == -1.f) {
+        pdf = 0.f;
+        if (status) *status = kSampleNoPdf;
+        return V3{0, 0, 0};
+    }
+    if (is_zero(L) || pdf == 0.f) {
+        if (status) *status = kSampleZero;
+        if (wiFail) *wiFail = wi;
+        return V3{0, 0, 0};
+    }
+    if (status) *status = kSampleOk;
     wiW = clamp_unit(to_world(it, wi));
     pdf /= 1;
     return L;

@@ -195,6 +195,9 @@ typedef struct c2_pt c2_pt;
 
 enum { C2_ARGORDER_RTL = 0, C2_ARGORDER_LTR = 1 };
 enum { C2_FILM_F32 = 0, C2_FILM_F32_KAHAN = 1, C2_FILM_F64 = 2 };
+/* PathTracingIntegrator (PathTracingIntegrator.cpp:28-278) or DirectLightingIntegrator
+ * (DirectLightingIntegrator.cpp:22-156, transparent chains limited to 7 surfaces). */
+enum { C2_INTEGRATOR_PATH = 0, C2_INTEGRATOR_DIRECT = 1 };
 
 typedef struct c2_render_opts {
     int32_t width, height;
@@ -208,6 +211,8 @@ typedef struct c2_render_opts {
                                  which loop grid-stride; 0 = the library default     */
     int32_t win_x0, win_y0;   /* pixel window (film coordinates, y = 0 at the bottom); */
     int32_t win_w, win_h;     /* win_w = 0: the whole film                             */
+    int32_t integrator;       /* C2_INTEGRATOR_*: "PathTracer" or "DirectLight"
+                                 (src/Integrators/Integrator.cpp:27-40)               */
 } c2_render_opts;
 
 typedef struct c2_pt_stats {

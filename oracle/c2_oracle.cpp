@@ -63,6 +63,10 @@ constexpr float HALF_PI = 1.57079632679489661923132169163975144f;
 constexpr float ONE_MINUS_EPS = 0x1.fffffep-1f;
 constexpr float EPS_UTIL = 1e-5f;  // src/Core/Utils.h:7
 constexpr float EPS_SPAWN = 1e-4f; // src/Core/SurfaceInteraction.cpp:6
+// DirectLightingIntegrator: transparent surfaces crossed before the first surface with a
+// BSDF. The reference recurses without a bound; the GPU path runs this many wavefront
+// steps, and the oracle stops at the same count.
+constexpr int kDlChainLimit = 8;
 
 // ------------------------------------------------------------------ glm-order vectors
 struct vec3 {
@@ -814,6 +818,7 @@ struct orc_scene {
     c2_camera cam;
     int accel = ORC_ACCEL_BVH;
     int tie = ORC_TIE_FIRST;
+    int integrator = ORC_INTEGRATOR_PATH;
     // median-split BVH, src/Accelerators/BVH.cpp:13-34,45-54,156-188,269-355
     struct Node {
         Box box;
@@ -1135,6 +1140,100 @@ struct orc_scene {
         return L;
     }
 
+    // DirectLightingIntegrator::EsimateDirect, DirectLightingIntegrator.cpp:77-156. Part (i)
+    // equals the path tracer's. Part (ii) has no early return for a zero BSDF sample,
+    // so a sample that failed inside the BxDF still traces its direction: BSDF::
+    // SampleDirection (BSDF.cpp:68-96) then leaves wi in the local frame and pdf at the
+    // BxDF's value, and f = 0 adds f * Li * w / pdf = 0, or NaN when that pdf is 0 and
+    // the local direction reaches this light (the sample is then discarded by
+    // FrameBuffer::AddSample). Two failures leave pdf_bsdf unwritten, and the reference
+    // then reads it uninitialised: no non-specular BxDF (SampleDirection returns before
+    // writing pdf and sampledType), and a microfacet sample whose half vector faces
+    // away or whose refraction fails (Microfacet.cpp:46,108,113-123,167 return before
+    // writing). This restatement takes those terms as 0, which is what any finite,
+    // nonzero leftover value gives; the reference's own films agree.
+    vec3 estimateDirectDL(const Isec& is, const BSDF& bsdf, vec2 uL, vec2 uB, int li, Sampler& smp) const {
+        const Light& light = lights[li];
+        vec3 L(0.f);
+        const int sampleType = ALL & ~SPEC;
+        vec3 P = is.p, N = is.n, wo = -is.dir;
+        SurfInfo surface{is.p, is.n};
+        vec3 lightPos;
+        float pdfL;
+        vec3 Li = sampleLi(light, surface, uL, &lightPos, &pdfL);
+        vec3 wi = normalize(lightPos - P);
+        bool delta = light.type == 1;
+        if (pdfL != 0.f && Li != vec3(0.f)) {
+            vec3 f = bsdf.f(wo, wi) * smax(0.f, dot(N, wi));
+            float pdfB = bsdf.pdf(wo, wi, sampleType);
+            if (f != vec3(0.f)) {
+                int ignore = delta ? -1 : light.entity;
+                if (intersectTest(is.spawnTo(lightPos), ignore, 0, 1 - EPS_UTIL)) Li = vec3(0.f);
+                if (Li != vec3(0.f)) {
+                    if (delta) L += f * Li / pdfL;
+                    else L += f * Li * powerHeuristic(1, pdfL, 1, pdfB) / pdfL;
+                }
+            }
+        }
+        if (!delta) {
+            // sentinels: sampled stays -1 when no BxDF matches, pdfB stays -1 when the
+            // BxDF returned before writing its pdf (the two uninitialised reads)
+            float pdfB = -1.f;
+            int sampled = -1;
+            vec3 f = bsdf.sample(smp.get1D(), uB, wo, &wi, &pdfB, sampleType, &sampled);
+            if (sampled == -1 || pdfB == -1.f) return L;
+            bool specB = (sampled & SPEC) != 0;
+            float weight = 1.0f;
+            if (!specB) {
+                f *= smax(0.f, dot(N, wi));
+                pdfL = pdfLight(light.entity, surface, wi);
+                if (pdfL == 0.f) return L;
+                weight = powerHeuristic(1, pdfB, 1, pdfL);
+            }
+            Isec li2;
+            vec3 Lb(0.f);
+            if (intersect(is.spawn(wi), &li2)) {
+                if (this->light[li2.entity] == li) Lb = Le(li2, -wi);
+            }  // a miss: Light::Le of an area light is 0
+            if (Lb != vec3(0.f)) L += f * Lb * weight / pdfB;
+        }
+        return L;
+    }
+
+    // DirectLightingIntegrator::UniformSampleAllLights, :62-73 (the hit's own light included)
+    vec3 sampleAllLightsDL(const Isec& is, const BSDF& bsdf, Sampler& smp) const {
+        vec3 L(0.f);
+        vec2 uL = smp.get2D();
+        vec2 uB = smp.get2D();
+        for (int li = 0; li < (int)lights.size(); li++) {
+            if (lights[li].flux() == vec3(0.f)) continue;
+            L += estimateDirectDL(is, bsdf, uL, uB, li, smp);
+        }
+        return L;
+    }
+
+    // DirectLightingIntegrator::Evaluate / eval_rec, :22-58: emission at the hit, then direct
+    // light at the first surface with a BSDF. A transparent hit recurses at the same
+    // level (no depth is consumed) unless its entity carries an area light. Depth only
+    // matters when it is 0 (level == maxDepth at level 0).
+    vec3 evaluateDL(Ray ray, Sampler& smp, int maxDepth) const {
+        if (maxDepth == 0) return vec3(0.f);
+        for (int chain = 0; chain < kDlChainLimit; chain++) {
+            Isec is;
+            if (!intersect(ray, &is)) return vec3(0.f);  // no skybox on this path
+            vec3 L = Le(is, -ray.d);
+            BSDF bsdf;
+            buildBSDF(is, bsdf, smp.idx, smp.dim + 2);  // uB's first dimension
+            if (bsdf.empty()) {
+                if (light[is.entity] >= 0) return L;
+                ray = is.spawn(ray.d);  // eval_rec returns the deeper result alone
+                continue;
+            }
+            return L + sampleAllLightsDL(is, bsdf, smp);
+        }
+        return vec3(0.f);
+    }
+
     // PathTracingIntegrator::Evaluate, :28-111
     vec3 evaluate(Ray ray, Sampler& smp, int maxDepth, int argOrder) const {
         vec3 beta(1.f), L(0.f);
@@ -1286,6 +1385,12 @@ int orc_scene_create(const c2_scene_desc* d, int accel, int tie, orc_scene** out
 
 void orc_scene_destroy(orc_scene* s) { delete s; }
 
+int orc_scene_set_integrator(orc_scene* s, int integrator) {
+    if (!s || (integrator != ORC_INTEGRATOR_PATH && integrator != ORC_INTEGRATOR_DIRECT)) return setErr("bad integrator");
+    s->integrator = integrator;
+    return C2_OK;
+}
+
 static int renderImpl(orc_scene* s, int W, int H, int spp, int depth, int argOrder, int threads, int y0, int y1,
                       float* film, float* samples, double* seconds) {
     if (!s || W <= 0 || H <= 0 || spp <= 0) return setErr("bad arguments");
@@ -1317,7 +1422,8 @@ static int renderImpl(orc_scene* s, int W, int H, int spp, int depth, int argOrd
                         vec2 pos((float)xx + sv.x, (float)yy + sv.y);
                         pos.x = pos.x / W;
                         pos.y = pos.y / H;
-                        vec3 c = s->evaluate(cam.generate(pos), smp, depth, argOrder);
+                        vec3 c = s->integrator == ORC_INTEGRATOR_DIRECT ? s->evaluateDL(cam.generate(pos), smp, depth)
+                                                                         : s->evaluate(cam.generate(pos), smp, depth, argOrder);
                         if (samples) {
                             float* o = samples + (((size_t)yy * W + xx) * spp + k) * 3;
                             o[0] = c.x; o[1] = c.y; o[2] = c.z;

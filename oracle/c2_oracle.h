@@ -35,6 +35,10 @@ enum { ORC_TIE_FIRST = 0, ORC_TIE_LOWEST = 1 };
 
 int  orc_scene_create(const c2_scene_desc* desc, int accel, int tie_rule, orc_scene** out);
 void orc_scene_destroy(orc_scene* s);
+/* Integrator of orc_render: ORC_INTEGRATOR_PATH (PathTracingIntegrator, the default)
+ * or ORC_INTEGRATOR_DIRECT (DirectLightingIntegrator). */
+enum { ORC_INTEGRATOR_PATH = 0, ORC_INTEGRATOR_DIRECT = 1 };
+int  orc_scene_set_integrator(orc_scene* s, int integrator);
 const char* orc_last_error(void);
 
 /* Render film rows [y_begin, y_end) (film coordinates, y = 0 at the bottom)

@@ -17,6 +17,7 @@ REF_BIN = os.path.join(HERE, "_ref", "crystal2_ref")
 
 ACCEL_BVH, ACCEL_BRUTE = 0, 1
 TIE_FIRST, TIE_LOWEST = 0, 1
+INTEGRATORS = {"path": 0, "direct": 1}  # PathTracingIntegrator, DirectLightingIntegrator
 
 _loaded = {}
 
@@ -36,6 +37,7 @@ def lib(variant: str = "portable"):
     L = C.CDLL(path)
     L.orc_scene_create.argtypes = [C.POINTER(_abi.c2_scene_desc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     L.orc_scene_destroy.argtypes = [C.c_void_p]
+    L.orc_scene_set_integrator.argtypes = [C.c_void_p, C.c_int]
     L.orc_last_error.restype = C.c_char_p
     L.orc_render.argtypes = [C.c_void_p] + [C.c_int] * 8 + [C.POINTER(C.c_float), C.POINTER(C.c_double)]
     L.orc_render_samples.argtypes = [C.c_void_p] + [C.c_int] * 6 + [C.POINTER(C.c_float)]
@@ -50,13 +52,16 @@ def lib(variant: str = "portable"):
 
 
 class OracleScene:
-    def __init__(self, scene, accel: int = ACCEL_BVH, tie: int = TIE_FIRST, variant: str = "portable"):
+    def __init__(self, scene, accel: int = ACCEL_BVH, tie: int = TIE_FIRST, variant: str = "portable",
+                 integrator: str = "path"):
         self._L = lib(variant)
         self._scene = scene  # keeps the product-side arrays alive
         d = scene.desc()
         self._h = C.c_void_p()
         rc = self._L.orc_scene_create(C.byref(d), accel, tie, C.byref(self._h))
         if rc != 0:
+            raise RuntimeError(self._L.orc_last_error().decode())
+        if self._L.orc_scene_set_integrator(self._h, INTEGRATORS[integrator]) != 0:
             raise RuntimeError(self._L.orc_last_error().decode())
 
     def render(self, width, height, spp, depth, arg_order="rtl", threads=None, y_range=None):

@@ -7,7 +7,8 @@ outputs on them. The reference binary is built with clang, which evaluates
 SampleDirection(Get1D(), Get2D()) left to right, so every film here is an
 arg_order="ltr" film.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # every fixture
+    python tests/golden/make_golden.py direct   # DirectLighting films only
 """
 import os
 import subprocess
@@ -29,12 +30,23 @@ def ref(*args, cwd):
     return out.stdout
 
 
-def film(name, w, h):
+def film(name, w, h, config="config.json", tag=""):
     d = os.path.join(SCN, name)
-    ref("render", "config.json", "scene.json", "film.f32", cwd=d)
+    ref("render", config, "scene.json", "film.f32", cwd=d)
     f = np.fromfile(os.path.join(d, "film.f32"), np.float32).reshape(h, w, 3)
     os.remove(os.path.join(d, "film.f32"))
-    np.save(os.path.join(HERE, f"film_{name}.npy"), f)
+    np.save(os.path.join(HERE, f"film_{name}{tag}.npy"), f)
+
+
+def direct_lighting():
+    """DirectLightingIntegrator films of the existing golden scenes (config_dl.json
+    beside each scene.json); leaves every other fixture untouched."""
+    if not os.path.exists(REF):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    for name, spp in (("cornell", 4), ("zoo", 8), ("zoo_nophong", 8)):
+        scenes.write_config(os.path.join(SCN, name, "config_dl.json"), 64, 64, spp, 6, integrator="DirectLight")
+        film(name, 64, 64, config="config_dl.json", tag="_dl")
+    print("DirectLighting films written to", HERE)
 
 
 def hits(name, rays):
@@ -109,4 +121,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["direct"]:
+        direct_lighting()
+    else:
+        main()

@@ -80,3 +80,21 @@ def test_batch_small_chunks(gpu_available):
     for i, r in enumerate(ref):
         assert np.array_equal(b.film(i), r)
     b.close()
+
+
+def test_batch_direct_lighting(gpu_available):
+    print("case batch direct lighting", flush=True)
+    items = _items()[:3]
+    ref = []
+    for sc, w, h, spp in items:
+        pt = c2.WavefrontPathIntegrator(sc, w, h, spp, DEPTH, arg_order="rtl", integrator="direct")
+        pt.render()
+        pt.sync()
+        ref.append(pt.film())
+        pt.close()
+    b = c2.BatchRenderer(items, DEPTH, arg_order="rtl", integrator="direct")
+    b.render(True)
+    b.sync()
+    for i, r in enumerate(ref):
+        assert np.array_equal(b.film(i), r)
+    b.close()

@@ -151,3 +151,28 @@ def test_pixel_window_matches_full_frame(gpu_available):
             assert np.array_equal(f[rows, 8:32], full[rows, 8:32])
         else:
             assert relmse(f[rows, 8:32], full[rows, 8:32]) < 1e-10
+
+
+@pytest.mark.parametrize("name,spp", [("cornell", 4), ("zoo_nophong", 8), ("zoo", 8)])
+def test_direct_lighting_bit_identical_to_oracle(gpu_available, name, spp):
+    """DirectLightingIntegrator on the wavefront kernels against the oracle's restatement."""
+    print(f"case direct lighting {name}", flush=True)
+    sc = load(name)
+    g = gpu_film(sc, 64, 64, spp, 6, "ltr", integrator="direct")
+    o, _ = pyoracle.OracleScene(sc, tie=pyoracle.TIE_LOWEST, integrator="direct").render(64, 64, spp, 6,
+                                                                                          arg_order="ltr")
+    assert (g == o).all(-1).mean() >= 0.999
+    assert relmse(g, o) <= 1e-6
+
+
+@pytest.mark.parametrize("name,spp,tol", [("cornell", 4, 1e-4), ("zoo_nophong", 8, 1e-4), ("zoo", 8, 1e-4)])
+def test_direct_lighting_vs_reference_golden(gpu_available, name, spp, tol):
+    print(f"case direct lighting golden {name}", flush=True)
+    g = gpu_film(load(name), 64, 64, spp, 6, "ltr", integrator="direct")
+    ref = np.load(os.path.join(GOLDEN, f"film_{name}_dl.npy"))
+    assert relmse(g, ref) <= tol
+
+
+def test_direct_lighting_depth_zero_is_black(gpu_available):
+    g = gpu_film(load("cornell"), 32, 32, 2, 0, integrator="direct")
+    assert not g[np.isfinite(g)].any()

@@ -130,3 +130,18 @@ def test_pow5_product_equals_rounded_double_pow():
     prod = (d2 * d2 * d).astype(np.float32)
     ref = np.power(d, 5.0).astype(np.float32)
     assert np.array_equal(prod, ref)
+
+
+@pytest.mark.parametrize("name,exact", [("cornell", True), ("zoo_nophong", True), ("zoo", False)])
+def test_oracle_direct_lighting_matches_reference(name, exact):
+    """DirectLightingIntegrator films of the reference binary (film_*_dl.npy): bit-identical
+    for the libm variant; with Phong the rejection redraws agree statistically only."""
+    sc = load(name)
+    spp = 4 if name == "cornell" else 8
+    film, _ = pyoracle.OracleScene(sc, variant="libm", integrator="direct").render(64, 64, spp, 6, arg_order="ltr")
+    ref = np.load(os.path.join(GOLDEN, f"film_{name}_dl.npy"))
+    if exact:
+        assert np.array_equal(film, ref)
+    else:
+        assert relmse(film, ref) < 1e-6
+        assert (film == ref).all(-1).mean() > 0.99
