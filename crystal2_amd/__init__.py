@@ -80,7 +80,10 @@ class Scene:
         check(lib.c2_scene_create(C.byref(h)))
         return cls(h)
 
-    def add_material(self, mtype: int, color, roughness=(0.0, 0.0), ior=1.0, phong_k=0, ndf=0) -> int:
+    def add_material(self, mtype: int, color, roughness=(0.0, 0.0), ior=1.0, phong_k=0, ndf=0, ks=(0.0, 0.0, 0.0),
+                     sigma=0.0) -> int:
+        """c2_material fields: `color` is the albedo (Plastic: Kd), `ks` Plastic's Ks,
+        `sigma` OrenNayar's Sigma, `ndf` 0 = GGX, 1 = Beckmann (Micro, Plastic)."""
         m = _abi.c2_material()
         m.type = mtype
         m.ndf = ndf
@@ -88,6 +91,8 @@ class Scene:
         m.color[:] = [float(x) for x in color]
         m.roughness[:] = [float(x) for x in roughness]
         m.ior = float(ior)
+        m.ks[:] = [float(x) for x in ks]
+        m.sigma = float(sigma)
         idx = C.c_int32()
         check(self._lib.c2_scene_add_material(self._h, C.byref(m), C.byref(idx)))
         return idx.value

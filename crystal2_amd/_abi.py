@@ -13,7 +13,7 @@ ERRORS = {-1: "C2_ERR_INVALID", -2: "C2_ERR_PARSE", -3: "C2_ERR_IO", -4: "C2_ERR
           -5: "C2_ERR_NOMEM", -6: "C2_ERR_UNSUPPORTED"}
 
 C2_SHAPE_TRIANGLE, C2_SHAPE_SPHERE = 0, 1
-C2_MAT_DIFFUSE, C2_MAT_MIRROR, C2_MAT_GLASS, C2_MAT_MICRO, C2_MAT_PHONG = range(5)
+C2_MAT_DIFFUSE, C2_MAT_MIRROR, C2_MAT_GLASS, C2_MAT_MICRO, C2_MAT_PHONG, C2_MAT_PLASTIC, C2_MAT_ORENNAYAR = range(7)
 C2_NDF_GGX, C2_NDF_BECKMANN = 0, 1
 C2_LIGHT_AREA, C2_LIGHT_POINT = 0, 1
 C2_ARGORDER_RTL, C2_ARGORDER_LTR = 0, 1
@@ -23,7 +23,8 @@ C2_INTEGRATOR_PATH, C2_INTEGRATOR_DIRECT = 0, 1
 
 class c2_material(C.Structure):
     _fields_ = [("type", C.c_int32), ("ndf", C.c_int32), ("phong_k", C.c_int32), ("pad_", C.c_int32),
-                ("color", C.c_float * 3), ("roughness", C.c_float * 2), ("ior", C.c_float)]
+                ("color", C.c_float * 3), ("roughness", C.c_float * 2), ("ior", C.c_float),
+                ("ks", C.c_float * 3), ("sigma", C.c_float)]
 
 
 class c2_light(C.Structure):

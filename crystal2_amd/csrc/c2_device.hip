@@ -173,7 +173,8 @@ __device__ __forceinline__ void cache_features(const DevScene& S, const Inter& i
         const c2_material m = S.materials[it.material];
         alb = V3{m.color[0], m.color[1], m.color[2]};
         if (m.type == C2_MAT_MIRROR) rough = 0.f;
-        else if (m.type == C2_MAT_GLASS || m.type == C2_MAT_MICRO) rough = sqrtf(m.roughness[0] * m.roughness[1]);
+        else if (m.type == C2_MAT_GLASS || m.type == C2_MAT_MICRO || m.type == C2_MAT_PLASTIC)
+            rough = sqrtf(m.roughness[0] * m.roughness[1]);
         else if (m.type == C2_MAT_PHONG) rough = sqrtf(2.f / (m.phong_k + 2.f));
     }
     out[0] = make_float4(it.p.x, it.p.y, it.p.z, -d.x);
@@ -564,7 +565,7 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
             root = V.root;
         }
         Inter it{};
-        Bsdf bs{};
+        uint32_t hidx = 0;  // Halton index: the BSDF's Phong redraw stream
         V3 wo{0, 0, 0}, misDir{0, 0, 0};
         float uL0 = 0.f, uL1 = 0.f;
         bool misCand = false;
@@ -581,7 +582,7 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
             it.material = vi.y;
             it.light = vi.z;
             it.front = vi.w != 0;
-            bs = make_bsdf(S, it, (uint32_t)meta.y, 0u);
+            hidx = (uint32_t)meta.y;
             wo = -it.dir;
             uL0 = vp.w;
             uL1 = vn.w;
@@ -604,6 +605,9 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
             uint8_t v = 0;
             if (nee && lj && (P.integrator == C2_INTEGRATOR_DIRECT || it.light != li)) {
                 const DevLight Lt = S.lights[li];
+                // rebuilt per light from the (cached) material record rather than kept
+                // live across the loop: fewer registers, more waves per SIMD
+                const Bsdf bs = make_bsdf(S, it, hidx, 0u);
                 // (i) light sampling, :199-230
                 V3 lightPos;
                 float pdfL;
@@ -840,8 +844,10 @@ static void validate_desc(const c2_scene_desc* d) {
         if (L.type != C2_LIGHT_AREA && L.type != C2_LIGHT_POINT) fail(C2_ERR_INVALID, "unknown light type");
     }
     for (int m = 0; m < d->num_materials; m++) {
-        if (d->materials[m].type == C2_MAT_MICRO && d->materials[m].ndf != C2_NDF_GGX)
-            fail(C2_ERR_UNSUPPORTED, "Micro material: only GGX");
+        const c2_material& M = d->materials[m];
+        if (M.type < C2_MAT_DIFFUSE || M.type > C2_MAT_ORENNAYAR) fail(C2_ERR_INVALID, "invalid material type");
+        if ((M.type == C2_MAT_MICRO || M.type == C2_MAT_PLASTIC) && M.ndf != C2_NDF_GGX && M.ndf != C2_NDF_BECKMANN)
+            fail(C2_ERR_INVALID, "invalid NDF");
     }
     if ((int64_t)d->num_entities >= (1ll << kSlotBits)) fail(C2_ERR_UNSUPPORTED, "too many entities");
 }

@@ -390,9 +390,9 @@ static M4 m4_inverse(const M4& M) {
 // ----------------------------------------------------------------------------------- scene ops
 
 static int add_material(SceneData& d, const c2_material& m) {
-    if (m.type < C2_MAT_DIFFUSE || m.type > C2_MAT_PHONG) fail(C2_ERR_INVALID, "invalid material type");
-    if (m.type == C2_MAT_MICRO && m.ndf != C2_NDF_GGX)
-        fail(C2_ERR_UNSUPPORTED, "Micro material: only the GGX NDF is supported");
+    if (m.type < C2_MAT_DIFFUSE || m.type > C2_MAT_ORENNAYAR) fail(C2_ERR_INVALID, "invalid material type");
+    if ((m.type == C2_MAT_MICRO || m.type == C2_MAT_PLASTIC) && m.ndf != C2_NDF_GGX && m.ndf != C2_NDF_BECKMANN)
+        fail(C2_ERR_INVALID, "invalid NDF");
     d.materials.push_back(m);
     return (int)d.materials.size() - 1;
 }
@@ -537,6 +537,12 @@ static std::string join_path(const std::string& base, const std::string& rel) {
 }
 
 // src/Materials/Material.cpp:22-126
+static int32_t parse_ndf(const std::string& ndf) {  // Material.cpp:52-64
+    if (ndf == "GGX") return C2_NDF_GGX;
+    if (ndf == "Beckmann") return C2_NDF_BECKMANN;
+    fail(C2_ERR_PARSE, "Cannot find NDF Type: " + ndf);
+}
+
 static c2_material parse_material(const JNode& n) {
     c2_material m{};
     const std::string& type = n.member("Type").as_string();
@@ -563,16 +569,25 @@ static c2_material parse_material(const JNode& n) {
         V2 r = jvec2(n.member("Roughness"));
         m.roughness[0] = r.x; m.roughness[1] = r.y;
         m.ior = (float)n.member("IOR").as_float();
-        const std::string& ndf = n.member("NDF").as_string();
-        if (ndf == "GGX") m.ndf = C2_NDF_GGX;
-        else if (ndf == "Beckmann") fail(C2_ERR_UNSUPPORTED, "Beckmann NDF is not supported");
-        else fail(C2_ERR_PARSE, "Cannot find NDF Type: " + ndf);
+        m.ndf = parse_ndf(n.member("NDF").as_string());
+    } else if (type == "Plastic") {  // Material.cpp:70-89: Kd and Ks are colours here
+        m.type = C2_MAT_PLASTIC;
+        V3 kd = jvec3(n.member("Kd")), ks = jvec3(n.member("Ks"));
+        m.color[0] = kd.x; m.color[1] = kd.y; m.color[2] = kd.z;
+        m.ks[0] = ks.x; m.ks[1] = ks.y; m.ks[2] = ks.z;
+        V2 r = jvec2(n.member("Roughness"));
+        m.roughness[0] = r.x; m.roughness[1] = r.y;
+        m.ndf = parse_ndf(n.member("NDF").as_string());
     } else if (type == "Phong") {
         m.type = C2_MAT_PHONG;
         m.phong_k = (int32_t)n.member("K").as_int();
         color();
-    } else if (type == "Plastic" || type == "OrenNayar") {
-        fail(C2_ERR_UNSUPPORTED, "material type " + type + " is not supported");
+    } else if (type == "OrenNayar") {  // Material.cpp:104-120 (a Kd texture is "Not implemented")
+        m.type = C2_MAT_ORENNAYAR;
+        if (n.has("Kd")) fail(C2_ERR_UNSUPPORTED, "OrenNayar with a Kd texture is not implemented");
+        V3 c = jvec3(n.member("Color"));
+        m.color[0] = c.x; m.color[1] = c.y; m.color[2] = c.z;
+        m.sigma = (float)n.member("Sigma").as_float();
     } else {
         fail(C2_ERR_PARSE, "Invalid material name");
     }

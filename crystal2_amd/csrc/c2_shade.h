@@ -224,13 +224,16 @@ __device__ __forceinline__ float power_heuristic(float fPdf, float gPdf) {  // U
 
 // ------------------------------------------------------------------ BxDFs
 enum : int { T_REFL = 1, T_TRANS = 2, T_GLOSSY = 4, T_SPEC = 8, T_DIFF = 16, T_ALL = 0x1F };
-enum : int { BK_NONE, BK_LAMBERT, BK_SPECREFL, BK_SPECFRESNEL, BK_MFREFL, BK_MFBLEND, BK_PHONG };
+enum : int { BK_NONE, BK_LAMBERT, BK_SPECREFL, BK_SPECFRESNEL, BK_MFREFL, BK_MFBLEND, BK_PHONG, BK_ORENNAYAR, BK_FBLEND };
 enum : int { FR_NOOP, FR_DIELECTRIC, FR_SCHLICK };
 
 struct Bsdf {
     int kind, type, fresnel, k;
+    int ndf;        // C2_NDF_* of the microfacet lobes
     V3 R;
+    V3 S;           // FresnelBlend: specular reflectance Ks
     float etaA, etaB, ax, ay;
+    float onA, onB; // OrenNayarReflection's A and B
     uint32_t ridx, rdim;  // Blinn-Phong redraw stream
 };
 
@@ -296,6 +299,63 @@ __device__ __forceinline__ V3 ggx_sample_wh(const Bsdf& b, float sx, float sy) {
 }
 __device__ __forceinline__ float ggx_pdf(const Bsdf& b, V3 wh) { return ggx_D(b, wh) * fabsf(wh.y); }
 
+// BeckmannDistribution, MicrofacetDistribution.cpp:84-161 (alpha = roughness as given:
+// the constructor does not call RoughnessToAlpha)
+__device__ __forceinline__ float beck_D(const Bsdf& b, V3 wh) {
+    float cos2 = wh.y * wh.y;
+    float sin2 = 1.f - cos2;
+    float tan2 = sin2 / cos2;
+    if (is_inf(tan2)) return 0.f;
+    float cos4 = cos2 * cos2;
+    float pc, ps;
+    cos_sin_phi(wh, sqrtf(sin2), pc, ps);
+    float bot = kPi * b.ax * b.ay * cos4;
+    float top = exp_r(-tan2 * (sq(pc) / sq(b.ax) + sq(ps) / sq(b.ay)));
+    return top / bot;
+}
+__device__ __forceinline__ float beck_lambda(const Bsdf& b, V3 w) {
+    float cosTheta = w.y;
+    float sinTheta = sqrtf(1.f - cosTheta * cosTheta);
+    float tanTheta = fabsf(sinTheta / cosTheta);
+    if (is_inf(tanTheta)) return 0.f;
+    float pc, ps;
+    cos_sin_phi(w, sinTheta, pc, ps);
+    float alpha = sqrtf(sq(pc) * sq(b.ax) + sq(ps) * sq(b.ay));
+    float a = 1.f / (tanTheta * alpha);
+    if (a >= 1.6f) return 0.f;
+    return (1 - 1.259f * a + 0.396f * a * a) / (3.535f * a + 2.181f * a * a);
+}
+__device__ __forceinline__ V3 beck_sample_wh(const Bsdf& b, float sx, float sy) {
+    float logSample = log_r(1.f - sx);
+    if (is_inf(logSample)) logSample = 0;
+    float tan2, phi;
+    if (b.ax == b.ay) {
+        tan2 = -sq(b.ax) * logSample;
+        phi = sy * kTwoPi;
+    } else {
+        phi = atan_r(b.ay / b.ax * tan_r(kTwoPi * sy + kHalfPi));
+        if (sy > 0.5) phi += kPi;
+        float sinPhi = sin_r(phi), cosPhi = cos_r(phi);
+        tan2 = -logSample / (cosPhi * cosPhi / sq(b.ax) + sinPhi * sinPhi / sq(b.ay));
+    }
+    float cosTheta = 1.f / sqrtf(1.f + tan2);
+    return unit_from_cos(cosTheta, phi);
+}
+
+// MicrofacetDistribution dispatch (GGX or Beckmann): D, G = 1 / (1 + lambda(wo) +
+// lambda(wi)), Sample_wh and Pdf = D * |wh.y| have the same form for both.
+__device__ __forceinline__ float mf_D(const Bsdf& b, V3 wh) {
+    return b.ndf == C2_NDF_BECKMANN ? beck_D(b, wh) : ggx_D(b, wh);
+}
+__device__ __forceinline__ float mf_G(const Bsdf& b, V3 wo, V3 wi) {
+    if (b.ndf == C2_NDF_BECKMANN) return 1.f / (1.f + beck_lambda(b, wo) + beck_lambda(b, wi));
+    return ggx_G(b, wo, wi);
+}
+__device__ __forceinline__ V3 mf_sample_wh(const Bsdf& b, float sx, float sy) {
+    return b.ndf == C2_NDF_BECKMANN ? beck_sample_wh(b, sx, sy) : ggx_sample_wh(b, sx, sy);
+}
+__device__ __forceinline__ float mf_pdf(const Bsdf& b, V3 wh) { return mf_D(b, wh) * fabsf(wh.y); }
+
 // refract(wo, wh, eta, wt), Utils.h:102-112
 __device__ __forceinline__ bool refract_h(V3 wo, V3 wh, float eta, V3& wt) {
     float cosO = clampf_ref(dot(wo, wh), -1.f, 1.f);
@@ -310,14 +370,14 @@ __device__ __forceinline__ bool refract_h(V3 wo, V3 wh, float eta, V3& wt) {
 // MicrofacetReflection, Microfacet.cpp:17-55
 __device__ __forceinline__ float mfr_pdf(const Bsdf& b, V3 wo, V3 wi) {
     V3 H = normalize(wo + wi);
-    return ggx_pdf(b, H) / (4 * dot(wo, H));
+    return mf_pdf(b, H) / (4 * dot(wo, H));
 }
 __device__ __forceinline__ V3 mfr_f(const Bsdf& b, V3 wo, V3 wi) {
     if (wo.y * wi.y < 0) return V3{0, 0, 0};
     V3 H = normalize(wo + wi);
     float F = fresnel(b, b.etaA, b.etaB, fmax_ref(0.f, dot(wo, H)));
-    float G = ggx_G(b, wo, wi);
-    float D = ggx_D(b, H);
+    float G = mf_G(b, wo, wi);
+    float D = mf_D(b, H);
     return b.R * D * G * F / (4 * fmax_ref(0.f, wi.y) * fmax_ref(0.f, wo.y));
 }
 // MicrofacetTransmission, Microfacet.cpp:57-125
@@ -327,7 +387,7 @@ __device__ __forceinline__ float mft_pdf(const Bsdf& b, V3 wo, V3 wi) {
     V3 H = normalize(wo + wi * eta);
     float absDot = fabsf(dot(H, wi));
     float dwh_dwi = eta * eta * absDot / sq(dot(wo, H) + eta * dot(wi, H));
-    return ggx_pdf(b, H) * dwh_dwi;
+    return mf_pdf(b, H) * dwh_dwi;
 }
 __device__ __forceinline__ V3 mft_f(const Bsdf& b, V3 wo, V3 wi) {
     if (wi.y * wo.y > 0) return V3{0, 0, 0};
@@ -337,8 +397,8 @@ __device__ __forceinline__ V3 mft_f(const Bsdf& b, V3 wo, V3 wi) {
     if (H.y < 0) H = -H;
     float bot = sq(dot(wo, H) + eta * dot(wi, H));
     float F = 1.f - fresnel(b, b.etaA, b.etaB, fmax_ref(0.f, dot(wo, H)));
-    float G = ggx_G(b, wo, wi);
-    float D = ggx_D(b, H);
+    float G = mf_G(b, wo, wi);
+    float D = mf_D(b, H);
     return b.R * D * G * F * fabsf(dot(H, wo)) * fabsf(dot(H, wi)) /
            (bot * fmax_ref(0.f, -wi.y) * fmax_ref(0.f, wo.y));
 }
@@ -354,6 +414,45 @@ __device__ __forceinline__ V3 phong_f(const Bsdf& b, V3 wo, V3 wi) {
            (4 * fmax_ref(0.f, wi.y) * fmax_ref(0.f, wo.y));
 }
 
+// OrenNayarReflection, Reflections.cpp:28-61 (A and B from sigma, :18-26)
+__device__ __forceinline__ V3 orennayar_f(const Bsdf& b, V3 wo, V3 wi) {
+    float sinThetaO = sqrtf(fmax_ref(0.f, 1.f - wo.y * wo.y));
+    float sinThetaI = sqrtf(fmax_ref(0.f, 1.f - wi.y * wi.y));
+    float sinAlpha, tanBeta;
+    if (wo.y < wi.y) {
+        sinAlpha = sinThetaI;
+        tanBeta = sinThetaO / wo.y;
+    } else {
+        sinAlpha = sinThetaO;
+        tanBeta = sinThetaI / wi.y;
+    }
+    float dCos = 0.f;
+    if (sinThetaI > 1e-5f && sinThetaO > 1e-5f) {  // EPS, src/Core/Utils.h:7
+        float ci, si, co, so;
+        cos_sin_phi(wi, sinThetaI, ci, si);
+        cos_sin_phi(wo, sinThetaO, co, so);
+        dCos = fmax_ref(0.f, ci * co + si * so);
+    }
+    return b.R / kPi * (b.onA + b.onB * (dCos * sinAlpha * tanBeta));
+}
+// FresnelBlend, FresnelBlend.cpp:15-68 (its own Schlick with R0 = Ks)
+__device__ __forceinline__ float fblend_pdf(const Bsdf& b, V3 wo, V3 wi) {
+    V3 H = normalize(wo + wi);
+    return 0.5f * (fmax_ref(0.f, wi.y) / kPi + mf_pdf(b, H) / (4 * dot(wo, H)));
+}
+__device__ __forceinline__ V3 fblend_f(const Bsdf& b, V3 wo, V3 wi) {
+    float cosThetaI = fmax_ref(0.f, wi.y);
+    float cosThetaO = fmax_ref(0.f, wo.y);
+    // (float)std::pow(x, 5) evaluates in double: pow5_r
+    V3 diffuse = (28.f / (23.f * kPi)) * b.R * (V3{1.f, 1.f, 1.f} - b.S) * (1.f - pow5_r(1.f - 0.5f * cosThetaO)) *
+                 (1.f - pow5_r(1.f - 0.5f * cosThetaI));
+    V3 H = normalize(wo + wi);
+    float c = dot(wi, H);
+    V3 schlick = b.S + (V3{1.f, 1.f, 1.f} - b.S) * pow5_r(1.f - c);  // std::pow(1 - cos, 5.f)
+    V3 specular = mf_D(b, H) / (4 * dot(wi, H) * fmax_ref(cosThetaI, cosThetaO)) * schlick;
+    return diffuse + specular;
+}
+
 // BxDF::DistributionFunction / Pdf in local coordinates
 __device__ __forceinline__ V3 bxdf_f(const Bsdf& b, V3 wo, V3 wi) {
     switch (b.kind) {
@@ -361,6 +460,8 @@ __device__ __forceinline__ V3 bxdf_f(const Bsdf& b, V3 wo, V3 wi) {
         case BK_MFREFL: return mfr_f(b, wo, wi);
         case BK_MFBLEND: return (wo.y * wi.y < 0) ? mft_f(b, wo, wi) : mfr_f(b, wo, wi);
         case BK_PHONG: return phong_f(b, wo, wi);
+        case BK_ORENNAYAR: return orennayar_f(b, wo, wi);
+        case BK_FBLEND: return fblend_f(b, wo, wi);
         default: return V3{0, 0, 0};
     }
 }
@@ -370,6 +471,8 @@ __device__ __forceinline__ float bxdf_pdf(const Bsdf& b, V3 wo, V3 wi) {
         case BK_MFREFL: return mfr_pdf(b, wo, wi);
         case BK_MFBLEND: return (wo.y * wi.y < 0) ? mft_pdf(b, wo, wi) : mfr_pdf(b, wo, wi);
         case BK_PHONG: return phong_pdf(b, wo, wi);
+        case BK_ORENNAYAR: return fmax_ref(0.f, wi.y) / kPi;
+        case BK_FBLEND: return fblend_pdf(b, wo, wi);
         default: return 0.f;
     }
 }
@@ -404,7 +507,7 @@ __device__ __forceinline__ V3 bxdf_sample(const Bsdf& b, float sx, float sy, V3 
             return (1.f - fr) * b.R;
         }
         case BK_MFREFL: {  // Microfacet.cpp:41-53
-            V3 wh = ggx_sample_wh(b, sx, sy);
+            V3 wh = mf_sample_wh(b, sx, sy);
             if (dot(wo, wh) <= 0) return V3{0, 0, 0};
             wi = reflect(-wo, wh);
             pdf = mfr_pdf(b, wo, wi);
@@ -412,7 +515,7 @@ __device__ __forceinline__ V3 bxdf_sample(const Bsdf& b, float sx, float sy, V3 
             return mfr_f(b, wo, wi);
         }
         case BK_MFBLEND: {  // Microfacet.cpp:162-198
-            V3 wh = ggx_sample_wh(b, sx, sy);
+            V3 wh = mf_sample_wh(b, sx, sy);
             if (dot(wo, wh) <= 0) return V3{0, 0, 0};
             float fr = fresnel(b, b.etaA, b.etaB, dot(wo, wh));
             if (sx < fr) {
@@ -434,6 +537,35 @@ __device__ __forceinline__ V3 bxdf_sample(const Bsdf& b, float sx, float sy, V3 
             pdf = bxdf_pdf(b, wo, wi) * (1 - fr);
             wi = clamp_unit(wi);
             return mfr_f(b, wo, wi);
+        }
+        case BK_ORENNAYAR: {  // Reflections.cpp:58-61: cosine-weighted hemisphere
+            float r = sqrtf(sx);
+            float phi = sy * kTwoPi;
+            float x = r * cos_r(phi), z = r * sin_r(phi);
+            float y = sqrtf(1.0f - r * r);
+            pdf = y / kPi;
+            wi = V3{x, y, z};
+            return orennayar_f(b, wo, wi);
+        }
+        case BK_FBLEND: {  // FresnelBlend.cpp:45-68
+            float u = sx;
+            if (u < 0.5) {
+                float s0 = u * 2.f;
+                float r = sqrtf(s0);
+                float phi = sy * kTwoPi;
+                float x = r * cos_r(phi), z = r * sin_r(phi);
+                float y = sqrtf(1.0f - r * r);
+                pdf = y / kPi;  // NextCosineUnitHemiSphere writes it; replaced below
+                wi = V3{x, y, z};
+            } else {
+                float s0 = (u - 0.5f) * 2.f;
+                V3 wh = mf_sample_wh(b, s0, sy);
+                V3 L = reflect(-wo, wh);
+                if (L.y * wo.y <= 0) return V3{0, 0, 0};  // before writing wi and pdf
+                wi = clamp_unit(L);
+            }
+            pdf = fblend_pdf(b, wo, wi);
+            return fblend_f(b, wo, wi);
         }
         case BK_PHONG: {  // BlinnPhongReflection.cpp:32-58, deterministic redraw stream
             for (uint32_t redraw = 0; redraw < 1024; redraw++) {
@@ -469,8 +601,11 @@ __device__ __forceinline__ Bsdf make_bsdf(const DevScene& S, const Inter& it, ui
     b.type = 0;
     b.fresnel = FR_NOOP;
     b.k = 0;
+    b.ndf = C2_NDF_GGX;
+    b.S = V3{0.f, 0.f, 0.f};
     b.etaA = b.etaB = 1.f;
     b.ax = b.ay = 0.f;
+    b.onA = b.onB = 0.f;
     b.ridx = idx;
     b.rdim = rdim;
     if (it.material < 0) return b;
@@ -512,6 +647,23 @@ __device__ __forceinline__ Bsdf make_bsdf(const DevScene& S, const Inter& it, ui
             b.fresnel = FR_SCHLICK;
             b.ax = m.roughness[0];
             b.ay = m.roughness[1];
+            b.ndf = m.ndf;
+            break;
+        }
+        case C2_MAT_PLASTIC:  // Plastic.cpp:13-27: one FresnelBlend(Kd, Ks, NDF)
+            b.kind = BK_FBLEND;
+            b.type = T_REFL | T_GLOSSY;
+            b.S = V3{m.ks[0], m.ks[1], m.ks[2]};
+            b.ax = m.roughness[0];
+            b.ay = m.roughness[1];
+            b.ndf = m.ndf;
+            break;
+        case C2_MAT_ORENNAYAR: {  // DiffuseMaterial.cpp:29-34, Reflections.cpp:18-26
+            b.kind = BK_ORENNAYAR;
+            b.type = T_DIFF | T_REFL;
+            float sigmaSq = m.sigma * m.sigma;
+            b.onA = 1.f - sigmaSq / (2 * (sigmaSq + 0.33f));
+            b.onB = (0.45f * sigmaSq) / (sigmaSq + 0.09f);
             break;
         }
         case C2_MAT_PHONG:

@@ -85,6 +85,8 @@ C2_HD float tan_r(float x) { return (float)tan((double)x); }
 C2_HD float atan_r(float x) { return (float)atan((double)x); }
 C2_HD float acos_r(float x) { return (float)acos((double)x); }
 C2_HD float atan2_r(float y, float x) { return (float)atan2((double)y, (double)x); }
+C2_HD float exp_r(float x) { return (float)exp((double)x); }
+C2_HD float log_r(float x) { return (float)log((double)x); }
 C2_HD float powf_r(float x, float y) { return (float)pow((double)x, (double)y); }
 // x^5 for a float x: x^2 is exact in double, the two further products round once
 // each (relative error < 2^-51), then one rounding to float. Deterministic IEEE

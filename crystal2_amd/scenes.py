@@ -147,13 +147,16 @@ def cornell(directory: str, width: int = 256, height: int = 256, spp: int = 16, 
 
 
 def material_zoo(directory: str, width: int = 64, height: int = 64, spp: int = 8, depth: int = 6,
-                 threads: int = 8, accel: str = "BVH", phong: bool = True) -> SceneFiles:
+                 threads: int = 8, accel: str = "BVH", phong: bool = True, extra: bool = False) -> SceneFiles:
     """Every supported material and light type in one small scene (parity fixture).
 
     Cornell-style box; ceiling area-light quad; an emissive analytic sphere (sphere
     area light); a point light; analytic spheres of specular glass, rough glass
     (GGX 0.2), isotropic GGX, anisotropic GGX, Blinn-Phong and mirror; a glass
-    icosphere mesh; and a material-less (transparent) triangle pair.
+    icosphere mesh; and a material-less (transparent) triangle pair. `extra` swaps the
+    six spheres for the remaining materials: Oren-Nayar (two sigmas), Plastic
+    (FresnelBlend) with GGX and anisotropic Beckmann, and Micro with isotropic and
+    anisotropic Beckmann.
     """
     os.makedirs(directory, exist_ok=True)
     white = {"Type": "Diffuse", "Color": [0.73, 0.73, 0.73]}
@@ -188,6 +191,19 @@ def material_zoo(directory: str, width: int = 64, height: int = 64, spp: int = 8
          else {"Type": "Diffuse", "Color": [0.8, 0.8, 0.5]}),
         ([0.0, 0.3, -0.5], 0.2, {"Type": "Mirror", "Color": [0.95, 0.95, 0.95]}),
     ]
+    if extra:
+        spheres = [
+            ([-0.55, -0.72, -0.35], 0.27, {"Type": "OrenNayar", "Color": [0.8, 0.6, 0.4], "Sigma": 0.5}),
+            ([0.0, -0.72, -0.45], 0.27, {"Type": "Plastic", "Kd": [0.5, 0.1, 0.1], "Ks": [0.4, 0.4, 0.4],
+                                         "Roughness": [0.2, 0.2], "NDF": "GGX"}),
+            ([0.55, -0.72, -0.35], 0.27, {"Type": "Micro", "Color": [0.9, 0.6, 0.3], "Roughness": [0.3, 0.3],
+                                          "IOR": 1.5, "NDF": "Beckmann"}),
+            ([-0.45, -0.2, 0.25], 0.22, {"Type": "Micro", "Color": [0.6, 0.8, 0.9], "Roughness": [0.1, 0.4],
+                                         "IOR": 1.5, "NDF": "Beckmann"}),
+            ([0.45, -0.2, 0.25], 0.22, {"Type": "Plastic", "Kd": [0.1, 0.3, 0.6], "Ks": [0.3, 0.3, 0.3],
+                                        "Roughness": [0.15, 0.35], "NDF": "Beckmann"}),
+            ([0.0, 0.3, -0.5], 0.2, {"Type": "OrenNayar", "Color": [0.9, 0.9, 0.9], "Sigma": 1.2}),
+        ]
     for pos, r, mat in spheres:
         objects.append({"Type": "Geometry", "Shape": {"Type": "Sphere", "Position": pos, "Radius": r},
                         "Material": mat})

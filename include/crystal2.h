@@ -59,19 +59,23 @@ enum {
     C2_MAT_MIRROR = 1,   /* SpecularReflection + NoOp       src/Materials/MirrorMaterial.cpp:11-13  */
     C2_MAT_GLASS = 2,    /* SpecularFresnel | MicrofacetBlend src/Materials/Glass.cpp:16-34         */
     C2_MAT_MICRO = 3,    /* MicrofacetReflection + Schlick  src/Materials/MicrofacetMaterial.cpp:18-42 */
-    C2_MAT_PHONG = 4     /* BlinnPhongReflection            src/Materials/PhongMaterial.cpp:13-22    */
+    C2_MAT_PHONG = 4,    /* BlinnPhongReflection            src/Materials/PhongMaterial.cpp:13-22    */
+    C2_MAT_PLASTIC = 5,  /* FresnelBlend                    src/Materials/Plastic.cpp:13-27          */
+    C2_MAT_ORENNAYAR = 6 /* OrenNayarReflection             src/Materials/DiffuseMaterial.cpp:29-34  */
 };
 enum { C2_NDF_GGX = 0, C2_NDF_BECKMANN = 1 };
 enum { C2_LIGHT_AREA = 0, C2_LIGHT_POINT = 1 };
 
 typedef struct c2_material {
     int32_t type;          /* C2_MAT_*                                      */
-    int32_t ndf;           /* C2_NDF_* (Micro only; Beckmann unsupported)   */
+    int32_t ndf;           /* C2_NDF_* (Micro, Plastic)                     */
     int32_t phong_k;       /* Blinn-Phong exponent K                        */
     int32_t pad_;
-    float color[3];        /* albedo / R (constant texture)                 */
-    float roughness[2];    /* GGX alpha_x, alpha_y (Glass: 0,0 = specular)  */
+    float color[3];        /* albedo / R (constant texture); Plastic: Kd    */
+    float roughness[2];    /* NDF alpha_x, alpha_y (Glass: 0,0 = specular)  */
     float ior;             /* IOR (Glass, Micro)                            */
+    float ks[3];           /* Plastic: Ks                                   */
+    float sigma;           /* OrenNayar: Sigma                              */
 } c2_material;
 
 typedef struct c2_light {

@@ -41,18 +41,25 @@ inline float o_sin(float x) { return std::sin(x); }
 inline float o_cos(float x) { return std::cos(x); }
 inline float o_tan(float x) { return std::tan(x); }
 inline float o_atan(float x) { return std::atan(x); }
+inline float o_exp(float x) { return std::exp(x); }
+inline float o_log(float x) { return std::log(x); }
 inline float o_pow5(float x) { return std::pow(x, 5.f); }
+// std::pow(float, int) promotes to double
+inline float o_pow5d(float x) { return (float)std::pow((double)x, 5); }
 #else
 inline float o_sin(float x) { return (float)std::sin((double)x); }
 inline float o_cos(float x) { return (float)std::cos((double)x); }
 inline float o_tan(float x) { return (float)std::tan((double)x); }
 inline float o_atan(float x) { return (float)std::atan((double)x); }
+inline float o_exp(float x) { return (float)std::exp((double)x); }
+inline float o_log(float x) { return (float)std::log((double)x); }
 // x^5 as three double products (x^2 exact), rounded once to float: the GPU's form.
 inline float o_pow5(float x) {
     double d = x;
     double d2 = d * d;
     return (float)(d2 * d2 * d);
 }
+inline float o_pow5d(float x) { return o_pow5(x); }
 #endif
 // std::pow(float, int) and std::pow(float, double) promote to double in both variants.
 inline double o_powd(double x, double y) { return std::pow(x, y); }
@@ -383,9 +390,23 @@ inline vec3 unitFromCos(float cosTheta, float phi) {  // Utils.h:79-83
     float r = std::sqrt(1 - cosTheta * cosTheta);
     return vec3(r * o_cos(phi), cosTheta, -r * o_sin(phi));
 }
-struct GGX {  // MicrofacetDistribution.cpp:15-82
+// GGXDistribution (MicrofacetDistribution.cpp:15-82) or, with beckmann set,
+// BeckmannDistribution (:84-161; alpha = roughness as given).
+struct GGX {
     vec2 a;
+    bool beckmann = false;
     float D(vec3 wh) const {
+        if (beckmann) {
+            float cos2 = wh.y * wh.y;
+            float sin2 = 1.f - cos2;
+            float tan2 = sin2 / cos2;
+            if (std::isinf(tan2)) return 0.f;
+            float cos4 = cos2 * cos2;
+            vec2 v = cosSinPhi(wh, std::sqrt(sin2));
+            float bot = PI * a.x * a.y * cos4;
+            float top = o_exp(-tan2 * (square(v.x) / square(a.x) + square(v.y) / square(a.y)));
+            return top / bot;
+        }
         float cos2 = wh.y * wh.y;
         float sin2 = smax(0.f, 1.f - cos2);
         vec2 phi = cosSinPhi(wh, std::sqrt(sin2));
@@ -393,6 +414,17 @@ struct GGX {  // MicrofacetDistribution.cpp:15-82
         return 1.f / (PI * a.x * a.y * square(A - cos2 * (A - 1)));
     }
     float lambda(vec3 w) const {
+        if (beckmann) {
+            float cosTheta = w.y;
+            float sinTheta = std::sqrt(1.f - cosTheta * cosTheta);
+            float tanTheta = std::abs(sinTheta / cosTheta);
+            if (std::isinf(tanTheta)) return 0.f;
+            vec2 v = cosSinPhi(w, sinTheta);
+            float alpha = std::sqrt(square(v.x) * square(a.x) + square(v.y) * square(a.y));
+            float aa = 1.f / (tanTheta * alpha);
+            if (aa >= 1.6f) return 0.f;
+            return (1 - 1.259f * aa + 0.396f * aa * aa) / (3.535f * aa + 2.181f * aa * aa);
+        }
         float cos2 = w.y * w.y;
         float sin2 = 1.f - cos2;
         float tan2 = sin2 / cos2;
@@ -403,6 +435,22 @@ struct GGX {  // MicrofacetDistribution.cpp:15-82
     }
     float G(vec3 wo, vec3 wi) const { return 1.f / (1.f + lambda(wo) + lambda(wi)); }
     vec3 sampleWh(vec3, vec2 s) const {
+        if (beckmann) {
+            float logSample = o_log(1.f - s.x);
+            if (std::isinf(logSample)) logSample = 0;
+            float tan2, phi;
+            if (a.x == a.y) {
+                tan2 = -square(a.x) * logSample;
+                phi = s.y * TWO_PI;
+            } else {
+                phi = o_atan(a.y / a.x * o_tan(TWO_PI * s.y + HALF_PI));
+                if (s.y > 0.5) phi += PI;
+                float sinPhi = o_sin(phi), cosPhi = o_cos(phi);
+                tan2 = -logSample / (cosPhi * cosPhi / square(a.x) + sinPhi * sinPhi / square(a.y));
+            }
+            float cosTheta = 1.f / std::sqrt(1.f + tan2);
+            return unitFromCos(cosTheta, phi);
+        }
         float cosTheta, phi;
         if (a.x == a.y) {
             cosTheta = std::sqrt((1 - s.x) / (s.x * (a.x * a.x - 1) + 1));
@@ -431,7 +479,8 @@ inline bool refractH(vec3 wo, vec3 wh, float eta, vec3* wt) {
     return true;
 }
 
-enum BxKind { BX_LAMBERT, BX_SPECREFL, BX_SPECFRESNEL, BX_MFREFL, BX_MFTRANS, BX_MFBLEND, BX_PHONG };
+enum BxKind { BX_LAMBERT, BX_SPECREFL, BX_SPECFRESNEL, BX_MFREFL, BX_MFTRANS, BX_MFBLEND, BX_PHONG, BX_ORENNAYAR,
+              BX_FBLEND };
 
 struct BxDF {
     BxKind kind;
@@ -442,6 +491,43 @@ struct BxDF {
     GGX dist{vec2(0, 0)};
     int k = 0;
     uint32_t redrawIdx = 0, redrawDim = 0;  // Blinn-Phong redraw stream
+    vec3 S{0.f};               // FresnelBlend: Ks
+    float onA = 0, onB = 0;    // OrenNayarReflection: A, B
+
+    // OrenNayarReflection, Reflections.cpp:7-56
+    static float sinThetaOf(vec3 w) { return std::sqrt(smax(0.f, 1.f - w.y * w.y)); }
+    vec3 orenNayarF(vec3 wo, vec3 wi) const {
+        float sinThetaO = sinThetaOf(wo), sinThetaI = sinThetaOf(wi);
+        float sinAlpha, tanBeta;
+        if (wo.y < wi.y) {
+            sinAlpha = sinThetaI;
+            tanBeta = sinThetaO / wo.y;
+        } else {
+            sinAlpha = sinThetaO;
+            tanBeta = sinThetaI / wi.y;
+        }
+        float dCos = 0.f;
+        if (sinThetaI > EPS_UTIL && sinThetaO > EPS_UTIL) {
+            vec2 dI = cosSinPhi(wi, sinThetaI), dO = cosSinPhi(wo, sinThetaO);
+            dCos = smax(0.f, dI.x * dO.x + dI.y * dO.y);
+        }
+        return R / PI * (onA + onB * (dCos * sinAlpha * tanBeta));
+    }
+    // FresnelBlend, FresnelBlend.cpp:15-43
+    float fblendPdf(vec3 wo, vec3 wi) const {
+        vec3 H = normalize(wo + wi);
+        return 0.5f * (smax(0.f, wi.y) / PI + dist.pdf(wo, H) / (4 * dot(wo, H)));
+    }
+    vec3 fblendF(vec3 wo, vec3 wi) const {
+        float cosThetaI = smax(0.f, wi.y);
+        float cosThetaO = smax(0.f, wo.y);
+        vec3 diffuse = (28.f / (23.f * PI)) * R * (vec3(1.f) - S) * (1.f - o_pow5d(1.f - 0.5f * cosThetaO)) *
+                       (1.f - o_pow5d(1.f - 0.5f * cosThetaI));
+        vec3 H = normalize(wo + wi);
+        vec3 schlick = S + (vec3(1.f) - S) * o_pow5(1.f - dot(wi, H));
+        vec3 specular = dist.D(H) / (4 * dot(wi, H) * smax(cosThetaI, cosThetaO)) * schlick;
+        return diffuse + specular;
+    }
 
     bool matches(int t) const { return (type & t) == type; }
     bool contains(int t) const { return (type & t) != 0; }
@@ -501,6 +587,8 @@ struct BxDF {
             case BX_MFTRANS: return mfTransF(wo, wi);
             case BX_MFBLEND: return (wo.y * wi.y < 0) ? mfTransF(wo, wi) : mfReflF(wo, wi);  // :197-207
             case BX_PHONG: return phongF(wo, wi);
+            case BX_ORENNAYAR: return orenNayarF(wo, wi);
+            case BX_FBLEND: return fblendF(wo, wi);
         }
         return vec3(0.f);
     }
@@ -513,6 +601,8 @@ struct BxDF {
             case BX_MFTRANS: return mfTransPdf(wo, wi);
             case BX_MFBLEND: return (wo.y * wi.y < 0) ? mfTransPdf(wo, wi) : mfReflPdf(wo, wi);  // :185-195
             case BX_PHONG: return phongPdf(wo, wi);
+            case BX_ORENNAYAR: return smax(0.f, wi.y) / PI;  // Reflections.cpp:28-31
+            case BX_FBLEND: return fblendPdf(wo, wi);
         }
         return 0.f;
     }
@@ -584,6 +674,35 @@ struct BxDF {
                 *pdfOut = pdf(wo, *wi) * (1 - fr);
                 *wi = fixVector(*wi);
                 return mfReflF(wo, *wi);
+            }
+            case BX_ORENNAYAR: {  // Reflections.cpp:58-61 + NextCosineUnitHemiSphere
+                float r = std::sqrt(s.x);
+                float phi = s.y * TWO_PI;
+                float x = r * o_cos(phi), z = r * o_sin(phi);
+                float y = std::sqrt(1.0f - r * r);
+                *pdfOut = y / PI;
+                *wi = vec3(x, y, z);
+                return orenNayarF(wo, *wi);
+            }
+            case BX_FBLEND: {  // FresnelBlend.cpp:45-68
+                float u = s.x;
+                if (u < 0.5) {
+                    s.x = u * 2.f;
+                    float r = std::sqrt(s.x);
+                    float phi = s.y * TWO_PI;
+                    float x = r * o_cos(phi), z = r * o_sin(phi);
+                    float y = std::sqrt(1.0f - r * r);
+                    *pdfOut = y / PI;
+                    *wi = vec3(x, y, z);
+                } else {
+                    s.x = (u - 0.5f) * 2.f;
+                    vec3 wh = dist.sampleWh(wo, s);
+                    vec3 L = reflect(-wo, wh);
+                    if (L.y * wo.y <= 0) return vec3(0.f);
+                    *wi = fixVector(L);
+                }
+                *pdfOut = fblendPdf(wo, *wi);
+                return fblendF(wo, *wi);
             }
             case BX_PHONG: {  // BlinnPhongReflection.cpp:32-58 (deterministic redraws)
                 for (uint32_t redraw = 0; redraw < 1024; redraw++) {
@@ -794,6 +913,8 @@ struct MatDesc {
     vec3 color;
     vec2 rough;
     float ior;
+    vec3 ks;
+    float sigma;
 };
 struct Light {
     int type;  // 0 area, 1 point
@@ -1025,7 +1146,23 @@ struct orc_scene {
                 b.etaA = etaA;
                 b.etaB = etaB;
                 b.F = Fresnel{F_SCHLICK, vec3(0.99f)};
-                b.dist = GGX{md.rough};
+                b.dist = GGX{md.rough, md.ndf == C2_NDF_BECKMANN};
+                break;
+            }
+            case C2_MAT_PLASTIC:  // Plastic.cpp:13-27
+                b.kind = BX_FBLEND;
+                b.type = REFL | GLOSSY;
+                b.R = md.color;
+                b.S = md.ks;
+                b.dist = GGX{md.rough, md.ndf == C2_NDF_BECKMANN};
+                break;
+            case C2_MAT_ORENNAYAR: {  // DiffuseMaterial.cpp:29-34, Reflections.cpp:18-26
+                b.kind = BX_ORENNAYAR;
+                b.type = DIFF | REFL;
+                b.R = md.color;
+                float sigmaSq = md.sigma * md.sigma;
+                b.onA = 1.f - sigmaSq / (2 * (sigmaSq + 0.33f));
+                b.onB = (0.45f * sigmaSq) / (sigmaSq + 0.09f);
                 break;
             }
             case C2_MAT_PHONG:  // PhongMaterial.cpp:13-22
@@ -1359,7 +1496,8 @@ int orc_scene_create(const c2_scene_desc* d, int accel, int tie, orc_scene** out
         for (int m = 0; m < d->num_materials; m++) {
             const c2_material& cm = d->materials[m];
             s->mats.push_back(MatDesc{cm.type, cm.ndf, cm.phong_k, vec3(cm.color[0], cm.color[1], cm.color[2]),
-                                      vec2(cm.roughness[0], cm.roughness[1]), cm.ior});
+                                      vec2(cm.roughness[0], cm.roughness[1]), cm.ior,
+                                      vec3(cm.ks[0], cm.ks[1], cm.ks[2]), cm.sigma});
         }
         for (int l = 0; l < d->num_lights; l++) {
             const c2_light& cl = d->lights[l];

@@ -9,6 +9,7 @@ arg_order="ltr" film.
 
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py direct   # DirectLighting films only
+    python tests/golden/make_golden.py extra    # Oren-Nayar / Plastic / Beckmann scene only
 """
 import os
 import subprocess
@@ -120,8 +121,23 @@ def main():
     print("golden fixtures written to", HERE)
 
 
+def extra_materials():
+    """The zoo with Oren-Nayar, Plastic and Beckmann spheres: the reference's films for
+    both integrators; leaves every other fixture untouched."""
+    if not os.path.exists(REF):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    d = os.path.join(SCN, "zoo_extra")
+    scenes.material_zoo(d, width=64, height=64, spp=8, depth=6, extra=True)
+    scenes.write_config(os.path.join(d, "config_dl.json"), 64, 64, 8, 6, integrator="DirectLight")
+    film("zoo_extra", 64, 64)
+    film("zoo_extra", 64, 64, config="config_dl.json", tag="_dl")
+    print("extra-material films written to", HERE)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["direct"]:
         direct_lighting()
+    elif sys.argv[1:] == ["extra"]:
+        extra_materials()
     else:
         main()

@@ -29,7 +29,8 @@ def gpu_film(sc, w, h, spp, depth, order="rtl", **kw):
     return f
 
 
-@pytest.mark.parametrize("name,spp,depth", [("cornell", 4, 5), ("zoo_nophong", 8, 6), ("zoo", 8, 6)])
+@pytest.mark.parametrize("name,spp,depth", [("cornell", 4, 5), ("zoo_nophong", 8, 6), ("zoo", 8, 6),
+                                            ("zoo_extra", 8, 6)])
 @pytest.mark.parametrize("order", ["ltr", "rtl"])
 def test_film_bit_identical_to_oracle(gpu_available, name, spp, depth, order):
     print(f"case film {name} {order}", flush=True)
@@ -41,7 +42,7 @@ def test_film_bit_identical_to_oracle(gpu_available, name, spp, depth, order):
 
 
 @pytest.mark.parametrize("name,spp,depth,tol", [("cornell", 4, 5, 1e-4), ("zoo_nophong", 8, 6, 1e-4),
-                                                ("zoo", 8, 6, 2e-3)])
+                                                ("zoo", 8, 6, 2e-3), ("zoo_extra", 8, 6, 1e-4)])
 def test_film_vs_reference_golden(gpu_available, name, spp, depth, tol):
     """The reference binary's films (clang build: left-to-right argument order)."""
     print(f"case golden film {name}", flush=True)
@@ -153,7 +154,7 @@ def test_pixel_window_matches_full_frame(gpu_available):
             assert relmse(f[rows, 8:32], full[rows, 8:32]) < 1e-10
 
 
-@pytest.mark.parametrize("name,spp", [("cornell", 4), ("zoo_nophong", 8), ("zoo", 8)])
+@pytest.mark.parametrize("name,spp", [("cornell", 4), ("zoo_nophong", 8), ("zoo", 8), ("zoo_extra", 8)])
 def test_direct_lighting_bit_identical_to_oracle(gpu_available, name, spp):
     """DirectLightingIntegrator on the wavefront kernels against the oracle's restatement."""
     print(f"case direct lighting {name}", flush=True)
@@ -165,7 +166,8 @@ def test_direct_lighting_bit_identical_to_oracle(gpu_available, name, spp):
     assert relmse(g, o) <= 1e-6
 
 
-@pytest.mark.parametrize("name,spp,tol", [("cornell", 4, 1e-4), ("zoo_nophong", 8, 1e-4), ("zoo", 8, 1e-4)])
+@pytest.mark.parametrize("name,spp,tol", [("cornell", 4, 1e-4), ("zoo_nophong", 8, 1e-4), ("zoo", 8, 1e-4),
+                                          ("zoo_extra", 8, 1e-4)])
 def test_direct_lighting_vs_reference_golden(gpu_available, name, spp, tol):
     print(f"case direct lighting golden {name}", flush=True)
     g = gpu_film(load(name), 64, 64, spp, 6, "ltr", integrator="direct")

@@ -61,7 +61,10 @@ def test_json_bom_and_numbers(tmp_path):
     (lambda s: s["Scene"]["Objects"][0]["Shape"].update({"Type": "Cone"}), -2),
     (lambda s: s["Scene"]["Objects"][0]["Material"].update({"Type": "Velvet"}), -2),
     (lambda s: s["Scene"]["Objects"][0]["Shape"].update({"Type": "Disk"}), -6),
-    (lambda s: s["Scene"]["Objects"][0]["Material"].update({"Type": "Plastic"}), -6),
+    (lambda s: s["Scene"]["Objects"][0]["Material"].update({"Type": "Plastic"}), -2),  # no Kd / Ks
+    (lambda s: s["Scene"]["Objects"][0]["Material"].update({"Type": "OrenNayar", "Kd": "wood"}), -6),
+    (lambda s: s["Scene"]["Objects"][0]["Material"].update({"Type": "Micro", "Roughness": [0.1, 0.1],
+                                                            "IOR": 1.5, "NDF": "Blinn"}), -2),
     (lambda s: s.update({"Camera": {"Type": "Fisheye"}}), -2),
 ])
 def test_scene_errors(tmp_path, mutate, code):
