@@ -123,6 +123,14 @@ class Scene:
         check(self._lib.c2_scene_add_sphere(self._h, _fp(c), float(radius), _fp(r), material,
                                             None if le is None else _fp(le)))
 
+    def add_disk(self, center, radius: float, material: int, lemit=None, rotation=(0.0, 0.0, 0.0)):
+        """Disk (Disk.cpp): facing local +y before the x, y, z rotations (degrees)."""
+        c = np.asarray(center, np.float32)
+        r = np.asarray(rotation, np.float32)
+        le = None if lemit is None else np.asarray(lemit, np.float32)
+        check(self._lib.c2_scene_add_disk(self._h, _fp(c), float(radius), _fp(r), material,
+                                          None if le is None else _fp(le)))
+
     def add_point_light(self, position, flux):
         p = np.asarray(position, np.float32)
         f = np.asarray(flux, np.float32)

@@ -12,7 +12,7 @@ C2_OK = 0
 ERRORS = {-1: "C2_ERR_INVALID", -2: "C2_ERR_PARSE", -3: "C2_ERR_IO", -4: "C2_ERR_DEVICE",
           -5: "C2_ERR_NOMEM", -6: "C2_ERR_UNSUPPORTED"}
 
-C2_SHAPE_TRIANGLE, C2_SHAPE_SPHERE = 0, 1
+C2_SHAPE_TRIANGLE, C2_SHAPE_SPHERE, C2_SHAPE_DISK = 0, 1, 2
 C2_MAT_DIFFUSE, C2_MAT_MIRROR, C2_MAT_GLASS, C2_MAT_MICRO, C2_MAT_PHONG, C2_MAT_PLASTIC, C2_MAT_ORENNAYAR = range(7)
 C2_NDF_GGX, C2_NDF_BECKMANN = 0, 1
 C2_LIGHT_AREA, C2_LIGHT_POINT = 0, 1
@@ -125,6 +125,7 @@ SIGNATURES = [
     ("c2_scene_add_material", C.c_int, [P, C.POINTER(c2_material), _i32p]),
     ("c2_scene_add_triangles", C.c_int, [P, _f32p, C.c_int32, _f32p, C.c_int32, _f32p]),
     ("c2_scene_add_sphere", C.c_int, [P, _f32p, C.c_float, _f32p, C.c_int32, _f32p]),
+    ("c2_scene_add_disk", C.c_int, [P, _f32p, C.c_float, _f32p, C.c_int32, _f32p]),
     ("c2_scene_add_point_light", C.c_int, [P, _f32p, _f32p]),
     ("c2_scene_set_camera", C.c_int, [P, C.POINTER(c2_camera)]),
     ("c2_scene_get_desc", C.c_int, [P, C.POINTER(c2_scene_desc)]),

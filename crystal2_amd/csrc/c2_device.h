@@ -145,6 +145,33 @@ C2_HD bool sphere_quad(const float* sp, V3 o, V3 d, V3& P, V3& dl, float& t1, fl
     t2 = (-b - disc) / (2 * a);
     return true;
 }
+// Disk::Intersect / IntersectTest, Disk.cpp:45-81: the local plane y = 0 within the radius
+C2_HD bool disk_plane(const float* sp, V3 o, V3 d, V3& P, V3& dl, float& t) {
+    V3 c{sp[0], sp[1], sp[2]};
+    P = mat3_mul(sp + 13, o - c);
+    dl = mat3_mul(sp + 13, d);
+    if (dl.y == 0.f) return false;
+    t = -P.y / dl.y;
+    return true;
+}
+C2_HD bool disk_closest(const float* sp, V3 o, V3 d, float& t) {
+    V3 P, dl;
+    float tt;
+    if (!disk_plane(sp, o, d, P, dl, tt)) return false;
+    if (tt < 0) return false;
+    V3 hit = P + dl * tt;
+    if (dot(hit, hit) > sp[3] * sp[3]) return false;
+    t = tt;
+    return true;
+}
+C2_HD bool disk_test(const float* sp, V3 o, V3 d, float tMin, float tMax) {
+    V3 P, dl;
+    float t;
+    if (!disk_plane(sp, o, d, P, dl, t)) return false;
+    if (t < tMin || t > tMax) return false;
+    V3 hit = P + dl * t;
+    return !(dot(hit, hit) > sp[3] * sp[3]);
+}
 // Sphere::Intersect distance selection, Sphere.cpp:44-49
 C2_HD bool sphere_closest(const float* sp, V3 o, V3 d, float& t) {
     V3 P, dl;
@@ -340,6 +367,27 @@ __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, 
                             bestE = ent;
                             bestU = res.x;
                             bestV = res.y;
+                        }
+                    }
+                } else if (kind == C2_SHAPE_DISK) {
+                    // box: centre -+ (r, r * sinTheta, r), Disk::GetBoundingBox (b.y holds r * sinTheta)
+                    const float* sp3 = S.spheres + (size_t)__float_as_int(b.x) * 22;
+                    float rr = a.w, oy = b.y;
+                    if (!slab(r, a.x - rr, a.y - oy, a.z - rr, a.x + rr, a.y + oy, a.z + rr, t1, t2)) continue;
+                    if (ANY) {
+                        if (ent == ignore) continue;
+                        if (disk_test(sp3, o, d, t1, t2)) {
+                            hit.entity = ent;
+                            return true;
+                        }
+                    } else {
+                        float dis;
+                        if (!disk_closest(sp3, o, d, dis)) continue;
+                        if (dis < best || (dis == best && ent < bestE)) {
+                            best = dis;
+                            bestE = ent;
+                            bestU = 0.f;
+                            bestV = 0.f;
                         }
                     }
                 } else {

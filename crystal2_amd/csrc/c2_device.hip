@@ -690,7 +690,7 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassP
                         if (e.x == C2_SHAPE_TRIANGLE) {
                             tri_frame(S, e.y, wi, h.u, h.v, hn, hdu, front);
                         } else {
-                            sphere_frame(S.spheres + (size_t)e.y * 22, o, wi, h.t, hp, hn, hdu, front);
+                            analytic_frame(e.x, S.spheres + (size_t)e.y * 22, o, wi, h.t, hp, hn, hdu, front);
                         }
                         V3 nact = front ? hn : -hn;
                         V3 Li = dot(nact, -wi) < 0 ? V3{0, 0, 0} : V3{Lt.L[0], Lt.L[1], Lt.L[2]};
@@ -816,6 +816,14 @@ struct DeviceArena {
     }
 };
 
+// Half extent along y of a disk's box, Disk::GetBoundingBox (Disk.cpp:34-43): r * sinTheta of the
+// world normal. q is the shared sphere/disk record (centre, radius, local-to-world, world-to-local).
+static float disk_offset_y(const float* q) {
+    V3 wn = mat3_mul(q + 4, V3{0, 1, 0});
+    float sinTheta = std::sqrt(1.f - wn.y * wn.y);
+    return q[3] * sinTheta;
+}
+
 static void validate_desc(const c2_scene_desc* d) {
     if (!d) fail(C2_ERR_INVALID, "null scene description");
     if (d->num_entities < 0 || d->num_triangles < 0 || d->num_spheres < 0 || d->num_lights < 0 ||
@@ -828,7 +836,7 @@ static void validate_desc(const c2_scene_desc* d) {
         int k = d->entity_shape_kind[e], s = d->entity_shape_index[e];
         if (k == C2_SHAPE_TRIANGLE) {
             if (s < 0 || s >= d->num_triangles) fail(C2_ERR_INVALID, "entity triangle index out of range");
-        } else if (k == C2_SHAPE_SPHERE) {
+        } else if (k == C2_SHAPE_SPHERE || k == C2_SHAPE_DISK) {
             if (s < 0 || s >= d->num_spheres) fail(C2_ERR_INVALID, "entity sphere index out of range");
         } else {
             fail(C2_ERR_INVALID, "unknown entity shape kind");
@@ -945,6 +953,13 @@ struct SceneGPU {
                     b.lo[c] = fmin_ref(fmin_ref(p[c], p[3 + c]), p[6 + c]);
                     b.hi[c] = fmax_ref(fmax_ref(p[c], p[3 + c]), p[6 + c]);
                 }
+            } else if (k == C2_SHAPE_DISK) {  // Disk::GetBoundingBox, Disk.cpp:34-43
+                const float* q = &sp3[(size_t)sh * 22];
+                float off[3] = {q[3], disk_offset_y(q), q[3]};
+                for (int c = 0; c < 3; c++) {
+                    b.lo[c] = q[c] - off[c];
+                    b.hi[c] = q[c] + off[c];
+                }
             } else {
                 const float* q = &sp3[(size_t)sh * 22];
                 for (int c = 0; c < 3; c++) {
@@ -980,9 +995,9 @@ struct SceneGPU {
             } else {
                 const float* q = &sp3[(size_t)sh * 22];
                 prims.push_back(make_float4(q[0], q[1], q[2], q[3]));
-                prims.push_back(make_float4(__int_as_float_host(sh + o.sph), 0.f, 0.f, 0.f));
-                prims.push_back(make_float4(0.f, __int_as_float_host(e + o.ent), __int_as_float_host(C2_SHAPE_SPHERE),
-                                            0.f));
+                prims.push_back(make_float4(__int_as_float_host(sh + o.sph), k == C2_SHAPE_DISK ? disk_offset_y(q) : 0.f,
+                                            0.f, 0.f));
+                prims.push_back(make_float4(0.f, __int_as_float_host(e + o.ent), __int_as_float_host(k), 0.f));
             }
         }
         for (int e = 0; e < ne; e++) {
@@ -1009,6 +1024,9 @@ struct SceneGPU {
                     const float* p = &tp[(size_t)D.shape * 9];
                     V3 v0{p[0], p[1], p[2]}, v1{p[3], p[4], p[5]}, v2{p[6], p[7], p[8]};
                     D.area = length(cross(v1 - v0, v2 - v0)) / 2.f;
+                } else if (D.shape_kind == C2_SHAPE_DISK) {  // Disk::SurfaceArea, Disk.cpp:83-86
+                    float r = sp3[(size_t)D.shape * 22 + 3];
+                    D.area = kPi * r * r;
                 } else {  // Sphere::SurfaceArea
                     float r = sp3[(size_t)D.shape * 22 + 3];
                     D.area = 4 * kPi * r * r;

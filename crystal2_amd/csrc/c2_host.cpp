@@ -433,7 +433,10 @@ static void add_triangles(SceneData& d, const float* vd, int ntri, const M4& M, 
 }
 
 // src/Shapes/Sphere.cpp:19-30
-static void add_sphere(SceneData& d, V3 pos, float radius, V3 rot, int material, const float* lemit) {
+// Sphere (Sphere.cpp:19-30) and Disk (Disk.cpp:19-29) share one record: centre, radius,
+// local2world = mat3(rotation), world2local = its transpose.
+static void add_sphere(SceneData& d, V3 pos, float radius, V3 rot, int material, const float* lemit,
+                       int kind = C2_SHAPE_SPHERE) {
     if (material < -1 || material >= (int)d.materials.size()) fail(C2_ERR_INVALID, "add_sphere: bad material index");
     M4 R = m4_identity();
     R = m4_rotate(R, rot.x / 180.f * kPi, V3{1, 0, 0});
@@ -449,7 +452,7 @@ static void add_sphere(SceneData& d, V3 pos, float radius, V3 rot, int material,
     int sidx = (int)(d.spheres.size() / 22);
     d.spheres.insert(d.spheres.end(), rec, rec + 22);
     int entity = (int)d.kind.size();
-    d.kind.push_back(C2_SHAPE_SPHERE);
+    d.kind.push_back(kind);
     d.shape.push_back(sidx);
     d.material.push_back(material);
     if (lemit) {
@@ -638,8 +641,12 @@ static void load_scene_json(SceneData& d, const std::string& path, const std::st
                 M = m4_rotate(M, r.z, V3{0, 0, 1});
                 M = m4_scale(M, s);
                 add_triangles(d, vd.data(), (int)(vd.size() / 24), M, material, hasLight ? lemit : nullptr);
-            } else if (st == "Disk") {
-                fail(C2_ERR_UNSUPPORTED, "Disk shapes are not supported");
+            } else if (st == "Disk") {  // Disk::CreateDisk, Disk.cpp:7-17
+                V3 pos = jvec3(sh.member("Position"));
+                float r = (float)sh.member("Radius").as_float();
+                V3 rot{0, 0, 0};
+                if (sh.has("Rotation")) rot = jvec3(sh.member("Rotation"));
+                add_sphere(d, pos, r, rot, material, hasLight ? lemit : nullptr, C2_SHAPE_DISK);
             } else {
                 fail(C2_ERR_PARSE, "Invalid Shape Type!");
             }
@@ -879,6 +886,16 @@ int c2_scene_add_sphere(c2_scene* s, const float center[3], float radius, const 
         V3 r{0, 0, 0};
         if (rot) r = V3{rot[0], rot[1], rot[2]};
         add_sphere(s->d, V3{center[0], center[1], center[2]}, radius, r, material, lemit);
+    });
+}
+
+int c2_scene_add_disk(c2_scene* s, const float center[3], float radius, const float rot[3], int32_t material,
+                      const float* lemit) {
+    return guard([&] {
+        if (!s || !center) fail(C2_ERR_INVALID, "null argument");
+        V3 r{0, 0, 0};
+        if (rot) r = V3{rot[0], rot[1], rot[2]};
+        add_sphere(s->d, V3{center[0], center[1], center[2]}, radius, r, material, lemit, C2_SHAPE_DISK);
     });
 }
 

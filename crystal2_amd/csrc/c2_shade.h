@@ -132,6 +132,26 @@ __device__ __forceinline__ void sphere_frame(const float* sp, V3 o, V3 d, float 
     n = mat3_mul(sp + 4, N);
 }
 
+// Disk::Intersect hit construction, Disk.cpp:54-64 (uv is not needed)
+__device__ __forceinline__ void disk_frame(const float* sp, V3 o, V3 d, float t, V3& p, V3& n, V3& du,
+                                           bool& front) {
+    V3 c{sp[0], sp[1], sp[2]};
+    V3 P = mat3_mul(sp + 13, o - c);
+    V3 dl = mat3_mul(sp + 13, d);
+    V3 hit = P + dl * t;
+    V3 N = mat3_mul(sp + 4, V3{0, 1, 0});
+    front = dl.y <= 0;
+    n = front ? N : -N;
+    p = mat3_mul(sp + 4, hit) + c;
+    du = normalize(c - p);
+}
+// Hit frame of an analytic shape (spheres and disks share the record)
+__device__ __forceinline__ void analytic_frame(int kind, const float* sp, V3 o, V3 d, float t, V3& p, V3& n,
+                                               V3& du, bool& front) {
+    if (kind == C2_SHAPE_DISK) disk_frame(sp, o, d, t, p, n, du, front);
+    else sphere_frame(sp, o, d, t, p, n, du, front);
+}
+
 __device__ __forceinline__ Inter make_inter(const DevScene& S, V3 o, V3 d, const Hit& h) {
     Inter it;
     int4 e = S.entities[h.entity];
@@ -143,7 +163,7 @@ __device__ __forceinline__ Inter make_inter(const DevScene& S, V3 o, V3 d, const
         tri_frame(S, e.y, d, h.u, h.v, it.n, it.dpdu, it.front);
         it.p = o + d * h.t;
     } else {
-        sphere_frame(S.spheres + (size_t)e.y * 22, o, d, h.t, it.p, it.n, it.dpdu, it.front);
+        analytic_frame(e.x, S.spheres + (size_t)e.y * 22, o, d, h.t, it.p, it.n, it.dpdu, it.front);
     }
     it.dpdv = cross(it.n, it.dpdu);
     return it;
@@ -165,10 +185,10 @@ __device__ __forceinline__ bool shape_hit(const DevScene& S, int kind, int shape
     }
     const float* sp = S.spheres + (size_t)shape * 22;
     float t;
-    if (!sphere_closest(sp, o, d, t)) return false;
+    if (!(kind == C2_SHAPE_DISK ? disk_closest(sp, o, d, t) : sphere_closest(sp, o, d, t))) return false;
     V3 du;
     bool front;
-    sphere_frame(sp, o, d, t, p, n, du, front);
+    analytic_frame(kind, sp, o, d, t, p, n, du, front);
     return true;
 }
 
@@ -203,12 +223,21 @@ __device__ __forceinline__ V3 sample_li(const DevScene& S, const DevLight& L, V3
         ln = normalize(b0 * V3{sh[0], sh[1], sh[2]} + su * V3{sh[3], sh[4], sh[5]} + sv * V3{sh[6], sh[7], sh[8]});
     } else {  // Sphere::SampleSurface + NextUnitSphere, Sphere.cpp:90-95, Utils.h:138-149
         const float* s3 = S.spheres + (size_t)L.shape * 22;
+        if (L.shape_kind == C2_SHAPE_DISK) {  // Disk::SampleSurface, Disk.cpp:88-96
+            float r = sqrtf(u0);
+            float phi = u1 * kTwoPi;
+            float R = s3[3];
+            V3 dw = mat3_mul(s3 + 4, V3{cos_r(phi) * r * R, 0.f, -sin_r(phi) * r * R});
+            lp = V3{s3[0], s3[1], s3[2]} + dw;
+            ln = mat3_mul(s3 + 4, V3{0, 1, 0});
+        } else {
         float y = 2.f * u0 - 1.f;
         float phi = u1 * kTwoPi;
         float r = sqrtf(1 - y * y);
         V3 dir{r * cos_r(phi), y, r * sin_r(phi)};
         lp = V3{s3[0], s3[1], s3[2]} + dir * s3[3];
         ln = dir;
+        }
     }
     endpoint = lp;
     V3 dir = endpoint - sp;

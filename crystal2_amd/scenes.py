@@ -30,7 +30,7 @@ import numpy as np
 
 __all__ = [
     "write_obj", "write_config", "cornell", "icosphere_mesh", "displaced_icosphere",
-    "icosphere_rays", "spheres_scene", "SceneFiles",
+    "icosphere_rays", "spheres_scene", "SceneFiles", "disk_scene",
 ]
 
 
@@ -214,6 +214,62 @@ def material_zoo(directory: str, width: int = 64, height: int = 64, spp: int = 8
     cp, cn, cf = _quad([-0.9, -1.0, 0.6], [0.5, 0, 0], [0, 0.5, 0], [0, 0, 1])
     write_obj(os.path.join(directory, "curtain.obj"), cp, cn, cf)
     objects.append(_mesh_object("curtain.obj", None))
+    scene = {"Camera": {"Type": "Normal", "Position": [0, 0, 3.7], "LookAt": [0, 0, 0], "Up": [0, 1, 0],
+                        "FOV": 40},
+             "Scene": {"Textures": {}, "Objects": objects}}
+    sj = os.path.join(directory, "scene.json")
+    with open(sj, "w") as fh:
+        json.dump(scene, fh, indent=1)
+    cj = os.path.join(directory, "config.json")
+    write_config(cj, width, height, spp, depth, threads=threads, accel=accel)
+    return SceneFiles(directory, sj, cj)
+
+
+def disk_scene(directory: str, width: int = 64, height: int = 64, spp: int = 8, depth: int = 6,
+               threads: int = 8, accel: str = "BVH") -> SceneFiles:
+    """Disks in every role (parity fixture for the Disk shape).
+
+    Cornell-style box lit by a disk area light under the ceiling (rotated 180 degrees about x,
+    so it faces down) and a small tilted emissive disk seen partly from behind; disks with a
+    mirror (on the floor, no Rotation key), diffuse (upright), glass, GGX and Oren-Nayar
+    material at several rotations; one diffuse sphere and a point light.
+    """
+    os.makedirs(directory, exist_ok=True)
+    white = {"Type": "Diffuse", "Color": [0.73, 0.73, 0.73]}
+    objects = [{"Type": "Light", "LightType": "Point", "Position": [-0.5, 0.5, 0.6], "Flux": [2, 2, 2]}]
+    walls = {
+        "floor.obj": (_quad([-1, -1, -1], [2, 0, 0], [0, 0, 2], [0, 1, 0]), white),
+        "ceiling.obj": (_quad([-1, 1, -1], [2, 0, 0], [0, 0, 2], [0, -1, 0]), white),
+        "back.obj": (_quad([-1, -1, -1], [2, 0, 0], [0, 2, 0], [0, 0, 1]), white),
+        "left.obj": (_quad([-1, -1, -1], [0, 0, 2], [0, 2, 0], [1, 0, 0]),
+                     {"Type": "Diffuse", "Color": [0.65, 0.05, 0.05]}),
+        "right.obj": (_quad([1, -1, -1], [0, 0, 2], [0, 2, 0], [-1, 0, 0]),
+                      {"Type": "Diffuse", "Color": [0.12, 0.45, 0.15]}),
+    }
+    for name, ((p, n, f), mat) in walls.items():
+        write_obj(os.path.join(directory, name), p, n, f)
+        objects.append(_mesh_object(name, mat))
+
+    def disk(pos, r, rot, mat, lemit=None):
+        sh = {"Type": "Disk", "Position": pos, "Radius": r}
+        if rot is not None:
+            sh["Rotation"] = rot
+        o = {"Type": "Geometry", "Shape": sh, "Material": mat}
+        if lemit is not None:
+            o["AreaLight"] = {"Type": "Diffused", "Lemit": lemit, "Samples": 1}
+        objects.append(o)
+
+    disk([0.0, 0.995, 0.0], 0.35, [180, 0, 0], white, lemit=[8, 8, 8])
+    disk([0.62, 0.35, -0.55], 0.15, [45, 30, 0], white, lemit=[3, 5, 8])
+    disk([-0.35, -0.99, -0.1], 0.45, None, {"Type": "Mirror", "Color": [0.95, 0.95, 0.95]})
+    disk([0.45, -0.55, -0.3], 0.3, [90, 0, 20], {"Type": "Diffuse", "Color": [0.8, 0.7, 0.3]})
+    disk([-0.45, 0.05, 0.2], 0.25, [60, 0, 0], {"Type": "Glass", "Color": [1, 1, 1], "Roughness": [0, 0],
+                                                "IOR": 1.5})
+    disk([0.0, -0.3, -0.75], 0.3, [-30, 45, 0], {"Type": "Micro", "Color": [0.9, 0.6, 0.3],
+                                                 "Roughness": [0.2, 0.2], "IOR": 1.5, "NDF": "GGX"})
+    disk([0.5, 0.3, 0.3], 0.2, [0, 0, 120], {"Type": "OrenNayar", "Color": [0.6, 0.8, 0.9], "Sigma": 0.7})
+    objects.append({"Type": "Geometry", "Shape": {"Type": "Sphere", "Position": [0.05, -0.8, 0.35], "Radius": 0.2},
+                    "Material": {"Type": "Diffuse", "Color": [0.9, 0.9, 0.9]}})
     scene = {"Camera": {"Type": "Normal", "Position": [0, 0, 3.7], "LookAt": [0, 0, 0], "Up": [0, 1, 0],
                         "FOV": 40},
              "Scene": {"Textures": {}, "Objects": objects}}

@@ -52,7 +52,7 @@ const char* c2_last_error(void);
  * (src/Core/Scene.cpp:94-199). Entity order is JSON object order; the entity
  * index is the primitive ID everywhere (hits, ignore lists, light attachment).
  */
-enum { C2_SHAPE_TRIANGLE = 0, C2_SHAPE_SPHERE = 1 };
+enum { C2_SHAPE_TRIANGLE = 0, C2_SHAPE_SPHERE = 1, C2_SHAPE_DISK = 2 };
 
 enum {
     C2_MAT_DIFFUSE = 0,  /* Lambertain                     src/Materials/DiffuseMaterial.cpp:16-20 */
@@ -104,8 +104,9 @@ typedef struct c2_scene_desc {
                                  world space, normals already transformed + normalised
                                  (src/Shapes/TriangleMesh.cpp:29-51)                     */
     int32_t num_spheres;
-    const float* spheres;     /* [num_spheres][22]: center.xyz, radius, local2world[9],
-                                 world2local[9] (glm column-major, src/Shapes/Sphere.cpp:19-30) */
+    const float* spheres;     /* [num_spheres][22]: analytic shapes (spheres and disks):
+                                 center.xyz, radius, local2world[9], world2local[9] (glm
+                                 column-major, src/Shapes/Sphere.cpp:19-30, Disk.cpp:19-29) */
     int32_t num_materials;
     const c2_material* materials;
     int32_t num_lights;
@@ -143,6 +144,10 @@ int  c2_scene_add_triangles(c2_scene* scene, const float* vertex_data, int32_t n
 /* Sphere at center with radius and XYZ rotation in degrees (src/Shapes/Sphere.cpp:19-30). */
 int  c2_scene_add_sphere(c2_scene* scene, const float center[3], float radius,
                          const float rotation_deg[3], int32_t material, const float* lemit);
+/* Disk::CreateDisk (src/Shapes/Disk.cpp:7-29): an entity of kind C2_SHAPE_DISK whose
+ * record sits in the spheres[] array. */
+int  c2_scene_add_disk(c2_scene* scene, const float center[3], float radius,
+                       const float rotation_deg[3], int32_t material, const float* lemit);
 int  c2_scene_add_point_light(c2_scene* scene, const float position[3], const float flux[3]);
 int  c2_scene_set_camera(c2_scene* scene, const c2_camera* camera);
 /* Fills *out with pointers into the scene's arrays (valid until the scene changes). */

@@ -300,11 +300,29 @@ struct Triangle {
     }
 };
 
+// Sphere, or with `disk` set, Disk (src/Shapes/Disk.cpp: the same record)
 struct Sphere {
     vec3 pos;
     float r;
     mat3 l2w, w2l;
-    Box box() const { return Box{pos - vec3(r), pos + vec3(r)}; }  // Sphere.cpp:31-33
+    bool disk = false;
+    Box box() const {
+        if (disk) {  // Disk::GetBoundingBox, Disk.cpp:34-43
+            vec3 wn = mul(l2w, vec3(0, 1, 0));
+            float sinTheta = std::sqrt(1.f - wn.y * wn.y);
+            vec3 off(r, r * sinTheta, r);
+            return Box{pos - off, pos + off};
+        }
+        return Box{pos - vec3(r), pos + vec3(r)};  // Sphere.cpp:31-33
+    }
+    // Disk::Intersect / IntersectTest, Disk.cpp:45-81: the plane y = 0 of the local frame
+    bool diskT(const Ray& ray, vec3& P, vec3& d, float& t) const {
+        P = mul(w2l, ray.o - pos);
+        d = mul(w2l, ray.d);
+        if (d.y == 0.f) return false;
+        t = -P.y / d.y;
+        return true;
+    }
     bool quad(const Ray& ray, vec3& P, vec3& d, float& t1, float& t2) const {
         P = mul(w2l, ray.o - pos);
         d = mul(w2l, ray.d);
@@ -320,6 +338,27 @@ struct Sphere {
     }
     // Sphere::Intersect, Sphere.cpp:39-75
     bool intersect(const Ray& ray, Isec* is) const {
+        if (disk) {
+            vec3 P, d;
+            float t;
+            if (!diskT(ray, P, d, t)) return false;
+            if (t < 0) return false;
+            vec3 hit = P + d * t;
+            if (dot(hit, hit) > r * r) return false;
+            vec3 N = mul(l2w, vec3(0, 1, 0));
+            bool front = d.y <= 0;
+            N = front ? N : -N;
+            vec3 hw = mul(l2w, hit) + pos;
+            vec3 du = normalize(pos - hw);
+            is->t = t;
+            is->p = hw;
+            is->dir = ray.d;
+            is->n = N;
+            is->front = front;
+            is->dpdu = du;
+            is->dpdv = cross(N, du);
+            return true;
+        }
         vec3 P, d;
         float t1, t2;
         if (!quad(ray, P, d, t1, t2)) return false;
@@ -345,14 +384,29 @@ struct Sphere {
     }
     // Sphere::IntersectTest, :77-88
     bool intersectTest(const Ray& ray, float tMin, float tMax) const {
+        if (disk) {
+            vec3 P, d;
+            float t;
+            if (!diskT(ray, P, d, t)) return false;
+            if (t < tMin || t > tMax) return false;
+            vec3 hit = P + d * t;
+            return !(dot(hit, hit) > r * r);
+        }
         if (tMin > tMax) return false;
         vec3 P, d;
         float t1, t2;
         if (!quad(ray, P, d, t1, t2)) return false;
         return (t1 >= tMin && t1 <= tMax) || (t2 >= tMin && t2 <= tMax);
     }
-    float area() const { return 4 * PI * r * r; }
+    float area() const { return disk ? PI * r * r : 4 * PI * r * r; }  // Disk.cpp:83-86
     SurfInfo sample(vec2 s) const {  // NextUnitSphere, Utils.h:138-149
+        if (disk) {  // Disk::SampleSurface, Disk.cpp:88-96
+            float rr = std::sqrt(s.x);
+            float phi = s.y * TWO_PI;
+            vec3 dw = mul(l2w, vec3(o_cos(phi) * rr * r, 0.f, -o_sin(phi) * rr * r));
+            vec3 nw = mul(l2w, vec3(0, 1, 0));
+            return SurfInfo{pos + dw, nw};
+        }
         float y = 2.f * s.x - 1.f;
         float phi = s.y * TWO_PI;
         float rr = std::sqrt(1 - y * y);
@@ -1488,6 +1542,7 @@ int orc_scene_create(const c2_scene_desc* d, int accel, int tie, orc_scene** out
             s->spheres.push_back(sp);
         }
         for (int e = 0; e < d->num_entities; e++) {
+            if (d->entity_shape_kind[e] == C2_SHAPE_DISK) s->spheres[d->entity_shape_index[e]].disk = true;
             s->kind.push_back(d->entity_shape_kind[e]);
             s->shape.push_back(d->entity_shape_index[e]);
             s->mat.push_back(d->entity_material[e]);

@@ -10,6 +10,7 @@ arg_order="ltr" film.
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py direct   # DirectLighting films only
     python tests/golden/make_golden.py extra    # Oren-Nayar / Plastic / Beckmann scene only
+    python tests/golden/make_golden.py disk     # Disk-shape scene only
 """
 import os
 import subprocess
@@ -134,10 +135,33 @@ def extra_materials():
     print("extra-material films written to", HERE)
 
 
+def disks():
+    """The Disk-shape scene: the reference's films for both integrators and its closest /
+    any hits for rays from inside the box; leaves every other fixture untouched."""
+    if not os.path.exists(REF):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    d = os.path.join(SCN, "disks")
+    scenes.disk_scene(d, width=64, height=64, spp=8, depth=6)
+    scenes.write_config(os.path.join(d, "config_dl.json"), 64, 64, 8, 6, integrator="DirectLight")
+    film("disks", 64, 64)
+    film("disks", 64, 64, config="config_dl.json", tag="_dl")
+    rng = np.random.default_rng(17)
+    zr = np.zeros((4096, 8), np.float32)
+    zr[:, 0:3] = rng.uniform(-0.9, 0.9, (4096, 3))
+    v = rng.normal(size=(4096, 3))
+    zr[:, 3:6] = v / np.linalg.norm(v, axis=1, keepdims=True)
+    zr[:, 6] = 0
+    zr[:, 7] = rng.uniform(0.1, 2.0, 4096)
+    hits("disks", zr)
+    print("disk films and hits written to", HERE)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["direct"]:
         direct_lighting()
     elif sys.argv[1:] == ["extra"]:
         extra_materials()
+    elif sys.argv[1:] == ["disk"]:
+        disks()
     else:
         main()

@@ -30,7 +30,7 @@ def gpu_film(sc, w, h, spp, depth, order="rtl", **kw):
 
 
 @pytest.mark.parametrize("name,spp,depth", [("cornell", 4, 5), ("zoo_nophong", 8, 6), ("zoo", 8, 6),
-                                            ("zoo_extra", 8, 6)])
+                                            ("zoo_extra", 8, 6), ("disks", 8, 6)])
 @pytest.mark.parametrize("order", ["ltr", "rtl"])
 def test_film_bit_identical_to_oracle(gpu_available, name, spp, depth, order):
     print(f"case film {name} {order}", flush=True)
@@ -42,7 +42,8 @@ def test_film_bit_identical_to_oracle(gpu_available, name, spp, depth, order):
 
 
 @pytest.mark.parametrize("name,spp,depth,tol", [("cornell", 4, 5, 1e-4), ("zoo_nophong", 8, 6, 1e-4),
-                                                ("zoo", 8, 6, 2e-3), ("zoo_extra", 8, 6, 1e-4)])
+                                                ("zoo", 8, 6, 2e-3), ("zoo_extra", 8, 6, 1e-4),
+                                                ("disks", 8, 6, 1e-4)])
 def test_film_vs_reference_golden(gpu_available, name, spp, depth, tol):
     """The reference binary's films (clang build: left-to-right argument order)."""
     print(f"case golden film {name}", flush=True)
@@ -51,7 +52,7 @@ def test_film_vs_reference_golden(gpu_available, name, spp, depth, tol):
     assert relmse(g, ref) <= tol
 
 
-@pytest.mark.parametrize("name", ["icosphere", "zoo"])
+@pytest.mark.parametrize("name", ["icosphere", "zoo", "disks"])
 def test_hits_vs_reference_golden(gpu_available, name):
     print(f"case hits {name}", flush=True)
     g = np.load(os.path.join(GOLDEN, f"hits_{name}.npz"))
@@ -68,10 +69,12 @@ def test_hits_vs_reference_golden(gpu_available, name):
     assert np.array_equal(anyh, g["any"].astype(bool))
 
 
-def test_any_hit_ignore_entity(gpu_available):
+@pytest.mark.parametrize("name", ["zoo", "disks"])
+def test_any_hit_ignore_entity(gpu_available, name):
     """IntersectTest skips the ignored entity (a light's own shape)."""
-    g = np.load(os.path.join(GOLDEN, "hits_zoo.npz"))
-    sc = load("zoo")
+    print(f"case any-hit ignore {name}", flush=True)
+    g = np.load(os.path.join(GOLDEN, f"hits_{name}.npz"))
+    sc = load(name)
     bvh = c2.GpuBVH(sc)
     ig = np.where(g["ids"] >= 0, g["ids"], -1).astype(np.int32)
     o = pyoracle.OracleScene(sc)
@@ -154,7 +157,8 @@ def test_pixel_window_matches_full_frame(gpu_available):
             assert relmse(f[rows, 8:32], full[rows, 8:32]) < 1e-10
 
 
-@pytest.mark.parametrize("name,spp", [("cornell", 4), ("zoo_nophong", 8), ("zoo", 8), ("zoo_extra", 8)])
+@pytest.mark.parametrize("name,spp", [("cornell", 4), ("zoo_nophong", 8), ("zoo", 8), ("zoo_extra", 8),
+                                      ("disks", 8)])
 def test_direct_lighting_bit_identical_to_oracle(gpu_available, name, spp):
     """DirectLightingIntegrator on the wavefront kernels against the oracle's restatement."""
     print(f"case direct lighting {name}", flush=True)
@@ -167,7 +171,7 @@ def test_direct_lighting_bit_identical_to_oracle(gpu_available, name, spp):
 
 
 @pytest.mark.parametrize("name,spp,tol", [("cornell", 4, 1e-4), ("zoo_nophong", 8, 1e-4), ("zoo", 8, 1e-4),
-                                          ("zoo_extra", 8, 1e-4)])
+                                          ("zoo_extra", 8, 1e-4), ("disks", 8, 1e-4)])
 def test_direct_lighting_vs_reference_golden(gpu_available, name, spp, tol):
     print(f"case direct lighting golden {name}", flush=True)
     g = gpu_film(load(name), 64, 64, spp, 6, "ltr", integrator="direct")

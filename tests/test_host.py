@@ -60,7 +60,8 @@ def test_json_bom_and_numbers(tmp_path):
     (lambda s: s["Scene"]["Objects"][0].update({"AreaLight": None}), -2),
     (lambda s: s["Scene"]["Objects"][0]["Shape"].update({"Type": "Cone"}), -2),
     (lambda s: s["Scene"]["Objects"][0]["Material"].update({"Type": "Velvet"}), -2),
-    (lambda s: s["Scene"]["Objects"][0]["Shape"].update({"Type": "Disk"}), -6),
+    (lambda s: (s["Scene"]["Objects"][0]["Shape"].update({"Type": "Disk"}),
+                s["Scene"]["Objects"][0]["Shape"].pop("Radius")), -2),
     (lambda s: s["Scene"]["Objects"][0]["Material"].update({"Type": "Plastic"}), -2),  # no Kd / Ks
     (lambda s: s["Scene"]["Objects"][0]["Material"].update({"Type": "OrenNayar", "Kd": "wood"}), -6),
     (lambda s: s["Scene"]["Objects"][0]["Material"].update({"Type": "Micro", "Roughness": [0.1, 0.1],
@@ -74,6 +75,23 @@ def test_scene_errors(tmp_path, mutate, code):
     with pytest.raises(c2.Crystal2Error) as e:
         c2.Scene.load(p)
     assert e.value.code == code
+
+
+def test_disk_record(tmp_path):
+    """Disk shares the sphere record: centre, radius, local-to-world, world-to-local (its transpose)."""
+    s = json.loads(json.dumps(BASE))
+    s["Scene"]["Objects"][0]["Shape"] = {"Type": "Disk", "Position": [1, 2, 3], "Radius": 0.25,
+                                         "Rotation": [90, 0, 0]}
+    sc = c2.Scene.load(_write(tmp_path, json.dumps(s)))   # the scene owns the arrays the desc points to
+    d = sc.desc()
+    assert d.num_entities == 1 and d.entity_shape_kind[0] == _abi.C2_SHAPE_DISK
+    rec = np.ctypeslib.as_array(d.spheres, shape=(22,))
+    assert np.array_equal(rec[:4], np.float32([1, 2, 3, 0.25]))
+    l2w = rec[4:13].reshape(3, 3)       # column-major: row c = column c of the matrix
+    w2l = rec[13:22].reshape(3, 3)
+    assert np.array_equal(w2l, l2w.T)
+    # rotation of 90 degrees about x takes the local normal (0, 1, 0) to (0, 0, 1)
+    assert np.allclose(l2w[1], [0, 0, 1], atol=1e-6)
 
 
 def test_malformed_json(tmp_path):

@@ -17,7 +17,8 @@ def load(name):
     return c2.Scene.load(os.path.join(d, "scene.json"), d)
 
 
-@pytest.mark.parametrize("name,spp,depth", [("cornell", 4, 5), ("zoo_nophong", 8, 6), ("zoo_extra", 8, 6)])
+@pytest.mark.parametrize("name,spp,depth", [("cornell", 4, 5), ("zoo_nophong", 8, 6), ("zoo_extra", 8, 6),
+                                            ("disks", 8, 6)])
 def test_oracle_film_bit_identical_to_reference(name, spp, depth):
     """libm variant + clang argument order = the reference binary, bit for bit."""
     sc = load(name)
@@ -44,7 +45,7 @@ def test_portable_math_variant_close_to_reference():
     assert relmse(film, ref) < 1e-4
 
 
-@pytest.mark.parametrize("name", ["icosphere", "zoo"])
+@pytest.mark.parametrize("name", ["icosphere", "zoo", "disks"])
 def test_oracle_hits_match_reference(name):
     g = np.load(os.path.join(GOLDEN, f"hits_{name}.npz"))
     sc = load(name)
@@ -133,7 +134,7 @@ def test_pow5_product_equals_rounded_double_pow():
 
 
 @pytest.mark.parametrize("name,exact", [("cornell", True), ("zoo_nophong", True), ("zoo", False),
-                                        ("zoo_extra", True)])
+                                        ("zoo_extra", True), ("disks", True)])
 def test_oracle_direct_lighting_matches_reference(name, exact):
     """DirectLightingIntegrator films of the reference binary (film_*_dl.npy): bit-identical
     for the libm variant; with Phong the rejection redraws agree statistically only."""
