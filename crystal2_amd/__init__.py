@@ -23,7 +23,7 @@ from . import _abi
 from ._abi import Crystal2Error, check
 
 __all__ = ["Scene", "GpuBVH", "WavefrontPathIntegrator", "RadianceCache", "Config", "Crystal2Error",
-           "halton_index", "halton_value", "load_library"]
+           "halton_index", "halton_value", "load_library", "load_image"]
 
 
 def load_library():
@@ -36,6 +36,18 @@ def _fp(a):
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def load_image(path: str) -> np.ndarray:
+    """stbi_load(path, 3) as the reference's ImageTexels calls it: (height, width, 3) uint8."""
+    lib = load_library()
+    w, h = C.c_int32(), C.c_int32()
+    buf = C.POINTER(C.c_uint8)()
+    check(lib.c2_image_load(path.encode(), C.byref(w), C.byref(h), C.byref(buf)))
+    try:
+        return np.ctypeslib.as_array(buf, shape=(h.value * w.value * 3,)).reshape(h.value, w.value, 3).copy()
+    finally:
+        lib.c2_image_free(buf)
 
 
 @dataclass
@@ -81,13 +93,15 @@ class Scene:
         return cls(h)
 
     def add_material(self, mtype: int, color, roughness=(0.0, 0.0), ior=1.0, phong_k=0, ndf=0, ks=(0.0, 0.0, 0.0),
-                     sigma=0.0) -> int:
+                     sigma=0.0, kd_texture: int = 0) -> int:
         """c2_material fields: `color` is the albedo (Plastic: Kd), `ks` Plastic's Ks,
-        `sigma` OrenNayar's Sigma, `ndf` 0 = GGX, 1 = Beckmann (Micro, Plastic)."""
+        `sigma` OrenNayar's Sigma, `ndf` 0 = GGX, 1 = Beckmann (Micro, Plastic),
+        `kd_texture` a texture id from :meth:`add_texture` (Diffuse, Phong; 0 = `color`)."""
         m = _abi.c2_material()
         m.type = mtype
         m.ndf = ndf
         m.phong_k = phong_k
+        m.kd_texture = int(kd_texture)
         m.color[:] = [float(x) for x in color]
         m.roughness[:] = [float(x) for x in roughness]
         m.ior = float(ior)
@@ -135,6 +149,21 @@ class Scene:
         p = np.asarray(position, np.float32)
         f = np.asarray(flux, np.float32)
         check(self._lib.c2_scene_add_point_light(self._h, _fp(p), _fp(f)))
+
+    def add_texture(self, rgb: np.ndarray) -> int:
+        """ImageTexels from an (height, width, 3) uint8 image, rows top-down; returns its id."""
+        a = np.ascontiguousarray(rgb, dtype=np.uint8)
+        if a.ndim != 3 or a.shape[2] != 3:
+            raise ValueError("texture must be (height, width, 3) uint8")
+        tid = C.c_int32()
+        check(self._lib.c2_scene_add_texture(self._h, a.shape[1], a.shape[0],
+                                             a.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(tid)))
+        return tid.value
+
+    def set_skybox(self, face_ids):
+        """Cubemap faces (posx, negx, posy, negy, posz, negz) as texture ids; all 0 removes it."""
+        ids = np.asarray(face_ids, np.int32)
+        check(self._lib.c2_scene_set_skybox(self._h, _ip(ids)))
 
     def set_camera(self, position, look_at, up, fov_degrees: float):
         cam = _abi.c2_camera()

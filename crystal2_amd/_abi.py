@@ -22,7 +22,7 @@ C2_INTEGRATOR_PATH, C2_INTEGRATOR_DIRECT = 0, 1
 
 
 class c2_material(C.Structure):
-    _fields_ = [("type", C.c_int32), ("ndf", C.c_int32), ("phong_k", C.c_int32), ("pad_", C.c_int32),
+    _fields_ = [("type", C.c_int32), ("ndf", C.c_int32), ("phong_k", C.c_int32), ("kd_texture", C.c_int32),
                 ("color", C.c_float * 3), ("roughness", C.c_float * 2), ("ior", C.c_float),
                 ("ks", C.c_float * 3), ("sigma", C.c_float)]
 
@@ -37,6 +37,10 @@ class c2_camera(C.Structure):
                 ("fov", C.c_float)]
 
 
+class c2_texture(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgb", C.POINTER(C.c_uint8))]
+
+
 class c2_scene_desc(C.Structure):
     _fields_ = [("num_entities", C.c_int32),
                 ("entity_shape_kind", C.POINTER(C.c_int32)), ("entity_shape_index", C.POINTER(C.c_int32)),
@@ -45,7 +49,8 @@ class c2_scene_desc(C.Structure):
                 ("num_spheres", C.c_int32), ("spheres", C.POINTER(C.c_float)),
                 ("num_materials", C.c_int32), ("materials", C.POINTER(c2_material)),
                 ("num_lights", C.c_int32), ("lights", C.POINTER(c2_light)),
-                ("camera", c2_camera)]
+                ("camera", c2_camera), ("num_textures", C.c_int32), ("textures", C.POINTER(c2_texture)),
+                ("skybox", C.c_int32 * 6)]
 
 
 class c2_config(C.Structure):
@@ -127,6 +132,10 @@ SIGNATURES = [
     ("c2_scene_add_sphere", C.c_int, [P, _f32p, C.c_float, _f32p, C.c_int32, _f32p]),
     ("c2_scene_add_disk", C.c_int, [P, _f32p, C.c_float, _f32p, C.c_int32, _f32p]),
     ("c2_scene_add_point_light", C.c_int, [P, _f32p, _f32p]),
+    ("c2_scene_add_texture", C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(C.c_uint8), _i32p]),
+    ("c2_scene_set_skybox", C.c_int, [P, _i32p]),
+    ("c2_image_load", C.c_int, [C.c_char_p, _i32p, _i32p, C.POINTER(C.POINTER(C.c_uint8))]),
+    ("c2_image_free", None, [C.POINTER(C.c_uint8)]),
     ("c2_scene_set_camera", C.c_int, [P, C.POINTER(c2_camera)]),
     ("c2_scene_get_desc", C.c_int, [P, C.POINTER(c2_scene_desc)]),
     ("c2_accel_build", C.c_int, [C.POINTER(c2_scene_desc), C.POINTER(P)]),

@@ -43,6 +43,10 @@ struct DevScene {
     const DevLight* lights;
     const uint16_t* perm;      // Halton permutation tables, dims 0..99
     const int32_t* perm_off;
+    const uchar4* texels;      // every texture's texels (rgb + pad), rows top-down
+    const int4* tex;           // texture id k: tex[k - 1] = (width, height, first texel, 0)
+    const float* lin;          // [256]: (b / 255)^2.2, the texel linearisation
+    const float* tri_uv;       // [ntri][6] vertex uv (allocated with textures only)
     int32_t num_lights;
     int32_t num_entities;
 };

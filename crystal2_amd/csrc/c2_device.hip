@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -107,6 +108,7 @@ struct PathBuffers {
     float4* vtx_du;  // dpdu, (BSDF-sampled f != 0) bits
     float4* vtx_d;   // incoming ray direction, pdf of the BSDF-sampled direction
     int4* vtx_i;     // entity, material, attached light, front face
+    int32_t* texel;  // Kd texel of the extension hit (textured materials only)
     float4* mis_w;   // BSDF-sampled direction (shared by all lights)
     float4* mis_f;   // f * cos for that direction
     float4* mis_L;   // resolved MIS term
@@ -146,6 +148,7 @@ struct View {
     int64_t film_off;        // first pixel of this view's film in the concatenated film
     V3 eye, look, up, right;
     float camW, camH;
+    int32_t sky[6];          // skybox face texture ids (posx, negx, posy, negy, posz, negz), 0 = none
 };
 
 struct PassParams {
@@ -166,12 +169,14 @@ struct PassParams {
 
 // Radiance-cache query features: position, outgoing direction, facing normal,
 // roughness (diffuse 1, specular 0, GGX sqrt(ax ay), Phong sqrt(2 / (K + 2))), albedo.
-__device__ __forceinline__ void cache_features(const DevScene& S, const Inter& it, V3 d, float4* out) {
+__device__ __forceinline__ void cache_features(const DevScene& S, const Inter& it, V3 d, float4* out,
+                                               const int32_t* texel) {
     float rough = 1.f;
     V3 alb{0.f, 0.f, 0.f};
     if (it.material >= 0) {
         const c2_material m = S.materials[it.material];
         alb = V3{m.color[0], m.color[1], m.color[2]};
+        if (m.kd_texture > 0) alb = texel_value(S, *texel);
         if (m.type == C2_MAT_MIRROR) rough = 0.f;
         else if (m.type == C2_MAT_GLASS || m.type == C2_MAT_MICRO || m.type == C2_MAT_PLASTIC)
             rough = sqrtf(m.roughness[0] * m.roughness[1]);
@@ -346,6 +351,42 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, Pa
     }
 }
 
+// Texture stage of scenes with images, between k_extend and k_shade (its own kernel so
+// that the double-evaluated atan2/acos of sphere and disk uv cost the traversal and
+// shading kernels no registers): the Kd texel of a textured hit, and for a ray that
+// left the scene L += beta * skybox(dir) (PathTracingIntegrator.cpp:39-46; the
+// direct-lighting integrator returns it unweighted, DirectLightingIntegrator.cpp:34-38,
+// and beta is 1 there).
+__global__ void __launch_bounds__(kBlock) k_surface(DevScene S, PathBuffers B, PassParams P, int q) {
+    int n = B.counters[q];
+    const int32_t* queue = B.queue[q];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int p = queue[i];
+        float4 hh = B.hit[p];
+        int ent = __float_as_int(hh.w);
+        float4 rd = B.ray_d[p];
+        V3 d{rd.x, rd.y, rd.z};
+        if (ent < 0) {
+            const View& V = P.views[B.vid[p]];
+            if (V.sky[0] > 0) {
+                float4 b4 = B.beta[p];
+                V3 Ls = V3{b4.x, b4.y, b4.z} * sky_lookup(S, V.sky, d);
+                float4 L = B.L[p];
+                B.L[p] = make_float4(L.x + Ls.x, L.y + Ls.y, L.z + Ls.z, L.w);
+            }
+            continue;
+        }
+        const int4 e = S.entities[ent];
+        if (e.z < 0) continue;
+        const int kd = S.materials[e.z].kd_texture;
+        if (kd <= 0) continue;
+        float4 ro = B.ray_o[p];
+        float u, v;
+        surface_uv(S, e, V3{ro.x, ro.y, ro.z}, d, Hit{ent, hh.x, hh.y, hh.z}, u, v);
+        B.texel[p] = texel_index(S, kd, u, v);
+    }
+}
+
 // Vertex stage: emission, BSDF, the BSDF-sampled MIS direction (shared by all
 // lights), continuation sample and Russian roulette; the light loop runs in k_nee.
 __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, PassParams P, int q, int bounce) {
@@ -393,7 +434,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                 const bool training = (meta.w & F_TRAIN) != 0;
                 if (it.material >= 0 && P.cache_bounce >= 0 && !training && bounce == P.cache_bounce) {
                     // the path ends in the radiance cache: L += beta * cache(x, wo) after this kernel
-                    cache_features(S, it, d, B.nrc_feat + (int64_t)p * 4);
+                    cache_features(S, it, d, B.nrc_feat + (int64_t)p * 4, B.texel + p);
                     query = true;
                 } else if (it.material < 0) {
                     // transparent: SpawnRay(dir), no dimensions consumed, :63-68. The
@@ -409,12 +450,14 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                     if (training) {  // training row: features, throughput and radiance so far
                         int64_t r = ((int64_t)(meta.w >> 8) * P.max_depth + bounce);
                         float4* rec = B.trec + r * 6;
-                        cache_features(S, it, d, rec);
+                        cache_features(S, it, d, rec, B.texel + p);
                         rec[4] = make_float4(beta.x, beta.y, beta.z, 0.f);
                         rec[5] = Lpre;
                         B.trec_valid[r] = 1;
                     }
-                    Bsdf bs = make_bsdf(S, it, (uint32_t)idx, (uint32_t)(dim + 2));
+                    // the Kd texel k_surface found, read only for a textured material
+                    const int32_t texel = S.materials[it.material].kd_texture > 0 ? B.texel[p] : 0;
+                    Bsdf bs = make_bsdf(S, it, (uint32_t)idx, (uint32_t)(dim + 2), texel);
                     V3 wo = -d;
                     // UniformSampleAllLights: uL = Get2D(), uB = Get2D(), :179-180
                     float uL0 = halton_value(S, hd, idx, dim), uL1 = halton_value(S, hd, idx, dim + 1);
@@ -565,6 +608,7 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
             root = V.root;
         }
         Inter it{};
+        int32_t texel = 0;
         uint32_t hidx = 0;  // Halton index: the BSDF's Phong redraw stream
         V3 wo{0, 0, 0}, misDir{0, 0, 0};
         float uL0 = 0.f, uL1 = 0.f;
@@ -582,6 +626,8 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
             it.material = vi.y;
             it.light = vi.z;
             it.front = vi.w != 0;
+            // B.texel still holds this vertex's texel (k_surface rewrites it next bounce)
+            if (it.material >= 0 && S.materials[it.material].kd_texture > 0) texel = B.texel[p];
             hidx = (uint32_t)meta.y;
             wo = -it.dir;
             uL0 = vp.w;
@@ -607,7 +653,7 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
                 const DevLight Lt = S.lights[li];
                 // rebuilt per light from the (cached) material record rather than kept
                 // live across the loop: fewer registers, more waves per SIMD
-                const Bsdf bs = make_bsdf(S, it, hidx, 0u);
+                const Bsdf bs = make_bsdf(S, it, hidx, 0u, texel);
                 // (i) light sampling, :199-230
                 V3 lightPos;
                 float pdfL;
@@ -856,7 +902,21 @@ static void validate_desc(const c2_scene_desc* d) {
         if (M.type < C2_MAT_DIFFUSE || M.type > C2_MAT_ORENNAYAR) fail(C2_ERR_INVALID, "invalid material type");
         if ((M.type == C2_MAT_MICRO || M.type == C2_MAT_PLASTIC) && M.ndf != C2_NDF_GGX && M.ndf != C2_NDF_BECKMANN)
             fail(C2_ERR_INVALID, "invalid NDF");
+        if (M.kd_texture < 0 || M.kd_texture > d->num_textures) fail(C2_ERR_INVALID, "material texture id out of range");
+        if (M.kd_texture > 0 && M.type != C2_MAT_DIFFUSE && M.type != C2_MAT_PHONG)
+            fail(C2_ERR_INVALID, "only Diffuse and Phong materials take a Kd texture");
     }
+    if (d->num_textures < 0 || (d->num_textures > 0 && !d->textures)) fail(C2_ERR_INVALID, "bad texture array");
+    for (int t = 0; t < d->num_textures; t++) {
+        const c2_texture& T = d->textures[t];
+        if (T.width <= 0 || T.height <= 0 || !T.rgb) fail(C2_ERR_INVALID, "bad texture");
+    }
+    int sky = 0;
+    for (int f = 0; f < 6; f++) {
+        if (d->skybox[f] < 0 || d->skybox[f] > d->num_textures) fail(C2_ERR_INVALID, "skybox face id out of range");
+        sky += d->skybox[f] > 0;
+    }
+    if (sky != 0 && sky != 6) fail(C2_ERR_INVALID, "a skybox needs all six faces");
     if ((int64_t)d->num_entities >= (1ll << kSlotBits)) fail(C2_ERR_UNSUPPORTED, "too many entities");
 }
 
@@ -869,11 +929,15 @@ struct SceneGPU {
     std::vector<float> tri_pos, tri_shade, spheres;
     std::vector<int4> ents;
     std::vector<DevLight> lights;
+    std::vector<uchar4> texels;
+    std::vector<int4> tex;
+    std::vector<float> tri_uv;
+    std::vector<std::array<int32_t, 6>> sky;  // per scene: skybox face ids in the merged table
     double build_seconds = 0;
 
     // Start of each scene in the concatenated arrays (a single scene has offsets 0).
     struct Offsets {
-        int32_t node, slot, ent, tri, sph, mat, light;
+        int32_t node, slot, ent, tri, sph, mat, light, tex;
     };
     std::vector<Offsets> offs;
     std::vector<Node4> nodes;  // every scene's tree; scene s is rooted at offs[s].node
@@ -891,7 +955,7 @@ struct SceneGPU {
         for (int k = 0; k < n; k++) {
             Offsets o{(int32_t)nodes.size(), (int32_t)(prims.size() / 3), (int32_t)ents.size(),
                       (int32_t)(tri_pos.size() / 9), (int32_t)(spheres.size() / 22), (int32_t)mats.size(),
-                      (int32_t)lights.size()};
+                      (int32_t)lights.size(), (int32_t)tex.size()};
             offs.push_back(o);
             append_scene(descs[k], o, k == 0);
             if ((int64_t)ents.size() >= (1ll << kSlotBits) || (int64_t)prims.size() / 3 >= (1ll << kSlotBits))
@@ -908,6 +972,16 @@ struct SceneGPU {
         dev.lights = arena.upload(lights);
         dev.perm = arena.upload(halton_perm_data());
         dev.perm_off = arena.upload(halton_perm_offsets());
+        if (!tex.empty()) {
+            dev.texels = arena.upload(texels);
+            dev.tex = arena.upload(tex);
+            dev.tri_uv = arena.upload(tri_uv);
+            // (b / 255)^2.2 per byte value: powf evaluated in double and rounded, the
+            // convention of every transcendental here
+            std::vector<float> lin(256);
+            for (int b = 0; b < 256; b++) lin[b] = (float)std::pow((double)((float)b / 255.f), (double)2.2f);
+            dev.lin = arena.upload(lin);
+        }
         dev.num_lights = (int32_t)lights.size();
         dev.num_entities = (int32_t)ents.size();
     }
@@ -1043,7 +1117,29 @@ struct SceneGPU {
         tri_pos.insert(tri_pos.end(), tp.begin(), tp.end());
         tri_shade.insert(tri_shade.end(), ts.begin(), ts.end());
         spheres.insert(spheres.end(), sp3.begin(), sp3.end());
-        mats.insert(mats.end(), d->materials, d->materials + d->num_materials);
+        for (int m = 0; m < d->num_materials; m++) {
+            c2_material mm = d->materials[m];
+            if (mm.kd_texture > 0) mm.kd_texture += o.tex;
+            mats.push_back(mm);
+        }
+        // textures: ids shift by the textures before this scene; vertex uv per triangle
+        for (int t = 0; t < d->num_textures; t++) {
+            const c2_texture& T = d->textures[t];
+            tex.push_back(make_int4(T.width, T.height, (int32_t)texels.size(), 0));
+            for (int64_t i = 0; i < (int64_t)T.width * T.height; i++)
+                texels.push_back(make_uchar4(T.rgb[i * 3], T.rgb[i * 3 + 1], T.rgb[i * 3 + 2], 255));
+            if (texels.size() >= (1ull << 31)) fail(C2_ERR_UNSUPPORTED, "textures exceed 2^31 texels");
+        }
+        for (int t = 0; t < nt; t++) {
+            const float* v = d->triangles + (size_t)t * 24;
+            for (int k = 0; k < 3; k++) {
+                tri_uv.push_back(v[k * 8 + 6]);
+                tri_uv.push_back(v[k * 8 + 7]);
+            }
+        }
+        std::array<int32_t, 6> sk{};
+        for (int f = 0; f < 6; f++) sk[f] = d->skybox[f] > 0 ? d->skybox[f] + o.tex : 0;
+        sky.push_back(sk);
     }
 
     static float __int_as_float_host(int32_t i) {
@@ -1382,6 +1478,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
         V.nd_elig = nd;
         maxElig = std::max(maxElig, V.num_elig);
         V.film_off = film_off;
+        for (int f = 0; f < 6; f++) V.sky[f] = SG.sky[k][f];
         film_off += (int64_t)v.width * v.height;
         pt->views.push_back(V);
     }
@@ -1439,6 +1536,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     B.mis_hit = A.alloc<int32_t>(ch);
     if (elig.empty()) elig.push_back(-1);
     B.elig = A.upload(elig);
+    B.texel = A.alloc<int32_t>(ch);
     B.mis_w = A.alloc<float4>(ch);
     B.mis_f = A.alloc<float4>(ch);
     B.mis_L = A.alloc<float4>(ch);
@@ -1604,6 +1702,11 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         {
                             KTimer kt(pt, C2_K_EXTEND, s);
                             hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
+                        }
+                        if (S.tex) {
+                            KTimer kt(pt, C2_K_SHADE, s);
+                            hipLaunchKernelGGL(k_surface, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
+                            pt->launches++;
                         }
                         {
                             KTimer kt(pt, C2_K_SHADE, s);

@@ -24,6 +24,7 @@
 #include <random>
 #include <sstream>
 
+#include "c2_image.h"
 #include "c2_vec.h"
 
 namespace c2 {
@@ -391,10 +392,24 @@ static M4 m4_inverse(const M4& M) {
 
 static int add_material(SceneData& d, const c2_material& m) {
     if (m.type < C2_MAT_DIFFUSE || m.type > C2_MAT_ORENNAYAR) fail(C2_ERR_INVALID, "invalid material type");
+    if (m.kd_texture < 0 || m.kd_texture > (int32_t)d.textures.size())
+        fail(C2_ERR_INVALID, "material names a texture id that does not exist");
+    if (m.kd_texture && m.type != C2_MAT_DIFFUSE && m.type != C2_MAT_PHONG)
+        fail(C2_ERR_INVALID, "only Diffuse and Phong materials take a Kd texture");
     if ((m.type == C2_MAT_MICRO || m.type == C2_MAT_PLASTIC) && m.ndf != C2_NDF_GGX && m.ndf != C2_NDF_BECKMANN)
         fail(C2_ERR_INVALID, "invalid NDF");
     d.materials.push_back(m);
     return (int)d.materials.size() - 1;
+}
+
+static int32_t add_texture(SceneData& d, int32_t w, int32_t h, const uint8_t* rgb) {
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (int64_t)1 << 28 || !rgb) fail(C2_ERR_INVALID, "bad texture size");
+    TextureData t;
+    t.width = w;
+    t.height = h;
+    t.rgb.assign(rgb, rgb + (size_t)w * h * 3);
+    d.textures.push_back(std::move(t));
+    return (int32_t)d.textures.size();
 }
 
 static void add_triangles(SceneData& d, const float* vd, int ntri, const M4& M, int material, const float* lemit) {
@@ -546,17 +561,24 @@ static int32_t parse_ndf(const std::string& ndf) {  // Material.cpp:52-64
     fail(C2_ERR_PARSE, "Cannot find NDF Type: " + ndf);
 }
 
-static c2_material parse_material(const JNode& n) {
+static c2_material parse_material(const JNode& n, const std::map<std::string, int32_t>& texByName) {
     c2_material m{};
     const std::string& type = n.member("Type").as_string();
     auto color = [&]() {
-        if (n.has("Kd")) fail(C2_ERR_UNSUPPORTED, "image textures (\"Kd\") are not supported");
         V3 c = jvec3(n.member("Color"));
         m.color[0] = c.x; m.color[1] = c.y; m.color[2] = c.z;
     };
+    // "Kd" names a texture for Diffuse and Phong, else "Color" (Material.cpp:25-35,91-102)
+    auto kd_or_color = [&]() {
+        if (!n.has("Kd")) return color();
+        const std::string& name = n.member("Kd").as_string();
+        auto it = texByName.find(name);
+        if (it == texByName.end()) fail(C2_ERR_PARSE, "Texture name doesn't exist: " + name);  // Scene.cpp:71-79
+        m.kd_texture = it->second;
+    };
     if (type == "Diffuse") {
         m.type = C2_MAT_DIFFUSE;
-        color();
+        kd_or_color();
     } else if (type == "Mirror") {
         m.type = C2_MAT_MIRROR;
         color();
@@ -584,7 +606,7 @@ static c2_material parse_material(const JNode& n) {
     } else if (type == "Phong") {
         m.type = C2_MAT_PHONG;
         m.phong_k = (int32_t)n.member("K").as_int();
-        color();
+        kd_or_color();
     } else if (type == "OrenNayar") {  // Material.cpp:104-120 (a Kd texture is "Not implemented")
         m.type = C2_MAT_ORENNAYAR;
         if (n.has("Kd")) fail(C2_ERR_UNSUPPORTED, "OrenNayar with a Kd texture is not implemented");
@@ -602,6 +624,12 @@ static void load_scene_json(SceneData& d, const std::string& path, const std::st
     const JNode& sc = root.member("Scene");
     const JNode& tex = sc.member("Textures");  // required (Scene.cpp:31)
     if (tex.type != JNode::Object) fail(C2_ERR_PARSE, "\"Textures\" must be an object");
+    // Scene::loadTextures (Scene.cpp:84-92): every named image is loaded, used or not
+    std::map<std::string, int32_t> texByName;
+    for (const auto& kv : tex.obj) {
+        ImageRGB im = load_image(join_path(base, kv.second.member("Path").as_string()));
+        texByName[kv.first] = add_texture(d, im.width, im.height, im.rgb.data());
+    }
     const JNode& objs = sc.member("Objects");
     if (objs.type != JNode::Array) fail(C2_ERR_PARSE, "\"Objects\" must be an array");
     for (const JNode& o : objs.arr) {
@@ -610,7 +638,7 @@ static void load_scene_json(SceneData& d, const std::string& path, const std::st
             const JNode& sh = o.member("Shape");
             const std::string& st = sh.member("Type").as_string();
             int material = -1;
-            if (o.has("Material")) material = add_material(d, parse_material(o.member("Material")));
+            if (o.has("Material")) material = add_material(d, parse_material(o.member("Material"), texByName));
             float lemit[3];
             bool hasLight = o.has("AreaLight");
             if (hasLight) {
@@ -665,8 +693,15 @@ static void load_scene_json(SceneData& d, const std::string& path, const std::st
             fail(C2_ERR_PARSE, "Invalid Object Type!");
         }
     }
-    if (sc.has("Skybox") && sc.member("Skybox").type != JNode::Null)
-        fail(C2_ERR_UNSUPPORTED, "skybox textures are not supported");
+    // Scene::loadSkybox (Scene.cpp:128-143): <Path>/posx.jpg ... negz.jpg
+    if (sc.has("Skybox") && sc.member("Skybox").type != JNode::Null) {
+        const std::string dir = sc.member("Skybox").member("Path").as_string();
+        static const char* faces[6] = {"posx.jpg", "negx.jpg", "posy.jpg", "negy.jpg", "posz.jpg", "negz.jpg"};
+        for (int f = 0; f < 6; f++) {
+            ImageRGB im = load_image(join_path(base, dir + "/" + faces[f]));
+            d.skybox[f] = add_texture(d, im.width, im.height, im.rgb.data());
+        }
+    }
     // Camera::CreateCamera (src/Cameras/Camera.cpp:7-20)
     const JNode& cam = root.member("Camera");
     if (cam.member("Type").as_string() != "Normal") fail(C2_ERR_PARSE, "Invalid Camera Type!");
@@ -910,6 +945,42 @@ int c2_scene_add_point_light(c2_scene* s, const float pos[3], const float flux[3
     });
 }
 
+int c2_scene_add_texture(c2_scene* s, int32_t w, int32_t h, const uint8_t* rgb, int32_t* id) {
+    return guard([&] {
+        if (!s || !rgb) fail(C2_ERR_INVALID, "null argument");
+        int32_t k = add_texture(s->d, w, h, rgb);
+        if (id) *id = k;
+    });
+}
+
+int c2_scene_set_skybox(c2_scene* s, const int32_t ids[6]) {
+    return guard([&] {
+        if (!s || !ids) fail(C2_ERR_INVALID, "null argument");
+        int set = 0;
+        for (int f = 0; f < 6; f++) {
+            if (ids[f] < 0 || ids[f] > (int32_t)s->d.textures.size()) fail(C2_ERR_INVALID, "skybox face id out of range");
+            set += ids[f] > 0;
+        }
+        if (set != 0 && set != 6) fail(C2_ERR_INVALID, "a skybox needs all six faces");
+        for (int f = 0; f < 6; f++) s->d.skybox[f] = ids[f];
+    });
+}
+
+int c2_image_load(const char* path, int32_t* w, int32_t* h, uint8_t** rgb) {
+    return guard([&] {
+        if (!path || !w || !h || !rgb) fail(C2_ERR_INVALID, "null argument");
+        ImageRGB im = load_image(path);
+        uint8_t* buf = (uint8_t*)std::malloc(im.rgb.size());
+        if (!buf) fail(C2_ERR_NOMEM, "image buffer");
+        std::memcpy(buf, im.rgb.data(), im.rgb.size());
+        *w = im.width;
+        *h = im.height;
+        *rgb = buf;
+    });
+}
+
+void c2_image_free(uint8_t* rgb) { std::free(rgb); }
+
 int c2_scene_set_camera(c2_scene* s, const c2_camera* cam) {
     return guard([&] {
         if (!s || !cam) fail(C2_ERR_INVALID, "null argument");
@@ -937,6 +1008,12 @@ int c2_scene_get_desc(const c2_scene* s, c2_scene_desc* out) {
         r.num_lights = (int32_t)d.lights.size();
         r.lights = d.lights.data();
         r.camera = d.camera;
+        d.texture_desc.resize(d.textures.size());
+        for (size_t i = 0; i < d.textures.size(); i++)
+            d.texture_desc[i] = c2_texture{d.textures[i].width, d.textures[i].height, d.textures[i].rgb.data()};
+        r.num_textures = (int32_t)d.textures.size();
+        r.textures = d.texture_desc.data();
+        for (int f = 0; f < 6; f++) r.skybox[f] = d.skybox[f];
         *out = r;
     });
 }

@@ -69,12 +69,20 @@ inline uint32_t hash_u32(uint32_t a, uint32_t b) {
     return h;
 }
 
+struct TextureData {
+    int32_t width = 0, height = 0;
+    std::vector<uint8_t> rgb;  // rows top-down
+};
+
 struct SceneData {
     std::vector<int32_t> kind, shape, material, light;
     std::vector<float> tris;     // [n][3][8]
     std::vector<float> spheres;  // [n][22]
     std::vector<c2_material> materials;
     std::vector<c2_light> lights;
+    std::vector<TextureData> textures;     // id k = textures[k - 1]
+    mutable std::vector<c2_texture> texture_desc;  // views of `textures` for c2_scene_desc
+    int32_t skybox[6] = {0, 0, 0, 0, 0, 0};
     c2_camera camera{};
     bool has_camera = false;
 };

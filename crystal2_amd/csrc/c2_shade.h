@@ -169,6 +169,83 @@ __device__ __forceinline__ Inter make_inter(const DevScene& S, V3 o, V3 d, const
     return it;
 }
 
+// ------------------------------------------------------------------------- textures
+// glm::clamp(x, 0, .99999f) = min(max(x, 0), .99999f) with glm's (x < y) ? y : x and
+// (y < x) ? y : x. A NaN coordinate passes the clamp and makes the reference's (int) cast
+// undefined; here it reads column / row 0.
+C2_HD float uv_clamp(float x) {
+    x = x < 0.f ? 0.f : x;
+    x = .99999f < x ? .99999f : x;
+    return x == x ? x : 0.f;
+}
+
+// ImageTexels::GetTexel, Texel.cpp:15-19: nearest texel, rows flipped (v = 0 is the
+// bottom row). Returns the flat index into texels.
+__device__ __forceinline__ int32_t texel_index(const DevScene& S, int id, float u, float v) {
+    const int4 t = S.tex[id - 1];
+    u = uv_clamp(u);
+    v = uv_clamp(v);
+    int x = (int)(u * (float)t.x);
+    int y = t.y - (int)(v * (float)t.y) - 1;
+    return t.z + y * t.x + x;
+}
+// texel value, linearised as (b / 255)^2.2 (Texel.cpp:24-38)
+__device__ __forceinline__ V3 texel_value(const DevScene& S, int32_t i) {
+    const uchar4 c = S.texels[i];
+    return V3{S.lin[c.x], S.lin[c.y], S.lin[c.z]};
+}
+__device__ __forceinline__ V3 tex_lookup(const DevScene& S, int id, float u, float v) {
+    return texel_value(S, texel_index(S, id, u, v));
+}
+
+// ImageCubemapTexture::Evaluate / XYZ2CubeUV, CubemapTexture.cpp:24-60
+__device__ __forceinline__ V3 sky_lookup(const DevScene& S, const int32_t* sky, V3 p) {
+    float ax = fabsf(p.x), ay = fabsf(p.y), az = fabsf(p.z);
+    int id;
+    float u, v;
+    if (ax >= ay && ax >= az) {
+        id = p.x >= 0 ? 0 : 1;
+        u = -p.z / p.x;
+        v = p.y / ax;
+    } else if (ay >= az) {
+        id = p.y >= 0 ? 2 : 3;
+        u = p.x / ay;
+        v = -p.z / p.y;
+    } else {
+        id = p.z >= 0 ? 4 : 5;
+        u = p.x / p.z;
+        v = p.y / az;
+    }
+    u = u * 0.5f + 0.5f;
+    v = v * 0.5f + 0.5f;
+    return tex_lookup(S, sky[id], u, v);
+}
+
+// Surface uv of a hit: barycentric vertex uv (Triangle.cpp:41-43), Sphere.cpp:56-58
+// (theta / pi, acos(y / r) / pi + 1 in the local frame), Disk.cpp:61-62.
+__device__ __forceinline__ void surface_uv(const DevScene& S, int4 e, V3 o, V3 d, const Hit& h, float& u,
+                                           float& v) {
+    if (e.x == C2_SHAPE_TRIANGLE) {
+        const float* t = S.tri_uv + (size_t)e.y * 6;
+        float b0 = 1 - h.u - h.v;
+        u = (b0 * t[0] + h.u * t[2]) + h.v * t[4];
+        v = (b0 * t[1] + h.u * t[3]) + h.v * t[5];
+        return;
+    }
+    const float* sp = S.spheres + (size_t)e.y * 22;
+    V3 c{sp[0], sp[1], sp[2]};
+    V3 P = mat3_mul(sp + 13, o - c);
+    V3 dl = mat3_mul(sp + 13, d);
+    V3 hit = P + dl * h.t;
+    if (e.x == C2_SHAPE_DISK) {
+        u = length(hit) / sp[3];
+        v = (atan2_r(-hit.z, hit.x) + kPi) / kTwoPi;
+    } else {
+        u = atan2_r(-hit.z, hit.x) / kPi;
+        v = acos_r(hit.y / sp[3]) / kPi + 1.0f;
+    }
+}
+
 // Full single-shape intersection used by Shape::PdfLight (Shape.cpp:4-12):
 // returns hit position and the facing shading normal.
 __device__ __forceinline__ bool shape_hit(const DevScene& S, int kind, int shape, V3 o, V3 d, V3& p, V3& n) {
@@ -624,7 +701,9 @@ __device__ __forceinline__ V3 bxdf_sample(const Bsdf& b, float sx, float sy, V3 
 
 // Material::ComputeScatteringFunctions -> one BxDF (all supported materials
 // build exactly one), src/Materials/*.cpp
-__device__ __forceinline__ Bsdf make_bsdf(const DevScene& S, const Inter& it, uint32_t idx, uint32_t rdim) {
+// texel: the Kd texel of the hit (flat index) for a textured Diffuse or Phong material
+__device__ __forceinline__ Bsdf make_bsdf(const DevScene& S, const Inter& it, uint32_t idx, uint32_t rdim,
+                                          int32_t texel) {
     Bsdf b;
     b.kind = BK_NONE;
     b.type = 0;
@@ -640,6 +719,8 @@ __device__ __forceinline__ Bsdf make_bsdf(const DevScene& S, const Inter& it, ui
     if (it.material < 0) return b;
     const c2_material m = S.materials[it.material];
     b.R = V3{m.color[0], m.color[1], m.color[2]};
+    // Diffuse / Phong with a "Kd" texture: R = texture(uv), DiffuseMaterial.cpp:16-20
+    if (m.kd_texture > 0) b.R = texel_value(S, texel);
     switch (m.type) {
         case C2_MAT_DIFFUSE:
             b.kind = BK_LAMBERT;

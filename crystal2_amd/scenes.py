@@ -38,8 +38,9 @@ def _f(x) -> str:
     return repr(float(np.float32(x)))
 
 
-def write_obj(path: str, positions: np.ndarray, normals: np.ndarray, faces: np.ndarray) -> None:
-    """Write an indexed triangle OBJ (``v``, ``vn``, ``f a//a b//b c//c``, 1-based)."""
+def write_obj(path: str, positions: np.ndarray, normals: np.ndarray, faces: np.ndarray, uvs=None) -> None:
+    """Write an indexed triangle OBJ (``v``, ``vn``, ``f a//a b//b c//c``, 1-based); with
+    ``uvs`` also ``vt`` lines and ``f a/a/a ...`` faces."""
     positions = np.asarray(positions, dtype=np.float32)
     normals = np.asarray(normals, dtype=np.float32)
     faces = np.asarray(faces, dtype=np.int64)
@@ -48,8 +49,14 @@ def write_obj(path: str, positions: np.ndarray, normals: np.ndarray, faces: np.n
         lines.append(f"v {_f(p[0])} {_f(p[1])} {_f(p[2])}")
     for n in normals:
         lines.append(f"vn {_f(n[0])} {_f(n[1])} {_f(n[2])}")
+    if uvs is not None:
+        for t in np.asarray(uvs, dtype=np.float32):
+            lines.append(f"vt {_f(t[0])} {_f(t[1])}")
     for a, b, c in faces + 1:
-        lines.append(f"f {a}//{a} {b}//{b} {c}//{c}")
+        if uvs is not None:
+            lines.append(f"f {a}/{a}/{a} {b}/{b}/{b} {c}/{c}/{c}")
+        else:
+            lines.append(f"f {a}//{a} {b}//{b} {c}//{c}")
     with open(path, "w") as fh:
         fh.write("\n".join(lines))
         fh.write("\n")

@@ -70,7 +70,9 @@ typedef struct c2_material {
     int32_t type;          /* C2_MAT_*                                      */
     int32_t ndf;           /* C2_NDF_* (Micro, Plastic)                     */
     int32_t phong_k;       /* Blinn-Phong exponent K                        */
-    int32_t pad_;
+    int32_t kd_texture;    /* Diffuse, Phong: 0 = the constant colour, k >= 1 = the
+                              texture c2_scene_add_texture returned as k ("Kd",
+                              Material.cpp:25-29,91-95)                      */
     float color[3];        /* albedo / R (constant texture); Plastic: Kd    */
     float roughness[2];    /* NDF alpha_x, alpha_y (Glass: 0,0 = specular)  */
     float ior;             /* IOR (Glass, Micro)                            */
@@ -92,6 +94,14 @@ typedef struct c2_camera {
     float fov;             /* radians, FOV/180*pi as Camera::CreateCamera computes it */
 } c2_camera;
 
+/* An 8-bit RGB image as stbi_load(..., 3) returns it: rows top-down, 3 bytes per texel.
+ * Texels are read nearest-neighbour with the reference's uv clamp and row flip, and
+ * converted to linear as (byte/255)^2.2 (src/Textures/Texel.cpp:15-38). */
+typedef struct c2_texture {
+    int32_t width, height;
+    const uint8_t* rgb;
+} c2_texture;
+
 typedef struct c2_scene_desc {
     int32_t num_entities;
     const int32_t* entity_shape_kind;   /* C2_SHAPE_*                                */
@@ -112,6 +122,10 @@ typedef struct c2_scene_desc {
     int32_t num_lights;
     const c2_light* lights;   /* the reference's light order (Scene::_lights)       */
     c2_camera camera;
+    int32_t num_textures;
+    const c2_texture* textures;  /* texture id k is textures[k - 1]                    */
+    int32_t skybox[6];        /* texture ids of the cubemap faces posx, negx, posy, negy,
+                                 posz, negz; all 0 = no skybox (Scene.cpp:128-143)     */
 } c2_scene_desc;
 
 /* ---------------------------------------------------------------- host scene builder
@@ -149,6 +163,19 @@ int  c2_scene_add_sphere(c2_scene* scene, const float center[3], float radius,
 int  c2_scene_add_disk(c2_scene* scene, const float center[3], float radius,
                        const float rotation_deg[3], int32_t material, const float* lemit);
 int  c2_scene_add_point_light(c2_scene* scene, const float position[3], const float flux[3]);
+/* ImageTexels (src/Textures/Texel.cpp:4-38): copies width*height*3 bytes of RGB, rows
+ * top-down; *id (>= 1) is what c2_material.kd_texture and c2_scene_set_skybox take. */
+int  c2_scene_add_texture(c2_scene* scene, int32_t width, int32_t height, const uint8_t* rgb,
+                          int32_t* id);
+/* ImageCubemapTexture (src/Textures/CubemapTexture.cpp:3-14): texture ids of the faces
+ * posx, negx, posy, negy, posz, negz (all > 0), or all 0 to remove the skybox. Rays that
+ * leave the scene then add beta * skybox(direction) (PathTracingIntegrator.cpp:39-46). */
+int  c2_scene_set_skybox(c2_scene* scene, const int32_t face_ids[6]);
+/* stbi_load(path, &w, &h, &c, 3) as the reference calls it (Texel.cpp:5): PNG or baseline
+ * JPEG to 8-bit RGB with stb_image's arithmetic. *rgb is malloc'd; free it with
+ * c2_image_free. Progressive JPEG gives C2_ERR_UNSUPPORTED. */
+int  c2_image_load(const char* path, int32_t* width, int32_t* height, uint8_t** rgb);
+void c2_image_free(uint8_t* rgb);
 int  c2_scene_set_camera(c2_scene* scene, const c2_camera* camera);
 /* Fills *out with pointers into the scene's arrays (valid until the scene changes). */
 int  c2_scene_get_desc(const c2_scene* scene, c2_scene_desc* out);

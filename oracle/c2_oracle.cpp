@@ -44,6 +44,9 @@ inline float o_atan(float x) { return std::atan(x); }
 inline float o_exp(float x) { return std::exp(x); }
 inline float o_log(float x) { return std::log(x); }
 inline float o_pow5(float x) { return std::pow(x, 5.f); }
+inline float o_atan2(float y, float x) { return std::atan2(y, x); }
+inline float o_acos(float x) { return std::acos(x); }
+inline float o_powf(float x, float y) { return std::pow(x, y); }
 // std::pow(float, int) promotes to double
 inline float o_pow5d(float x) { return (float)std::pow((double)x, 5); }
 #else
@@ -53,6 +56,9 @@ inline float o_tan(float x) { return (float)std::tan((double)x); }
 inline float o_atan(float x) { return (float)std::atan((double)x); }
 inline float o_exp(float x) { return (float)std::exp((double)x); }
 inline float o_log(float x) { return (float)std::log((double)x); }
+inline float o_atan2(float y, float x) { return (float)std::atan2((double)y, (double)x); }
+inline float o_acos(float x) { return (float)std::acos((double)x); }
+inline float o_powf(float x, float y) { return (float)std::pow((double)x, (double)y); }
 // x^5 as three double products (x^2 exact), rounded once to float: the GPU's form.
 inline float o_pow5(float x) {
     double d = x;
@@ -89,6 +95,8 @@ struct vec2 {
     vec2() = default;
     vec2(float a, float b) : x(a), y(b) {}
 };
+inline vec2 operator+(vec2 a, vec2 b) { return {a.x + b.x, a.y + b.y}; }
+inline vec2 operator*(float s, vec2 a) { return {s * a.x, s * a.y}; }
 inline vec3 operator+(vec3 a, vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 inline vec3 operator-(vec3 a, vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline vec3 operator-(vec3 a) { return {-a.x, -a.y, -a.z}; }
@@ -189,6 +197,7 @@ struct Isec {
     float t = INFINITY;
     bool front = false;
     vec3 p, dir, dpdu, dpdv, n;
+    vec2 uv;
     // SpawnRay, SurfaceInteraction.cpp:12-16
     Ray spawn(vec3 d) const {
         bool back = dot(d, n) < 0;
@@ -277,6 +286,7 @@ struct Triangle {
         is->front = front;
         is->dpdu = du;
         is->dpdv = cross(N, du);
+        is->uv = bary.x * v[0]->uv + bary.y * v[1]->uv + bary.z * v[2]->uv;  // :43
         return true;
     }
     // Triangle::IntersectTest, Triangle.cpp:83-93
@@ -357,6 +367,7 @@ struct Sphere {
             is->front = front;
             is->dpdu = du;
             is->dpdv = cross(N, du);
+            is->uv = vec2(length(hit) / r, (o_atan2(-hit.z, hit.x) + PI) / TWO_PI);  // Disk.cpp:61-62
             return true;
         }
         vec3 P, d;
@@ -368,6 +379,8 @@ struct Sphere {
         if (std::isnan(t) || t < 0) return false;
         vec3 hit = P + d * t;
         vec3 N = normalize(hit);
+        float theta = o_atan2(-hit.z, hit.x) / PI;  // Sphere.cpp:57-58
+        float phi = o_acos(hit.y / r) / PI + 1.0f;
         bool front = dot(d, N) < 0;
         N = front ? N : -N;
         vec3 real = mul(l2w, hit) + pos;
@@ -380,6 +393,7 @@ struct Sphere {
         is->front = front;
         is->dpdu = du;
         is->dpdv = cross(N, du);
+        is->uv = vec2(theta, phi);
         return true;
     }
     // Sphere::IntersectTest, :77-88
@@ -969,6 +983,25 @@ struct MatDesc {
     float ior;
     vec3 ks;
     float sigma;
+    int kdTex;  // texture id (1-based), 0 = the constant colour
+};
+// ImageTexels, src/Textures/Texel.cpp:15-38: texels linearised at load, nearest lookup
+struct Texture {
+    int w = 0, h = 0;
+    std::vector<vec3> data;
+    // glm::clamp(uv, 0, .99999f) with glm's (x < y) ? y : x / (y < x) ? y : x; a NaN
+    // coordinate (undefined in the reference's int cast) reads column / row 0
+    static float clampUV(float x) {
+        x = x < 0.f ? 0.f : x;
+        x = .99999f < x ? .99999f : x;
+        return x == x ? x : 0.f;
+    }
+    vec3 texel(vec2 uv) const {
+        float u = clampUV(uv.x), v = clampUV(uv.y);
+        int x = (int)(u * w);
+        int y = h - (int)(v * h) - 1;
+        return data[(size_t)y * w + x];
+    }
 };
 struct Light {
     int type;  // 0 area, 1 point
@@ -990,6 +1023,8 @@ struct orc_scene {
     std::vector<int> kind, shape, mat, light;
     std::vector<MatDesc> mats;
     std::vector<Light> lights;
+    std::vector<Texture> textures;
+    int sky[6] = {0, 0, 0, 0, 0, 0};
     c2_camera cam;
     int accel = ORC_ACCEL_BVH;
     int tie = ORC_TIE_FIRST;
@@ -1162,10 +1197,10 @@ struct orc_scene {
         const MatDesc& md = mats[m];
         BxDF b;
         switch (md.type) {
-            case C2_MAT_DIFFUSE:  // DiffuseMaterial.cpp:16-20
+            case C2_MAT_DIFFUSE:  // DiffuseMaterial.cpp:16-20 (Kd texture: UVTexture::Evaluate)
                 b.kind = BX_LAMBERT;
                 b.type = DIFF | REFL;
-                b.R = md.color;
+                b.R = md.kdTex ? textures[md.kdTex - 1].texel(is.uv) : md.color;
                 break;
             case C2_MAT_MIRROR:  // MirrorMaterial.cpp:11-13
                 b.kind = BX_SPECREFL;
@@ -1222,7 +1257,7 @@ struct orc_scene {
             case C2_MAT_PHONG:  // PhongMaterial.cpp:13-22
                 b.kind = BX_PHONG;
                 b.type = GLOSSY | REFL;
-                b.R = md.color;
+                b.R = md.kdTex ? textures[md.kdTex - 1].texel(is.uv) : md.color;
                 b.k = md.k;
                 break;
             default: return;
@@ -1230,6 +1265,26 @@ struct orc_scene {
         b.redrawIdx = (uint32_t)idx;
         b.redrawDim = (uint32_t)dimHint;
         bsdf.b[bsdf.n++] = b;
+    }
+
+    // ImageCubemapTexture::Evaluate + XYZ2CubeUV, CubemapTexture.cpp:24-60
+    vec3 skybox(vec3 p) const {
+        float ax = std::abs(p.x), ay = std::abs(p.y), az = std::abs(p.z);
+        int id;
+        vec2 uv;
+        if (ax >= ay && ax >= az) {
+            id = p.x >= 0 ? 0 : 1;
+            uv = vec2(-p.z / p.x, p.y / ax);
+        } else if (ay >= az) {
+            id = p.y >= 0 ? 2 : 3;
+            uv = vec2(p.x / ay, -p.z / p.y);
+        } else {
+            id = p.z >= 0 ? 4 : 5;
+            uv = vec2(p.x / p.z, p.y / az);
+        }
+        uv.x = uv.x * 0.5f + 0.5f;
+        uv.y = uv.y * 0.5f + 0.5f;
+        return textures[sky[id] - 1].texel(uv);
     }
 
     // DiffusedAreaLight::EvalEmitRadiance, DiffusedAreaLight.cpp:11-21
@@ -1411,7 +1466,7 @@ struct orc_scene {
         if (maxDepth == 0) return vec3(0.f);
         for (int chain = 0; chain < kDlChainLimit; chain++) {
             Isec is;
-            if (!intersect(ray, &is)) return vec3(0.f);  // no skybox on this path
+            if (!intersect(ray, &is)) return sky[0] ? skybox(ray.d) : vec3(0.f);  // :34-38
             vec3 L = Le(is, -ray.d);
             BSDF bsdf;
             buildBSDF(is, bsdf, smp.idx, smp.dim + 2);  // uB's first dimension
@@ -1431,7 +1486,10 @@ struct orc_scene {
         bool lightPath = true;
         for (int bounces = 0; bounces < maxDepth; bounces++) {
             Isec is;
-            if (!intersect(ray, &is)) break;
+            if (!intersect(ray, &is)) {
+                if (sky[0]) L += beta * skybox(ray.d);  // :39-46
+                break;
+            }
             vec3 wo = -ray.d;
             vec3 N = is.n;
             if (lightPath) L += beta * Le(is, -ray.d);
@@ -1552,8 +1610,22 @@ int orc_scene_create(const c2_scene_desc* d, int accel, int tie, orc_scene** out
             const c2_material& cm = d->materials[m];
             s->mats.push_back(MatDesc{cm.type, cm.ndf, cm.phong_k, vec3(cm.color[0], cm.color[1], cm.color[2]),
                                       vec2(cm.roughness[0], cm.roughness[1]), cm.ior,
-                                      vec3(cm.ks[0], cm.ks[1], cm.ks[2]), cm.sigma});
+                                      vec3(cm.ks[0], cm.ks[1], cm.ks[2]), cm.sigma, cm.kd_texture});
         }
+        for (int t = 0; t < d->num_textures; t++) {  // ImageTexels ctor, Texel.cpp:28-38
+            const c2_texture& ct = d->textures[t];
+            Texture tx;
+            tx.w = ct.width;
+            tx.h = ct.height;
+            tx.data.resize((size_t)ct.width * ct.height);
+            for (size_t i = 0; i < tx.data.size(); i++) {
+                float c[3];
+                for (int k = 0; k < 3; k++) c[k] = o_powf((float)ct.rgb[i * 3 + k] / 255.f, 2.2f);
+                tx.data[i] = vec3(c[0], c[1], c[2]);
+            }
+            s->textures.push_back(std::move(tx));
+        }
+        for (int f = 0; f < 6; f++) s->sky[f] = d->skybox[f];
         for (int l = 0; l < d->num_lights; l++) {
             const c2_light& cl = d->lights[l];
             Light L;

@@ -23,6 +23,10 @@
 //       StartPixel((x,y)); idx = GetVectorIndexForSample(k)
 //       (src/Samplers/HaltonSampler.cpp:251-270); SampleValue(idx, d) for
 //       d < ndims (:272-291). out.bin: n int32 idx, then n x ndims float32.
+//   crystal2_ref decode <image> <out.bin>
+//       stbi_load(image, 3 components), the call ImageTexels::CreateImageTexels
+//       makes (src/Textures/Texel.cpp:4-13). out.bin: int32 width, int32 height,
+//       then width*height*3 bytes, rows top-down.
 #include <Core/Scene.h>
 #include <Core/FrameBuffer.h>
 #include <Core/SurfaceInteraction.h>
@@ -33,6 +37,7 @@
 #include <Loaders/ConfigLoader.h>
 #include <Samplers/HaltonSampler.h>
 #include <SJson/SJson.h>
+#include <stbi/stb_image.h>
 
 #include <chrono>
 #include <cstdio>
@@ -155,11 +160,26 @@ int cmdHalton(int argc, char** argv) {
     return 0;
 }
 
+int cmdDecode(int argc, char** argv) {
+    if (argc < 4) return 2;
+    int w = 0, h = 0, c = 0;
+    unsigned char* px = stbi_load(argv[2], &w, &h, &c, 3);
+    if (!px) return 3;
+    FILE* f = std::fopen(argv[3], "wb");
+    if (!f) return 3;
+    int32_t wh[2] = {w, h};
+    std::fwrite(wh, sizeof(int32_t), 2, f);
+    std::fwrite(px, 1, (size_t)w * h * 3, f);
+    std::fclose(f);
+    stbi_image_free(px);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: crystal2_ref render|hits|halton ...\n");
+        std::fprintf(stderr, "usage: crystal2_ref render|hits|halton|decode ...\n");
         return 2;
     }
     std::string cmd = argv[1];
@@ -167,6 +187,7 @@ int main(int argc, char** argv) {
     if (cmd == "render") rc = cmdRender(argc, argv);
     else if (cmd == "hits") rc = cmdHits(argc, argv);
     else if (cmd == "halton") rc = cmdHalton(argc, argv);
+    else if (cmd == "decode") rc = cmdDecode(argc, argv);
     if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", cmd.c_str());
     std::fflush(stdout);
     // The reference's FixedThreadPool workers are detached and still blocked on

@@ -11,6 +11,8 @@ arg_order="ltr" film.
     python tests/golden/make_golden.py direct   # DirectLighting films only
     python tests/golden/make_golden.py extra    # Oren-Nayar / Plastic / Beckmann scene only
     python tests/golden/make_golden.py disk     # Disk-shape scene only
+    python tests/golden/make_golden.py images   # image-decoder fixtures only
+    python tests/golden/make_golden.py textured # textures + skybox scene only
 """
 import os
 import subprocess
@@ -21,6 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
 from crystal2_amd import scenes  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref", "crystal2_ref")
@@ -32,9 +35,9 @@ def ref(*args, cwd):
     return out.stdout
 
 
-def film(name, w, h, config="config.json", tag=""):
+def film(name, w, h, config="config.json", tag="", scene="scene.json"):
     d = os.path.join(SCN, name)
-    ref("render", config, "scene.json", "film.f32", cwd=d)
+    ref("render", config, scene, "film.f32", cwd=d)
     f = np.fromfile(os.path.join(d, "film.f32"), np.float32).reshape(h, w, 3)
     os.remove(os.path.join(d, "film.f32"))
     np.save(os.path.join(HERE, f"film_{name}{tag}.npy"), f)
@@ -156,6 +159,135 @@ def disks():
     print("disk films and hits written to", HERE)
 
 
+def _grad(h, w, seed):
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w]
+    img = np.stack([(x * 255) // max(w - 1, 1), (y * 255) // max(h - 1, 1), ((x + y) * 7) % 256], -1).astype(np.uint8)
+    img[::5] = rng.integers(0, 256, img[::5].shape)
+    return img
+
+
+def _stb_decode(path):
+    out = path + ".bin"
+    subprocess.run([REF, "decode", path, out], check=True)
+    raw = open(out, "rb").read()
+    os.remove(out)
+    w, h = np.frombuffer(raw[:8], np.int32)
+    return np.frombuffer(raw[8:], np.uint8).reshape(h, w, 3)
+
+
+def images():
+    """Image-decoder fixtures: small PNG / JPEG files of every kind the decoders handle,
+    written by tests/golden/imgcodec.py, and what the reference's stb_image decodes them to."""
+    import imgcodec as ic
+    if not os.path.exists(REF):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    d = os.path.join(HERE, "images")
+    os.makedirs(d, exist_ok=True)
+    rng = np.random.default_rng(12)
+    files = []
+    img = _grad(21, 35, 1)
+    for samp in ((1, 1), (2, 1), (1, 2), (2, 2)):
+        for rst in (0, 2):
+            f = f"j{samp[0]}{samp[1]}_r{rst}.jpg"
+            ic.write_jpeg(os.path.join(d, f), img, sampling=samp, restart=rst)
+            files.append(f)
+    for mk in ("adobe0", "rgbids", "none"):
+        ic.write_jpeg(os.path.join(d, f"j_{mk}.jpg"), img, marker=mk)
+        files.append(f"j_{mk}.jpg")
+    ic.write_jpeg(os.path.join(d, "j_gray.jpg"), img[..., 1])
+    files.append("j_gray.jpg")
+    ic.write_jpeg(os.path.join(d, "j_1x1.jpg"), img[:1, :1])
+    files.append("j_1x1.jpg")
+    for color, ch, depths in ((0, 1, (1, 2, 4, 8, 16)), (2, 3, (8, 16)), (3, 1, (1, 4, 8)), (4, 2, (8, 16)),
+                              (6, 4, (8, 16))):
+        for dep in depths:
+            for il in (False, True):
+                hi = (1 << dep) if color != 3 else min(1 << dep, 100)
+                smp = rng.integers(0, hi, (13, 11, ch))
+                pal = rng.integers(0, 256, (100, 3)) if color == 3 else None
+                f = f"p{color}_{dep}{'_i' if il else ''}.png"
+                ic.write_png(os.path.join(d, f), smp, color, dep, palette=pal, interlace=il)
+                files.append(f)
+    np.savez_compressed(os.path.join(HERE, "images_stb.npz"), **{f: _stb_decode(os.path.join(d, f)) for f in files})
+    print(len(files), "image fixtures written to", d)
+
+
+def textured():
+    """Textures and the skybox: Kd textures (PNG and JPEG) on a triangle floor and wall,
+    a sphere and a disk, a cubemap skybox (six JPEGs) seen directly, in a mirror and as
+    lighting through the open box; the reference's films for both integrators."""
+    import imgcodec as ic
+    if not os.path.exists(REF):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    d = os.path.join(SCN, "textured")
+    os.makedirs(os.path.join(d, "sky"), exist_ok=True)
+    y, x = np.mgrid[0:32, 0:32]
+    checker = np.where(((x // 4) + (y // 4)) % 2 == 0, 200, 40).astype(np.uint8)
+    checker = np.stack([checker, (x * 8).astype(np.uint8), checker[::-1]], -1)
+    ic.write_png(os.path.join(d, "checker.png"), checker, 2, 8)
+    ic.write_jpeg(os.path.join(d, "wall.jpg"), _grad(48, 40, 2), sampling=(2, 2))
+    ic.write_png(os.path.join(d, "unused.png"), checker[:4, :4, :1], 0, 8)
+    tint = [(255, 120, 60), (60, 120, 255), (240, 240, 255), (90, 70, 50), (120, 255, 120), (255, 255, 120)]
+    for f, col in zip(("posx", "negx", "posy", "negy", "posz", "negz"), tint):
+        yy, xx = np.mgrid[0:24, 0:24]
+        face = (np.asarray(col, np.float64)[None, None, :] * (0.5 + 0.5 * ((xx // 6 + yy // 6) % 2))[..., None])
+        face[yy < 3] = 255  # top rows bright: shows the row flip
+        ic.write_jpeg(os.path.join(d, "sky", f + ".jpg"), face.astype(np.uint8), sampling=(2, 2))
+    white = {"Type": "Diffuse", "Color": [0.73, 0.73, 0.73]}
+    objects = [{"Type": "Light", "LightType": "Point", "Position": [-0.4, 0.4, 0.8], "Flux": [3, 3, 3]}]
+    fp, fn, ff = scenes._quad([-1, -1, -1], [2, 0, 0], [0, 0, 2], [0, 1, 0])
+    scenes.write_obj(os.path.join(d, "floor.obj"), fp, fn, ff, uvs=[[-0.25, -0.25], [1.25, -0.25], [1.25, 1.25],
+                                                                    [-0.25, 1.25]])
+    objects.append({"Type": "Geometry", "Shape": {"Type": "TriangleMesh", "ObjFile": "floor.obj",
+                                                  "Translation": [0, 0, 0], "Scale": [1, 1, 1], "Rotation": [0, 0, 0]},
+                    "Material": {"Type": "Diffuse", "Kd": "checker"}})
+    bp, bn, bf = scenes._quad([-1, -1, -1], [2, 0, 0], [0, 2, 0], [0, 0, 1])
+    scenes.write_obj(os.path.join(d, "back.obj"), bp, bn, bf, uvs=[[0, 0], [1, 0], [1, 1], [0, 1]])
+    objects.append({"Type": "Geometry", "Shape": {"Type": "TriangleMesh", "ObjFile": "back.obj",
+                                                  "Translation": [0, 0, 0], "Scale": [1, 1, 1], "Rotation": [0, 0, 0]},
+                    "Material": {"Type": "Diffuse", "Kd": "wall"}})
+    lp, ln, lf = scenes._quad([-1, -1, -1], [0, 0, 2], [0, 2, 0], [1, 0, 0])
+    scenes.write_obj(os.path.join(d, "left.obj"), lp, ln, lf)
+    objects.append(scenes._mesh_object("left.obj", {"Type": "Diffuse", "Color": [0.65, 0.05, 0.05]}))
+    objects.append({"Type": "Geometry", "Shape": {"Type": "Sphere", "Position": [-0.45, -0.62, -0.25], "Radius": 0.35,
+                                                  "Rotation": [0, 30, 0]},
+                    "Material": {"Type": "Diffuse", "Kd": "checker"}})
+    objects.append({"Type": "Geometry", "Shape": {"Type": "Disk", "Position": [0.45, -0.35, -0.2], "Radius": 0.3,
+                                                  "Rotation": [70, 0, -20]},
+                    "Material": {"Type": "Diffuse", "Kd": "wall"}})
+    objects.append({"Type": "Geometry", "Shape": {"Type": "Sphere", "Position": [0.35, -0.75, 0.45], "Radius": 0.25},
+                    "Material": {"Type": "Mirror", "Color": [0.95, 0.95, 0.95]}})
+    objects.append({"Type": "Geometry", "Shape": {"Type": "Sphere", "Position": [-0.3, 0.1, 0.5], "Radius": 0.15},
+                    "Material": {"Type": "Glass", "Color": [1, 1, 1], "Roughness": [0, 0], "IOR": 1.5}})
+    objects.append({"Type": "Geometry", "Shape": {"Type": "Disk", "Position": [0.2, 0.85, -0.3], "Radius": 0.25,
+                                                  "Rotation": [180, 0, 0]},
+                    "Material": white, "AreaLight": {"Type": "Diffused", "Lemit": [6, 6, 6], "Samples": 1}})
+    scene = {"Camera": {"Type": "Normal", "Position": [0, 0, 3.7], "LookAt": [0, 0, 0], "Up": [0, 1, 0], "FOV": 40},
+             "Scene": {"Skybox": {"Path": "sky"},
+                       "Textures": {"checker": {"Path": "checker.png"}, "wall": {"Path": "wall.jpg"},
+                                    "unused": {"Path": "unused.png"}},
+                       "Objects": objects}}
+    import copy
+    import json
+    with open(os.path.join(d, "scene.json"), "w") as fh:
+        json.dump(scene, fh, indent=1)
+    # the same scene with Blinn-Phong (K 12 / 30) on the textured wall and disk: its
+    # rejection redraws use the reference's shared generator, so that film agrees
+    # statistically only
+    phong = copy.deepcopy(scene)
+    for o, k in ((phong["Scene"]["Objects"][2], 12), (phong["Scene"]["Objects"][5], 30)):
+        o["Material"] = {"Type": "Phong", "K": k, "Kd": "wall"}
+    with open(os.path.join(d, "scene_phong.json"), "w") as fh:
+        json.dump(phong, fh, indent=1)
+    scenes.write_config(os.path.join(d, "config.json"), 64, 64, 8, 6)
+    scenes.write_config(os.path.join(d, "config_dl.json"), 64, 64, 8, 6, integrator="DirectLight")
+    film("textured", 64, 64)
+    film("textured", 64, 64, config="config_dl.json", tag="_dl")
+    film("textured", 64, 64, tag="_phong", scene="scene_phong.json")
+    print("textured-scene films written to", HERE)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["direct"]:
         direct_lighting()
@@ -163,5 +295,9 @@ if __name__ == "__main__":
         extra_materials()
     elif sys.argv[1:] == ["disk"]:
         disks()
+    elif sys.argv[1:] == ["images"]:
+        images()
+    elif sys.argv[1:] == ["textured"]:
+        textured()
     else:
         main()

@@ -16,8 +16,10 @@ SCN = os.path.join(GOLDEN, "scenes")
 
 
 def load(name):
+    # "<dir>:<json>" names another scene file in a fixture directory
+    name, _, js = name.partition(":")
     d = os.path.join(SCN, name)
-    return c2.Scene.load(os.path.join(d, "scene.json"), d)
+    return c2.Scene.load(os.path.join(d, js or "scene.json"), d)
 
 
 def gpu_film(sc, w, h, spp, depth, order="rtl", **kw):
@@ -30,7 +32,8 @@ def gpu_film(sc, w, h, spp, depth, order="rtl", **kw):
 
 
 @pytest.mark.parametrize("name,spp,depth", [("cornell", 4, 5), ("zoo_nophong", 8, 6), ("zoo", 8, 6),
-                                            ("zoo_extra", 8, 6), ("disks", 8, 6)])
+                                            ("zoo_extra", 8, 6), ("disks", 8, 6), ("textured", 8, 6),
+                                            ("textured:scene_phong.json", 8, 6)])
 @pytest.mark.parametrize("order", ["ltr", "rtl"])
 def test_film_bit_identical_to_oracle(gpu_available, name, spp, depth, order):
     print(f"case film {name} {order}", flush=True)
@@ -43,7 +46,7 @@ def test_film_bit_identical_to_oracle(gpu_available, name, spp, depth, order):
 
 @pytest.mark.parametrize("name,spp,depth,tol", [("cornell", 4, 5, 1e-4), ("zoo_nophong", 8, 6, 1e-4),
                                                 ("zoo", 8, 6, 2e-3), ("zoo_extra", 8, 6, 1e-4),
-                                                ("disks", 8, 6, 1e-4)])
+                                                ("disks", 8, 6, 1e-4), ("textured", 8, 6, 1e-4)])
 def test_film_vs_reference_golden(gpu_available, name, spp, depth, tol):
     """The reference binary's films (clang build: left-to-right argument order)."""
     print(f"case golden film {name}", flush=True)
@@ -67,6 +70,30 @@ def test_hits_vs_reference_golden(gpu_available, name):
     assert np.array_equal(t[hit], g["t"][hit])
     anyh = bvh.intersect_test(g["rays"])
     assert np.array_equal(anyh, g["any"].astype(bool))
+
+
+def test_textured_phong_vs_reference_golden(gpu_available):
+    """Blinn-Phong with a Kd texture against the reference binary: statistical only (the
+    reference's Phong redraws come from a shared generator)."""
+    print("case golden film textured phong", flush=True)
+    g = gpu_film(load("textured:scene_phong.json"), 64, 64, 8, 6, "ltr")
+    ref = np.load(os.path.join(GOLDEN, "film_textured_phong.npy"))
+    assert relmse(g, ref) <= 2e-3
+
+
+def test_textures_in_batch_match_single(gpu_available):
+    """Texture ids and skyboxes shift per scene when scenes are batched."""
+    print("case textured batch", flush=True)
+    a, b = load("textured"), load("disks")
+    items = [(a, 32, 32, 4), (b, 32, 32, 4), (a, 24, 40, 2)]
+    br = c2.BatchRenderer(items, max_depth=6, arg_order="ltr")
+    alone = [gpu_film(sc, w, h, spp, 6, "ltr") for sc, w, h, spp in items]
+    for use_graph in (False, True):
+        br.render(use_graph=use_graph)
+        br.sync()
+        for i in range(len(items)):
+            assert np.array_equal(br.film(i), alone[i]), (use_graph, i)
+    br.close()
 
 
 @pytest.mark.parametrize("name", ["zoo", "disks"])
@@ -158,7 +185,7 @@ def test_pixel_window_matches_full_frame(gpu_available):
 
 
 @pytest.mark.parametrize("name,spp", [("cornell", 4), ("zoo_nophong", 8), ("zoo", 8), ("zoo_extra", 8),
-                                      ("disks", 8)])
+                                      ("disks", 8), ("textured", 8)])
 def test_direct_lighting_bit_identical_to_oracle(gpu_available, name, spp):
     """DirectLightingIntegrator on the wavefront kernels against the oracle's restatement."""
     print(f"case direct lighting {name}", flush=True)
@@ -171,7 +198,7 @@ def test_direct_lighting_bit_identical_to_oracle(gpu_available, name, spp):
 
 
 @pytest.mark.parametrize("name,spp,tol", [("cornell", 4, 1e-4), ("zoo_nophong", 8, 1e-4), ("zoo", 8, 1e-4),
-                                          ("zoo_extra", 8, 1e-4), ("disks", 8, 1e-4)])
+                                          ("zoo_extra", 8, 1e-4), ("disks", 8, 1e-4), ("textured", 8, 1e-4)])
 def test_direct_lighting_vs_reference_golden(gpu_available, name, spp, tol):
     print(f"case direct lighting golden {name}", flush=True)
     g = gpu_film(load(name), 64, 64, spp, 6, "ltr", integrator="direct")

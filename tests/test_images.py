@@ -1,0 +1,64 @@
+"""Image decoding (textures, skybox) against what the reference's stb_image returns for
+the same files (tests/golden/images_stb.npz, written through oracle/_ref by
+tests/golden/make_golden.py images), and the texel linearisation."""
+import ctypes
+import ctypes.util
+import os
+
+import numpy as np
+import pytest
+
+import crystal2_amd as c2
+from conftest import GOLDEN
+
+IMAGES = os.path.join(GOLDEN, "images")
+STB = np.load(os.path.join(GOLDEN, "images_stb.npz"))
+
+
+@pytest.mark.parametrize("name", sorted(STB.files))
+def test_decode_matches_stb(name):
+    got = c2.load_image(os.path.join(IMAGES, name))
+    assert got.shape == STB[name].shape and np.array_equal(got, STB[name]), name
+
+
+def test_scene_textures_decode_like_stb():
+    d = os.path.join(GOLDEN, "scenes", "textured")
+    for f in ("checker.png", "wall.jpg", "sky/posx.jpg"):
+        img = c2.load_image(os.path.join(d, f))
+        assert img.dtype == np.uint8 and img.ndim == 3 and img.shape[2] == 3
+
+
+def test_decode_errors(tmp_path):
+    with pytest.raises(c2.Crystal2Error) as e:
+        c2.load_image(str(tmp_path / "missing.png"))
+    assert e.value.code == -3
+    p = tmp_path / "prog.jpg"  # SOI + SOF2 (progressive)
+    p.write_bytes(bytes([0xFF, 0xD8, 0xFF, 0xC2, 0, 11, 8, 0, 8, 0, 8, 1, 1, 0x11, 0]))
+    with pytest.raises(c2.Crystal2Error) as e:
+        c2.load_image(str(p))
+    assert e.value.code == -6
+    p = tmp_path / "bad.png"
+    good = open(os.path.join(IMAGES, "p2_8.png"), "rb").read()
+    p.write_bytes(good[:60])  # truncated inside IDAT
+    with pytest.raises(c2.Crystal2Error) as e:
+        c2.load_image(str(p))
+    assert e.value.code == -2
+    p = tmp_path / "x.gif"
+    p.write_bytes(b"GIF89a....")
+    with pytest.raises(c2.Crystal2Error) as e:
+        c2.load_image(str(p))
+    assert e.value.code == -6
+
+
+def test_texel_linearisation_powf_equals_double_rounded():
+    """Texel.cpp:24-26 linearises with the C library's powf((float)x / 255.f, 2.2f); the
+    GPU table is the double pow rounded to float. They agree for all 256 byte values
+    (glibc's powf here; numpy's own float32 power differs in 60 of them)."""
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.powf.restype = ctypes.c_float
+    libm.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+    x = np.arange(256, dtype=np.float32) / np.float32(255)
+    f32 = np.array([libm.powf(float(v), float(np.float32(2.2))) for v in x], np.float32)
+    f64 = np.power(x.astype(np.float64), np.float64(np.float32(2.2))).astype(np.float32)
+    assert np.array_equal(f32, f64)
+    assert f64[0] == 0 and f64[255] == 1

@@ -18,13 +18,24 @@ def load(name):
 
 
 @pytest.mark.parametrize("name,spp,depth", [("cornell", 4, 5), ("zoo_nophong", 8, 6), ("zoo_extra", 8, 6),
-                                            ("disks", 8, 6)])
+                                            ("disks", 8, 6), ("textured", 8, 6)])
 def test_oracle_film_bit_identical_to_reference(name, spp, depth):
     """libm variant + clang argument order = the reference binary, bit for bit."""
     sc = load(name)
     film, _ = pyoracle.OracleScene(sc, variant="libm").render(64, 64, spp, depth, arg_order="ltr")
     ref = np.load(os.path.join(GOLDEN, f"film_{name}.npy"))
     assert np.array_equal(film, ref)
+
+
+def test_oracle_textured_phong_statistical():
+    """Blinn-Phong with a Kd texture: statistical agreement (the redraw generator), and
+    every pixel the Phong surfaces do not reach stays bit-identical."""
+    d = os.path.join(SCN, "textured")
+    sc = c2.Scene.load(os.path.join(d, "scene_phong.json"), d)
+    film, _ = pyoracle.OracleScene(sc, variant="libm").render(64, 64, 8, 6, arg_order="ltr")
+    ref = np.load(os.path.join(GOLDEN, "film_textured_phong.npy"))
+    assert relmse(film, ref) < 2e-3
+    assert (film == ref).all(-1).mean() > 0.8
 
 
 def test_oracle_phong_statistical():
@@ -134,7 +145,7 @@ def test_pow5_product_equals_rounded_double_pow():
 
 
 @pytest.mark.parametrize("name,exact", [("cornell", True), ("zoo_nophong", True), ("zoo", False),
-                                        ("zoo_extra", True), ("disks", True)])
+                                        ("zoo_extra", True), ("disks", True), ("textured", True)])
 def test_oracle_direct_lighting_matches_reference(name, exact):
     """DirectLightingIntegrator films of the reference binary (film_*_dl.npy): bit-identical
     for the libm variant; with Phong the rejection redraws agree statistically only."""
