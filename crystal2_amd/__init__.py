@@ -23,7 +23,7 @@ from . import _abi
 from ._abi import Crystal2Error, check
 
 __all__ = ["Scene", "GpuBVH", "WavefrontPathIntegrator", "RadianceCache", "Config", "Crystal2Error",
-           "halton_index", "halton_value", "load_library", "load_image"]
+           "halton_index", "halton_value", "load_library", "load_image", "write_hdr", "write_pfm"]
 
 
 def load_library():
@@ -36,6 +36,18 @@ def _fp(a):
 
 def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def write_hdr(path: str, film: np.ndarray):
+    """The reference's result.hdr (stbi_write_hdr) for an (h, w, 3) float32 film."""
+    f = np.ascontiguousarray(film, np.float32)
+    check(load_library().c2_film_write_hdr(path.encode(), f.shape[1], f.shape[0], _fp(f)))
+
+
+def write_pfm(path: str, film: np.ndarray):
+    """Lossless little-endian PFM of an (h, w, 3) float32 film."""
+    f = np.ascontiguousarray(film, np.float32)
+    check(load_library().c2_film_write_pfm(path.encode(), f.shape[1], f.shape[0], _fp(f)))
 
 
 def load_image(path: str) -> np.ndarray:
@@ -91,6 +103,17 @@ class Scene:
         h = C.c_void_p()
         check(lib.c2_scene_create(C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def load_binary(cls, path: str) -> "Scene":
+        """A scene written by :meth:`save` (the binary scene cache)."""
+        lib = load_library()
+        h = C.c_void_p()
+        check(lib.c2_scene_load_binary(path.encode(), C.byref(h)))
+        return cls(h)
+
+    def save(self, path: str):
+        check(self._lib.c2_scene_save(self._h, path.encode()))
 
     def add_material(self, mtype: int, color, roughness=(0.0, 0.0), ior=1.0, phong_k=0, ndf=0, ks=(0.0, 0.0, 0.0),
                      sigma=0.0, kd_texture: int = 0) -> int:

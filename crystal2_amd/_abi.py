@@ -138,6 +138,10 @@ SIGNATURES = [
     ("c2_image_free", None, [C.POINTER(C.c_uint8)]),
     ("c2_scene_set_camera", C.c_int, [P, C.POINTER(c2_camera)]),
     ("c2_scene_get_desc", C.c_int, [P, C.POINTER(c2_scene_desc)]),
+    ("c2_scene_save", C.c_int, [P, C.c_char_p]),
+    ("c2_scene_load_binary", C.c_int, [C.c_char_p, C.POINTER(P)]),
+    ("c2_film_write_hdr", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _f32p]),
+    ("c2_film_write_pfm", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _f32p]),
     ("c2_accel_build", C.c_int, [C.POINTER(c2_scene_desc), C.POINTER(P)]),
     ("c2_accel_destroy", None, [P]),
     ("c2_accel_get_stats", C.c_int, [P, C.POINTER(c2_accel_stats)]),

@@ -179,6 +179,19 @@ void c2_image_free(uint8_t* rgb);
 int  c2_scene_set_camera(c2_scene* scene, const c2_camera* camera);
 /* Fills *out with pointers into the scene's arrays (valid until the scene changes). */
 int  c2_scene_get_desc(const c2_scene* scene, c2_scene_desc* out);
+/* Binary scene cache (no counterpart in the reference, whose OBJ parsing is sscanf-bound,
+ * src/Loaders/ObjLoader.cpp:90-149): every array of the scene in one file, reloaded at
+ * file-read speed into a scene equal to the saved one. */
+int  c2_scene_save(const c2_scene* scene, const char* path);
+int  c2_scene_load_binary(const char* path, c2_scene** out);
+
+/* ---------------------------------------------------------------- film output
+ * rgb: width*height*3 floats, rows top-down, as c2_pt_get_film returns them.
+ * c2_film_write_hdr writes the file stbi_write_hdr("result.hdr", w, h, 3, rgb) writes at
+ * the end of the reference's render (src/Core/RayTracer.cpp:40), byte for byte.
+ * c2_film_write_pfm writes the film losslessly as a little-endian PFM. */
+int  c2_film_write_hdr(const char* path, int32_t width, int32_t height, const float* rgb);
+int  c2_film_write_pfm(const char* path, int32_t width, int32_t height, const float* rgb);
 
 /* ---------------------------------------------------------------- Accelerator (GPU BVH)
  * Build: SAH binary BVH (leaf <= 4) collapsed to a 4-wide BVH with 128-byte,

@@ -23,6 +23,9 @@
 //       StartPixel((x,y)); idx = GetVectorIndexForSample(k)
 //       (src/Samplers/HaltonSampler.cpp:251-270); SampleValue(idx, d) for
 //       d < ndims (:272-291). out.bin: n int32 idx, then n x ndims float32.
+//   crystal2_ref writehdr <film.f32> <W> <H> <out.hdr>
+//       stbi_write_hdr(out, W, H, 3, film) as RayTracer::TraceAsync calls it
+//       (src/Core/RayTracer.cpp:40); film.f32 is W*H*3 float32, rows top-down.
 //   crystal2_ref decode <image> <out.bin>
 //       stbi_load(image, 3 components), the call ImageTexels::CreateImageTexels
 //       makes (src/Textures/Texel.cpp:4-13). out.bin: int32 width, int32 height,
@@ -38,6 +41,7 @@
 #include <Samplers/HaltonSampler.h>
 #include <SJson/SJson.h>
 #include <stbi/stb_image.h>
+#include <stbi/stb_image_write.h>
 
 #include <chrono>
 #include <cstdio>
@@ -160,6 +164,18 @@ int cmdHalton(int argc, char** argv) {
     return 0;
 }
 
+int cmdWriteHdr(int argc, char** argv) {
+    if (argc < 6) return 2;
+    int W = std::atoi(argv[3]), H = std::atoi(argv[4]);
+    std::vector<float> film((size_t)W * H * 3);
+    FILE* f = std::fopen(argv[2], "rb");
+    if (!f) return 3;
+    size_t got = std::fread(film.data(), sizeof(float), film.size(), f);
+    std::fclose(f);
+    if (got != film.size()) return 3;
+    return stbi_write_hdr(argv[5], W, H, 3, film.data()) ? 0 : 3;
+}
+
 int cmdDecode(int argc, char** argv) {
     if (argc < 4) return 2;
     int w = 0, h = 0, c = 0;
@@ -179,7 +195,7 @@ int cmdDecode(int argc, char** argv) {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: crystal2_ref render|hits|halton|decode ...\n");
+        std::fprintf(stderr, "usage: crystal2_ref render|hits|halton|decode|writehdr ...\n");
         return 2;
     }
     std::string cmd = argv[1];
@@ -188,6 +204,7 @@ int main(int argc, char** argv) {
     else if (cmd == "hits") rc = cmdHits(argc, argv);
     else if (cmd == "halton") rc = cmdHalton(argc, argv);
     else if (cmd == "decode") rc = cmdDecode(argc, argv);
+    else if (cmd == "writehdr") rc = cmdWriteHdr(argc, argv);
     if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", cmd.c_str());
     std::fflush(stdout);
     // The reference's FixedThreadPool workers are detached and still blocked on

@@ -13,6 +13,7 @@ arg_order="ltr" film.
     python tests/golden/make_golden.py disk     # Disk-shape scene only
     python tests/golden/make_golden.py images   # image-decoder fixtures only
     python tests/golden/make_golden.py textured # textures + skybox scene only
+    python tests/golden/make_golden.py hdr      # stbi_write_hdr outputs only
 """
 import os
 import subprocess
@@ -288,6 +289,32 @@ def textured():
     print("textured-scene films written to", HERE)
 
 
+def hdr_films():
+    """The reference's result.hdr (stbi_write_hdr) for a rendered film, a film narrower than
+    8 pixels (no run-length coding) and one with long runs, zeros and large values."""
+    if not os.path.exists(REF):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    d = os.path.join(HERE, "hdr")
+    os.makedirs(d, exist_ok=True)
+    rng = np.random.default_rng(4)
+    runs = np.zeros((6, 300, 3), np.float32)
+    runs[1] = 0.5
+    runs[2, :150] = rng.uniform(0, 3, (150, 3))
+    runs[3, ::2] = 1e5
+    runs[4] = rng.choice([0.25, 0.25, 0.25, 7.0], (300, 3))
+    runs[5, 100:260] = 2e-33
+    films = {"zoo": np.load(os.path.join(HERE, "film_zoo.npy")),
+             "narrow": rng.uniform(0, 2, (3, 5, 3)).astype(np.float32), "runs": runs}
+    for name, f in films.items():
+        src = os.path.join(d, name + ".f32")
+        np.ascontiguousarray(f, np.float32).tofile(src)
+        subprocess.run([REF, "writehdr", src, str(f.shape[1]), str(f.shape[0]), os.path.join(d, name + ".hdr")],
+                       check=True)
+        np.save(os.path.join(d, name + ".npy"), f)
+        os.remove(src)
+    print("HDR fixtures written to", d)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["direct"]:
         direct_lighting()
@@ -299,5 +326,7 @@ if __name__ == "__main__":
         images()
     elif sys.argv[1:] == ["textured"]:
         textured()
+    elif sys.argv[1:] == ["hdr"]:
+        hdr_films()
     else:
         main()
