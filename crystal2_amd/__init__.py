@@ -221,12 +221,16 @@ class Scene:
 class GpuBVH:
     """Accelerator on the GPU (src/Accelerators/Accelerator.h:8-22): Intersect / IntersectTest."""
 
-    def __init__(self, scene: Scene):
+    ACCELS = {"bvh": _abi.C2_ACCEL_BVH, "BVH": _abi.C2_ACCEL_BVH, "GPUBVH": _abi.C2_ACCEL_BVH,
+              "kdtree": _abi.C2_ACCEL_KDTREE, "KDTree": _abi.C2_ACCEL_KDTREE}
+
+    def __init__(self, scene: Scene, accel: str = "bvh"):
+        """``accel``: "bvh" (the GPU BVH) or "kdtree" (the reference's KD-tree, "KDTree")."""
         self._lib = load_library()
         self._scene = scene
         self._h = C.c_void_p()
         d = scene.desc()
-        check(self._lib.c2_accel_build(C.byref(d), C.byref(self._h)))
+        check(self._lib.c2_accel_create(C.byref(d), self.ACCELS[accel], C.byref(self._h)))
 
     def stats(self) -> dict:
         s = _abi.c2_accel_stats()
@@ -285,6 +289,7 @@ class WavefrontPathIntegrator:
     ``film``: "f32" (reference FrameBuffer), "kahan" or "f64".
     ``integrator``: "path" (PathTracingIntegrator) or "direct" (DirectLightingIntegrator,
     DirectLightingIntegrator.cpp:22-156, on the same kernels).
+    ``accel``: "bvh" or "kdtree" (config "Acceleration": the reference's KD-tree).
     """
 
     FILMS = {"f32": _abi.C2_FILM_F32, "kahan": _abi.C2_FILM_F32_KAHAN, "f64": _abi.C2_FILM_F64}
@@ -293,10 +298,11 @@ class WavefrontPathIntegrator:
 
     def __init__(self, scene: Scene, width: int, height: int, spp: int, max_depth: int, arg_order: str = "rtl",
                  film: str = "f32", paths_in_flight: int = 0, window=None, queue_blocks: int = 0,
-                 integrator: str = "path"):
+                 integrator: str = "path", accel: str = "bvh"):
         self._lib = load_library()
         self.width, self.height, self.spp, self.max_depth = width, height, spp, max_depth
         o = _abi.c2_render_opts()
+        o.accel = GpuBVH.ACCELS[accel]
         o.width, o.height, o.spp, o.max_depth = width, height, spp, max_depth
         o.arg_order = _abi.C2_ARGORDER_LTR if arg_order == "ltr" else _abi.C2_ARGORDER_RTL
         o.film_mode = self.FILMS[film]

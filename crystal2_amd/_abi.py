@@ -19,6 +19,7 @@ C2_LIGHT_AREA, C2_LIGHT_POINT = 0, 1
 C2_ARGORDER_RTL, C2_ARGORDER_LTR = 0, 1
 C2_FILM_F32, C2_FILM_F32_KAHAN, C2_FILM_F64 = 0, 1, 2
 C2_INTEGRATOR_PATH, C2_INTEGRATOR_DIRECT = 0, 1
+C2_ACCEL_BVH, C2_ACCEL_KDTREE = 0, 1
 
 
 class c2_material(C.Structure):
@@ -68,7 +69,7 @@ class c2_render_opts(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("max_depth", C.c_int32),
                 ("arg_order", C.c_int32), ("film_mode", C.c_int32), ("paths_in_flight", C.c_int32),
                 ("queue_blocks", C.c_int32), ("win_x0", C.c_int32), ("win_y0", C.c_int32), ("win_w", C.c_int32),
-                ("win_h", C.c_int32), ("integrator", C.c_int32)]
+                ("win_h", C.c_int32), ("integrator", C.c_int32), ("accel", C.c_int32)]
 
 
 class c2_nrc_opts(C.Structure):
@@ -143,6 +144,7 @@ SIGNATURES = [
     ("c2_film_write_hdr", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _f32p]),
     ("c2_film_write_pfm", C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _f32p]),
     ("c2_accel_build", C.c_int, [C.POINTER(c2_scene_desc), C.POINTER(P)]),
+    ("c2_accel_create", C.c_int, [C.POINTER(c2_scene_desc), C.c_int32, C.POINTER(P)]),
     ("c2_accel_destroy", None, [P]),
     ("c2_accel_get_stats", C.c_int, [P, C.POINTER(c2_accel_stats)]),
     ("c2_accel_intersect", C.c_int, [P, _f32p, C.c_int64, _i32p, _f32p]),

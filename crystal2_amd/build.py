@@ -21,9 +21,9 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("CRYSTAL2_ARCH", "gfx950")
 
-HOST_SOURCES = ["c2_host.cpp", "c2_bvh.cpp", "c2_image.cpp", "c2_film.cpp"]
+HOST_SOURCES = ["c2_host.cpp", "c2_bvh.cpp", "c2_image.cpp", "c2_film.cpp", "c2_kdtree.cpp"]
 HIP_SOURCES = ["c2_device.hip", "c2_nrc.hip"]
-HEADERS = ["c2_host.h", "c2_vec.h", "c2_bvh.h", "c2_device.h", "c2_image.h"]
+HEADERS = ["c2_host.h", "c2_vec.h", "c2_bvh.h", "c2_device.h", "c2_image.h", "c2_kdtree.h"]
 
 CXXFLAGS = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wno-unused-function",
             "-pthread", f"-I{os.path.join(HERE, '..', 'include')}"]

@@ -47,6 +47,9 @@ struct DevScene {
     const int4* tex;           // texture id k: tex[k - 1] = (width, height, first texel, 0)
     const float* lin;          // [256]: (b / 255)^2.2, the texel linearisation
     const float* tri_uv;       // [ntri][6] vertex uv (allocated with textures only)
+    const int4* kd_nodes;      // KD-tree accelerator (null with the BVH): nodes, leaf lists,
+    const int32_t* kd_objs;    // and per-entity boxes (lo, hi)
+    const float4* kd_box;
     int32_t num_lights;
     int32_t num_entities;
 };
@@ -436,6 +439,136 @@ __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, 
             break;
         }
     }
+}
+
+// ------------------------------------------------------------------------- KD-tree
+// RayBoxTest(ray, box, tMin, tMax) (src/Core/Geometry.h:118-133): the swap follows the
+// sign of the direction itself (not of its reciprocal, as in the BVH's form).
+C2_HD bool ray_box_ref(V3 o, V3 d, V3 inv, const float* lo, const float* hi, float& t0, float& t1) {
+    const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z}, ii[3] = {inv.x, inv.y, inv.z};
+    for (int i = 0; i < 3; i++) {
+        float a = (lo[i] - oo[i]) * ii[i], b = (hi[i] - oo[i]) * ii[i];
+        if (dd[i] < 0) {
+            float t = a;
+            a = b;
+            b = t;
+        }
+        t0 = fmax_ref(t0, a);
+        t1 = fmin_ref(t1, b);
+        if (t1 < t0) return false;
+    }
+    return true;
+}
+
+constexpr int kKdStack = 24;  // > the tree's depth limit of 20: at most one entry per level
+
+// KD-tree traversal with the reference's windows (KDTree.cpp:73-81, 195-266), recursion
+// turned into an explicit stack in the same visiting order:
+//  * inner node: t_split = (split - o[axis]) / d[axis]; the near child (below, or above
+//    for a negative direction) gets [tMin, min(t_split, tMax, best)] when t_split >= tMin,
+//    the far child [max(t_split, 0), min(tMax, best after the near child)] when
+//    t_split <= tMax; a node is skipped when tMin > tMax, or when tMin >= best (closest);
+//  * closest-hit leaf: each entity's box is tested against [tMin, min(tMax, best)], then
+//    the full shape intersection replaces the best hit when strictly closer (first found
+//    wins ties); the ray is first clipped to the union box of all entities;
+//  * any-hit leaf: the box test as above, then the shape's own test over the node's
+//    window [tMin, tMax], skipping the ignored entity; the first hit ends the search.
+template <bool ANY>
+__device__ __forceinline__ bool kd_traverse(const DevScene& S, int32_t root, const float* mlo, const float* mhi,
+                                            V3 o, V3 d, float tMin, float tMax, int ignore, Hit& hit) {
+    hit.entity = -1;
+    if (root <= 0) return false;
+    const V3 inv{1.f / d.x, 1.f / d.y, 1.f / d.z};
+    if (!ANY && !ray_box_ref(o, d, inv, mlo, mhi, tMin, tMax)) return false;
+    float best = INFINITY, bestU = 0.f, bestV = 0.f;
+    int32_t bestE = -1;
+    int32_t sn[kKdStack];
+    float s0[kKdStack], s1[kKdStack];
+    int sp = 0;
+    int32_t p = root;
+    float t0 = tMin, t1 = tMax;
+    while (true) {
+        if (p != 0 && !(t0 > t1) && (ANY || !(t0 >= best))) {
+            const int4 nd = S.kd_nodes[p];
+            if (nd.x < 0) {
+                for (int k = 0; k < nd.w; k++) {
+                    const int32_t e = S.kd_objs[nd.z + k];
+                    const float4 blo = S.kd_box[2 * e], bhi = S.kd_box[2 * e + 1];
+                    const float lo[3] = {blo.x, blo.y, blo.z}, hi[3] = {bhi.x, bhi.y, bhi.z};
+                    float a = t0, b = ANY ? t1 : fmin_ref(t1, best);
+                    if (!ray_box_ref(o, d, inv, lo, hi, a, b)) continue;
+                    if (ANY && e == ignore) continue;
+                    const int4 en = S.entities[e];
+                    if (en.x == C2_SHAPE_TRIANGLE) {
+                        const float* tp = S.tri_pos + (size_t)en.y * 9;
+                        V3 v0{tp[0], tp[1], tp[2]}, v1{tp[3], tp[4], tp[5]}, v2{tp[6], tp[7], tp[8]};
+                        if (ANY) {
+                            if (tri_test(o, d, v0, v1, v2, t0, t1)) {
+                                hit.entity = e;
+                                return true;
+                            }
+                        } else {
+                            V3 res = tri_solve(o, d, v0, v1, v2);
+                            if (tri_accept_closest(res) && res.z < best) {
+                                best = res.z;
+                                bestE = e;
+                                bestU = res.x;
+                                bestV = res.y;
+                            }
+                        }
+                    } else {
+                        const float* sp3 = S.spheres + (size_t)en.y * 22;
+                        const bool disk = en.x == C2_SHAPE_DISK;
+                        if (ANY) {
+                            if (disk ? disk_test(sp3, o, d, t0, t1) : sphere_test(sp3, o, d, t0, t1)) {
+                                hit.entity = e;
+                                return true;
+                            }
+                        } else {
+                            float dis;
+                            bool h = disk ? disk_closest(sp3, o, d, dis) : sphere_closest(sp3, o, d, dis);
+                            if (h && dis < best) {
+                                best = dis;
+                                bestE = e;
+                                bestU = 0.f;
+                                bestV = 0.f;
+                            }
+                        }
+                    }
+                }
+            } else {
+                const int axis = nd.x;
+                const float split = __int_as_float(nd.y);
+                const float oa = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
+                const float da = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+                const float ts = (split - oa) / da;
+                const bool neg = da < 0;
+                const int32_t nearC = neg ? nd.w : nd.z, farC = neg ? nd.z : nd.w;
+                if (ts <= t1 && sp < kKdStack) {  // visited after the near subtree
+                    sn[sp] = farC;
+                    s0[sp] = fmax_ref(ts, 0.f);
+                    s1[sp] = t1;
+                    sp++;
+                }
+                if (ts >= t0) {
+                    p = nearC;
+                    t1 = ANY ? fmin_ref(ts, t1) : fmin_ref(fmin_ref(ts, t1), best);
+                    continue;
+                }
+            }
+        }
+        if (sp == 0) break;
+        sp--;
+        p = sn[sp];
+        t0 = s0[sp];
+        t1 = ANY ? s1[sp] : fmin_ref(s1[sp], best);
+    }
+    if (ANY) return false;
+    hit.entity = bestE;
+    hit.t = best;
+    hit.u = bestU;
+    hit.v = bestV;
+    return bestE >= 0;
 }
 
 }  // namespace c2

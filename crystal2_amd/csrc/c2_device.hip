@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "c2_bvh.h"
+#include "c2_kdtree.h"
 #include "c2_device.h"
 #include "c2_host.h"
 #include "c2_shade.h"
@@ -58,28 +59,45 @@ constexpr int kQueueBlocksDefault = 8192;
 
 // ============================================================================ accelerator kernels
 
+// Master box of a KD-tree scene (KDTree::Intersect clips the ray to it first)
+struct KdMaster {
+    float lo[3], hi[3];
+    int32_t root;
+};
+
+template <bool KD>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, const float* __restrict__ rays, int64_t n,
-                                                      int32_t* __restrict__ prim, float* __restrict__ tout) {
+                                                      int32_t* __restrict__ prim, float* __restrict__ tout, KdMaster M) {
     __shared__ TravLds<false> lds;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
         float4 a = r4[0], b = r4[1];
         Hit h;
-        bool hit = traverse<false>(S, 0, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h, lds);
+        bool hit;
+        if constexpr (KD)
+            hit = kd_traverse<false>(S, M.root, M.lo, M.hi, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h);
+        else
+            hit = traverse<false>(S, 0, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h, lds);
         prim[i] = hit ? h.entity : -1;
         tout[i] = hit ? h.t : INFINITY;
     }
 }
 
+template <bool KD>
 __global__ void __launch_bounds__(kBlock) k_intersect_test(DevScene S, const float* __restrict__ rays,
                                                            const int32_t* __restrict__ ignore, int64_t n,
-                                                           uint8_t* __restrict__ out) {
+                                                           uint8_t* __restrict__ out, KdMaster M) {
     __shared__ TravLds<true> lds;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
         float4 a = r4[0], b = r4[1];
         Hit h;
-        bool hit = traverse<true>(S, 0, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ignore ? ignore[i] : -1, h, lds);
+        const int ig = ignore ? ignore[i] : -1;
+        bool hit;
+        if constexpr (KD)
+            hit = kd_traverse<true>(S, M.root, M.lo, M.hi, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ig, h);
+        else
+            hit = traverse<true>(S, 0, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ig, h, lds);
         out[i] = hit ? 1 : 0;
     }
 }
@@ -149,6 +167,7 @@ struct View {
     V3 eye, look, up, right;
     float camW, camH;
     int32_t sky[6];          // skybox face texture ids (posx, negx, posy, negy, posz, negz), 0 = none
+    float kd_lo[3], kd_hi[3];  // KD-tree scenes: union box of the entities (root is the KD root)
 };
 
 struct PassParams {
@@ -336,6 +355,7 @@ __global__ void k_reset(PathBuffers B, int next_q) {
     B.counters[3] = 0;
 }
 
+template <bool KD>
 __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, PassParams P, int q) {
     __shared__ TravLds<false> lds;
     int n = B.counters[q];
@@ -344,9 +364,14 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, Pa
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int p = queue[i];
         float4 o = B.ray_o[p], d = B.ray_d[p];
-        const int root = P.views[B.vid[p]].root;
+        const View& V = P.views[B.vid[p]];
         Hit h;
-        bool hit = traverse<false>(S, root, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY, -1, h, lds);
+        bool hit;
+        if constexpr (KD)
+            hit = kd_traverse<false>(S, V.root, V.kd_lo, V.kd_hi, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY,
+                                     -1, h);
+        else
+            hit = traverse<false>(S, V.root, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY, -1, h, lds);
         B.hit[p] = make_float4(h.t, h.u, h.v, __int_as_float(hit ? h.entity : -1));
     }
 }
@@ -691,6 +716,7 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
     }
 }
 
+template <bool KD>
 __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
     __shared__ TravLds<true> lds;
     int n = B.counters[2];
@@ -699,14 +725,20 @@ __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
         int s = B.sh_list[i];
         float4 o = B.sh_o[s], d = B.sh_d[s];
         Hit h;
-        bool occluded = traverse<true>(S, __float_as_int(d.w), V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f,
-                                       __float_as_int(o.w), h, lds);
+        bool occluded;
+        if constexpr (KD)
+            occluded = kd_traverse<true>(S, __float_as_int(d.w), nullptr, nullptr, V3{o.x, o.y, o.z},
+                                         V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f, __float_as_int(o.w), h);
+        else
+            occluded = traverse<true>(S, __float_as_int(d.w), V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f,
+                                      __float_as_int(o.w), h, lds);
         B.sh_vis[s] = occluded ? 0 : 1;
     }
 }
 
 // The shared MIS ray: Scene::Intersect(isec.SpawnRay(wi)); only the light owning the
 // hit entity can receive f * Le * w / pdf_bsdf, :255-273
+template <bool KD>
 __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassParams P) {
     __shared__ TravLds<false> lds;
     int n = B.counters[3];
@@ -721,7 +753,13 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassP
         Hit h;
         int lh = -1;
         V3 Lb{0, 0, 0};
-        if (traverse<false>(S, P.views[B.vid[p]].root, o, wi, 0.f, INFINITY, -1, h, lds)) {
+        const View& V = P.views[B.vid[p]];
+        bool found;
+        if constexpr (KD)
+            found = kd_traverse<false>(S, V.root, V.kd_lo, V.kd_hi, o, wi, 0.f, INFINITY, -1, h);
+        else
+            found = traverse<false>(S, V.root, o, wi, 0.f, INFINITY, -1, h, lds);
+        if (found) {
             int4 e = S.entities[h.entity];
             int li = e.w;
             if (li >= 0 && (P.integrator == C2_INTEGRATOR_DIRECT || li != own)) {
@@ -933,6 +971,12 @@ struct SceneGPU {
     std::vector<int4> tex;
     std::vector<float> tri_uv;
     std::vector<std::array<int32_t, 6>> sky;  // per scene: skybox face ids in the merged table
+    // KD-tree accelerator (built instead of being traversed by the BVH when kd is set)
+    bool kd = false;
+    std::vector<int4> kd_nodes{make_int4(-1, 0, 0, 0)};  // node 0: "none"
+    std::vector<int32_t> kd_objs;
+    std::vector<float4> kd_box;
+    std::vector<KdMaster> kdm;  // per scene: root and union box
     double build_seconds = 0;
 
     // Start of each scene in the concatenated arrays (a single scene has offsets 0).
@@ -972,6 +1016,11 @@ struct SceneGPU {
         dev.lights = arena.upload(lights);
         dev.perm = arena.upload(halton_perm_data());
         dev.perm_off = arena.upload(halton_perm_offsets());
+        if (kd) {
+            dev.kd_nodes = arena.upload(kd_nodes);
+            dev.kd_objs = arena.upload(kd_objs.empty() ? std::vector<int32_t>{0} : kd_objs);
+            dev.kd_box = arena.upload(kd_box.empty() ? std::vector<float4>{make_float4(0, 0, 0, 0)} : kd_box);
+        }
         if (!tex.empty()) {
             dev.texels = arena.upload(texels);
             dev.tex = arena.upload(tex);
@@ -1041,6 +1090,33 @@ struct SceneGPU {
                     b.hi[c] = q[c] + q[3];
                 }
             }
+        }
+        if (kd) {
+            KdTree kt;
+            auto k0 = std::chrono::steady_clock::now();
+            build_kdtree(boxes, kt);
+            build_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - k0).count();
+            if (kt.depth > 20) fail(C2_ERR_INVALID, "KD-tree deeper than its limit");
+            const int32_t nb = (int32_t)kd_nodes.size() - 1, ob = (int32_t)kd_objs.size();  // node 0 is shared
+            for (size_t i = 1; i < kt.nodes.size(); i++) {
+                const KdNode& nd = kt.nodes[i];
+                int32_t sb;
+                std::memcpy(&sb, &nd.split, 4);
+                if (nd.axis < 0) kd_nodes.push_back(make_int4(-1, 0, nd.a + ob, nd.b));
+                else kd_nodes.push_back(make_int4(nd.axis, sb, nd.a ? nd.a + nb : 0, nd.b ? nd.b + nb : 0));
+            }
+            for (int32_t e : kt.objs) kd_objs.push_back(e + o.ent);
+            for (const Box3& b : boxes) {
+                kd_box.push_back(make_float4(b.lo[0], b.lo[1], b.lo[2], 0.f));
+                kd_box.push_back(make_float4(b.hi[0], b.hi[1], b.hi[2], 0.f));
+            }
+            KdMaster m;
+            for (int c = 0; c < 3; c++) {
+                m.lo[c] = kt.master.lo[c];
+                m.hi[c] = kt.master.hi[c];
+            }
+            m.root = kt.root ? kt.root + nb : 0;
+            kdm.push_back(m);
         }
         BuiltBvh local;
         BuiltBvh& bv = keep_bvh ? bvh : local;
@@ -1218,10 +1294,14 @@ static int grid_for(int64_t n) {
 
 extern "C" {
 
-int c2_accel_build(const c2_scene_desc* desc, c2_accel** out) {
+int c2_accel_build(const c2_scene_desc* desc, c2_accel** out) { return c2_accel_create(desc, C2_ACCEL_BVH, out); }
+
+int c2_accel_create(const c2_scene_desc* desc, int32_t accel, c2_accel** out) {
     return guard([&] {
         if (!out) fail(C2_ERR_INVALID, "null output");
+        if (accel != C2_ACCEL_BVH && accel != C2_ACCEL_KDTREE) fail(C2_ERR_INVALID, "unknown accelerator kind");
         std::unique_ptr<c2_accel> a(new c2_accel());
+        a->scene.kd = accel == C2_ACCEL_KDTREE;
         a->scene.build(desc);
         a->stats.num_nodes_bvh2 = (int64_t)a->scene.bvh.bvh2.size();
         a->stats.num_nodes_bvh4 = (int64_t)a->scene.nodes.size();
@@ -1251,8 +1331,12 @@ int c2_accel_intersect_device(c2_accel* a, const float* rays, int64_t n, int32_t
             }
             return;
         }
-        hipLaunchKernelGGL(k_intersect, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, a->scene.dev, rays,
-                           n, prim, t);
+        if (a->scene.kd)
+            hipLaunchKernelGGL(k_intersect<true>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                               a->scene.dev, rays, n, prim, t, a->scene.kdm[0]);
+        else
+            hipLaunchKernelGGL(k_intersect<false>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                               a->scene.dev, rays, n, prim, t, KdMaster{});
         HIP_CHECK(hipGetLastError());
     });
 }
@@ -1266,8 +1350,12 @@ int c2_accel_intersect_test_device(c2_accel* a, const float* rays, const int32_t
             HIP_CHECK(hipMemsetAsync(hit, 0, n, (hipStream_t)stream));
             return;
         }
-        hipLaunchKernelGGL(k_intersect_test, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, a->scene.dev,
-                           rays, ignore, n, hit);
+        if (a->scene.kd)
+            hipLaunchKernelGGL(k_intersect_test<true>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                               a->scene.dev, rays, ignore, n, hit, a->scene.kdm[0]);
+        else
+            hipLaunchKernelGGL(k_intersect_test<false>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                               a->scene.dev, rays, ignore, n, hit, KdMaster{});
         HIP_CHECK(hipGetLastError());
     });
 }
@@ -1405,6 +1493,8 @@ void check_opts(const c2_render_opts* opts) {
         fail(C2_ERR_INVALID, "render options: negative queue_blocks / paths_in_flight");
     if (opts->integrator != C2_INTEGRATOR_PATH && opts->integrator != C2_INTEGRATOR_DIRECT)
         fail(C2_ERR_INVALID, "render options: bad integrator");
+    if (opts->accel != C2_ACCEL_BVH && opts->accel != C2_ACCEL_KDTREE)
+        fail(C2_ERR_INVALID, "render options: bad accel");
 }
 
 // Builds a renderer over n scenes (one view each) sharing one concatenated scene,
@@ -1422,6 +1512,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     pt->opts = *opts;
     std::unique_ptr<c2_accel> acc(new c2_accel());
     acc->scene.leaf_size = kLeafSizeRender;
+    acc->scene.kd = opts->accel == C2_ACCEL_KDTREE;
     acc->scene.build(descs, n);
     pt->accel = std::move(acc);
     const SceneGPU& SG = pt->accel->scene;
@@ -1443,6 +1534,13 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
         V.width = v.width;
         V.height = v.height;
         V.root = descs[k]->num_entities > 0 ? SG.offs[k].node : -1;
+        if (SG.kd) {
+            V.root = SG.kdm[k].root;
+            for (int c = 0; c < 3; c++) {
+                V.kd_lo[c] = SG.kdm[k].lo[c];
+                V.kd_hi[c] = SG.kdm[k].hi[c];
+            }
+        }
         // camera (PerspCamera ctor: tan evaluated on the host with the float libm call)
         const c2_camera& c = descs[k]->camera;
         float aspect = (float)v.width / v.height;
@@ -1666,6 +1764,7 @@ namespace {
 void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipStream_t s, bool per_sample_passes) {
         PassParams P = pt->pp;
         DevScene S = pt->accel->scene.dev;
+        const bool SGkd = pt->accel->scene.kd;
         for (int k = sample_begin; k < sample_begin + sample_count; k++) {
             const c2_pt::PassTab& tab = pt->passes[per_sample_passes ? k : 0];
             const int64_t npix = tab.npix;
@@ -1701,7 +1800,10 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         }
                         {
                             KTimer kt(pt, C2_K_EXTEND, s);
-                            hipLaunchKernelGGL(k_extend, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
+                            if (SGkd)
+                                hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
+                            else
+                                hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
                         }
                         if (S.tex) {
                             KTimer kt(pt, C2_K_SHADE, s);
@@ -1738,14 +1840,21 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         HIP_CHECK(hipStreamWaitEvent(pt->side, pt->ev_fork, 0));
                         {
                             KTimer kt(pt, C2_K_MIS, pt->side);
-                            hipLaunchKernelGGL(k_mis, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, pt->side, S,
-                                               pt->B, P);
+                            if (SGkd)
+                                hipLaunchKernelGGL(k_mis<true>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0,
+                                                   pt->side, S, pt->B, P);
+                            else
+                                hipLaunchKernelGGL(k_mis<false>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0,
+                                                   pt->side, S, pt->B, P);
                         }
                         HIP_CHECK(hipEventRecord(pt->ev_join, pt->side));
                         int gs = std::min(grid_for((int64_t)P.pix_count * P.lmax), qcap);
                         {
                             KTimer kt(pt, C2_K_OCCLUDE, s);
-                            hipLaunchKernelGGL(k_occlude, dim3(gs), dim3(kBlock), 0, s, S, pt->B);
+                            if (SGkd)
+                                hipLaunchKernelGGL(k_occlude<true>, dim3(gs), dim3(kBlock), 0, s, S, pt->B);
+                            else
+                                hipLaunchKernelGGL(k_occlude<false>, dim3(gs), dim3(kBlock), 0, s, S, pt->B);
                         }
                         HIP_CHECK(hipStreamWaitEvent(s, pt->ev_join, 0));
                         {

@@ -65,6 +65,10 @@ enum {
 };
 enum { C2_NDF_GGX = 0, C2_NDF_BECKMANN = 1 };
 enum { C2_LIGHT_AREA = 0, C2_LIGHT_POINT = 1 };
+/* Accelerator::GetAccelerator names (src/Accelerators/Accelerator.cpp:8-23): "BVH" and
+ * "GPUBVH" run the GPU BVH; "KDTree" runs the reference's KD-tree (the same tree, traversed
+ * on the GPU with the reference's windows, so results match it rather than the BVH). */
+enum { C2_ACCEL_BVH = 0, C2_ACCEL_KDTREE = 1 };
 
 typedef struct c2_material {
     int32_t type;          /* C2_MAT_*                                      */
@@ -216,7 +220,10 @@ typedef struct c2_accel_stats {
     int64_t device_bytes;         /* node + primitive bytes resident in HBM     */
 } c2_accel_stats;
 
-int  c2_accel_build(const c2_scene_desc* scene, c2_accel** out);
+int  c2_accel_build(const c2_scene_desc* scene, c2_accel** out);  /* c2_accel_create(scene, C2_ACCEL_BVH) */
+/* KDTree::Build / Intersect / IntersectTest (src/Accelerators/KDTree.cpp:59-266) when accel
+ * is C2_ACCEL_KDTREE. */
+int  c2_accel_create(const c2_scene_desc* scene, int32_t accel, c2_accel** out);
 void c2_accel_destroy(c2_accel* accel);
 int  c2_accel_get_stats(const c2_accel* accel, c2_accel_stats* out);
 int  c2_accel_intersect(c2_accel* accel, const float* rays, int64_t n,
@@ -262,6 +269,7 @@ typedef struct c2_render_opts {
     int32_t win_w, win_h;     /* win_w = 0: the whole film                             */
     int32_t integrator;       /* C2_INTEGRATOR_*: "PathTracer" or "DirectLight"
                                  (src/Integrators/Integrator.cpp:27-40)               */
+    int32_t accel;            /* C2_ACCEL_*: config "Acceleration" (0 = the BVH)         */
 } c2_render_opts;
 
 typedef struct c2_pt_stats {

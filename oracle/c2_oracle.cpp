@@ -178,6 +178,21 @@ struct Box {
         return axis;
     }
 };
+// RayBoxTest(ray, box, tMin, tMax), src/Core/Geometry.h:118-133: swaps by the sign of the
+// direction (the KD-tree's form)
+inline bool rayBoxDir(const Ray& r, const Box& b, float& tMin, float& tMax) {
+    vec3 invD(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    bool inv[3] = {r.d.x < 0, r.d.y < 0, r.d.z < 0};
+    vec3 minP = (b.lo - r.o) * invD;
+    vec3 maxP = (b.hi - r.o) * invD;
+    for (int i = 0; i < 3; i++) {
+        if (inv[i]) std::swap(minP[i], maxP[i]);
+        tMin = smax(tMin, minP[i]);
+        tMax = smin(tMax, maxP[i]);
+        if (tMax < tMin) return false;
+    }
+    return true;
+}
 // RayBoxTest with precomputed inverse, src/Core/Geometry.h:136-157
 inline bool rayBox(const Ray& r, const bool inv[3], vec3 invD, const Box& b, float& tMin, float& tMax) {
     vec3 minP = (b.lo - r.o) * invD;
@@ -1038,10 +1053,153 @@ struct orc_scene {
     std::vector<Node> nodes;
     std::vector<int> order;
     int root = 0, tot = 0;
+    // KDTree, src/Accelerators/KDTree.cpp:8-266 (node 0 = none)
+    struct KdNode {
+        std::vector<int> objs;
+        int ch[2] = {0, 0};
+        int axis = -1;
+        float split = -1;
+    };
+    std::vector<KdNode> kd;
+    Box kdMaster;
+    int kdRoot = 0;
 
     Box entityBox(int e) const { return kind[e] == SH_TRI ? tris[shape[e]].box : spheres[shape[e]].box(); }
 
+    static float surfaceArea(const Box& b) {  // BoundingBox::SurfaceArea, Geometry.h:101-109
+        float area = 0;
+        for (int i = 0; i < 3; i++) area += 2 * (b.hi[i] - b.lo[i]) * (b.hi[(i + 1) % 3] - b.lo[(i + 1) % 3]);
+        return area;
+    }
+    // KDTree::_build, KDTree.cpp:97-193
+    int kdBuild(const Box& outer, std::vector<int>& objs, int depth) {
+        if (objs.size() <= 2 || depth == 20) {
+            kd.push_back(KdNode());
+            kd.back().objs = objs;
+            return (int)kd.size() - 1;
+        }
+        int split = 0;
+        float splitPos = 0;
+        float smallestCost = INFINITY;
+        float totalSA = surfaceArea(outer);
+        vec3 diff = outer.hi - outer.lo;
+        struct Edge {
+            bool end;
+            float pos;
+            bool operator<(const Edge& e) const { return pos == e.pos ? end > e.end : pos < e.pos; }
+        };
+        for (int d = 0; d < 3; d++) {
+            std::vector<Edge> edges;
+            for (int o : objs) {
+                edges.push_back(Edge{false, entityBox(o).lo[d]});
+                edges.push_back(Edge{true, entityBox(o).hi[d]});
+            }
+            int nBelow = 0, nAbove = (int)objs.size();
+            float L = outer.lo[d], R = outer.hi[d];
+            std::sort(edges.begin(), edges.end());
+            for (const Edge& e : edges) {
+                if (e.end) --nAbove;
+                if (e.pos > L && e.pos < R) {
+                    float LA = diff[(d + 1) % 3];
+                    float LB = diff[(d + 2) % 3];
+                    float belowSA = 2 * (LA * LB + (e.pos - L) * (LA + LB));
+                    float aboveSA = 2 * (LA * LB + (R - e.pos) * (LA + LB));
+                    float eb = (nAbove == 0 || nBelow == 0) ? 0.2 : 0;
+                    float pb = belowSA / totalSA, pa = aboveSA / totalSA;
+                    float cost = 0.05 + (1 - eb) * (pb * nBelow + pa * nAbove);
+                    if (cost < smallestCost) {
+                        smallestCost = cost;
+                        split = d;
+                        splitPos = e.pos;
+                    }
+                }
+                if (!e.end) ++nBelow;
+            }
+        }
+        if (smallestCost >= objs.size()) {
+            kd.push_back(KdNode());
+            kd.back().objs = objs;
+            return (int)kd.size() - 1;
+        }
+        Box left = outer, right = outer;
+        left.hi[split] = splitPos;
+        right.lo[split] = splitPos;
+        std::vector<int> lo, hi;
+        for (int o : objs) {
+            if (entityBox(o).lo[split] <= splitPos) lo.push_back(o);
+            if (entityBox(o).hi[split] > splitPos) hi.push_back(o);
+        }
+        kd.push_back(KdNode());
+        int self = (int)kd.size() - 1;
+        kd[self].axis = split;
+        kd[self].split = splitPos;
+        int a = kdBuild(left, lo, depth + 1);
+        int b = kdBuild(right, hi, depth + 1);
+        kd[self].ch[0] = a;
+        kd[self].ch[1] = b;
+        return self;
+    }
+    // KDTree::ray_test, KDTree.cpp:195-238
+    bool kdRayTest(int p, const Ray& ray, Isec* info, float tMin, float tMax) const {
+        if (!p || tMin > tMax) return false;
+        if (tMin >= info->t) return false;
+        const KdNode& nd = kd[p];
+        if (nd.axis == -1) {
+            bool hit = false;
+            for (int o : nd.objs) {
+                float t1 = tMin, t2 = smin(tMax, info->t);
+                if (!rayBoxDir(ray, entityBox(o), t1, t2)) continue;
+                Isec tmp;
+                if (entityIntersect(o, ray, &tmp)) {
+                    if (tmp.t < info->t) {
+                        tmp.entity = o;
+                        *info = tmp;
+                    }
+                    hit = true;
+                }
+            }
+            return hit;
+        }
+        float tSplit = (nd.split - ray.o[nd.axis]) / ray.d[nd.axis];
+        bool hit = false;
+        int d = ray.d[nd.axis] < 0;
+        if (tSplit >= tMin) hit |= kdRayTest(nd.ch[d], ray, info, tMin, smin(smin(tSplit, tMax), info->t));
+        if (tSplit <= tMax) hit |= kdRayTest(nd.ch[!d], ray, info, smax(tSplit, 0.0f), smin(tMax, info->t));
+        return hit;
+    }
+    // KDTree::ray_test_p, KDTree.cpp:240-266
+    bool kdRayTestP(int p, const Ray& ray, int ignore, float tMin, float tMax) const {
+        if (!p || tMin > tMax) return false;
+        const KdNode& nd = kd[p];
+        if (nd.axis == -1) {
+            for (int o : nd.objs) {
+                float t1 = tMin, t2 = tMax;
+                if (!rayBoxDir(ray, entityBox(o), t1, t2)) continue;
+                if (o != ignore && entityTest(o, ray, tMin, tMax)) return true;
+            }
+            return false;
+        }
+        float tSplit = (nd.split - ray.o[nd.axis]) / ray.d[nd.axis];
+        bool hit = false;
+        int d = ray.d[nd.axis] < 0;
+        if (tSplit >= tMin) hit |= kdRayTestP(nd.ch[d], ray, ignore, tMin, smin(tSplit, tMax));
+        if (hit) return true;
+        if (tSplit <= tMax) hit |= kdRayTestP(nd.ch[!d], ray, ignore, smax(tSplit, 0.0f), tMax);
+        return hit;
+    }
+
     void build() {
+        if (accel == ORC_ACCEL_KDTREE) {  // KDTree::Build, KDTree.cpp:59-71
+            kd.assign(1, KdNode());
+            kdMaster = Box();
+            std::vector<int> all;
+            for (int e = 0; e < (int)kind.size(); e++) {
+                kdMaster = kdMaster.unite(entityBox(e));
+                all.push_back(e);
+            }
+            kdRoot = kdBuild(kdMaster, all, 0);
+            return;
+        }
         int n = (int)kind.size();
         order.resize(n);
         for (int i = 0; i < n; i++) order[i] = i;
@@ -1094,6 +1252,11 @@ struct orc_scene {
     // BVH::Intersect, BVH.cpp:56-108 / Brute::Intersect, Brute.cpp:19-40
     bool intersect(const Ray& ray, Isec* info) const {
         bool hit = false;
+        if (accel == ORC_ACCEL_KDTREE) {  // KDTree::Intersect, KDTree.cpp:73-77
+            float tMin = 0, tMax = INFINITY;
+            if (!rayBoxDir(ray, kdMaster, tMin, tMax)) return false;
+            return kdRayTest(kdRoot, ray, info, tMin, tMax);
+        }
         if (accel == ORC_ACCEL_BRUTE) {
             for (int e = 0; e < (int)kind.size(); e++) {
                 Isec tmp;
@@ -1149,6 +1312,7 @@ struct orc_scene {
     // BVH::IntersectTest, BVH.cpp:110-154 (ignore: entity whose shape is skipped,
     // GeometricEntity::IntersectTest, src/Core/Entities.cpp:26-31)
     bool intersectTest(const Ray& ray, int ignore, float tMin, float tMax) const {
+        if (accel == ORC_ACCEL_KDTREE) return kdRayTestP(kdRoot, ray, ignore, tMin, tMax);  // KDTree.cpp:79-82
         if (accel == ORC_ACCEL_BRUTE) {
             for (int e = 0; e < (int)kind.size(); e++) {
                 if (e == ignore) continue;

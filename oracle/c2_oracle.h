@@ -27,7 +27,7 @@ extern "C" {
 
 typedef struct orc_scene orc_scene;
 
-enum { ORC_ACCEL_BVH = 0, ORC_ACCEL_BRUTE = 1 };
+enum { ORC_ACCEL_BVH = 0, ORC_ACCEL_BRUTE = 1, ORC_ACCEL_KDTREE = 2 };
 /* Closest-hit tie rule: ORC_TIE_FIRST = the reference's "first found in
  * traversal order wins" (strict <); ORC_TIE_LOWEST = among equal t, the lowest
  * entity index wins (Brute's rule, src/Accelerators/Brute.cpp:19-40). */

@@ -15,7 +15,7 @@ LIBS = {"portable": os.path.join(HERE, "_build", "libc2oracle.so"),
         "libm": os.path.join(HERE, "_build", "libc2oracle_libm.so")}
 REF_BIN = os.path.join(HERE, "_ref", "crystal2_ref")
 
-ACCEL_BVH, ACCEL_BRUTE = 0, 1
+ACCEL_BVH, ACCEL_BRUTE, ACCEL_KDTREE = 0, 1, 2
 TIE_FIRST, TIE_LOWEST = 0, 1
 INTEGRATORS = {"path": 0, "direct": 1}  # PathTracingIntegrator, DirectLightingIntegrator
 

@@ -14,6 +14,7 @@ arg_order="ltr" film.
     python tests/golden/make_golden.py images   # image-decoder fixtures only
     python tests/golden/make_golden.py textured # textures + skybox scene only
     python tests/golden/make_golden.py hdr      # stbi_write_hdr outputs only
+    python tests/golden/make_golden.py kdtree   # "Acceleration": "KDTree" films and hits only
 """
 import os
 import subprocess
@@ -315,6 +316,46 @@ def hdr_films():
     print("HDR fixtures written to", d)
 
 
+def kdtree():
+    """"Acceleration": "KDTree": the reference's films (both integrators) and closest / any
+    hits through its KD-tree, for scenes whose fixtures exist already."""
+    import json
+    if not os.path.exists(REF):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    for name in ("cornell", "zoo_nophong", "disks", "textured"):
+        d = os.path.join(SCN, name)
+        for src, dst, tag in (("config.json", "config_kd.json", "_kd"), ("config_dl.json", "config_dl_kd.json",
+                                                                       "_dl_kd")):
+            c = json.load(open(os.path.join(d, src)))
+            c["Acceleration"] = "KDTree"
+            with open(os.path.join(d, dst), "w") as fh:
+                json.dump(c, fh, indent=1)
+            w, h = c["Width"], c["Height"]
+            film(name, w, h, config=dst, tag=tag)
+    for name in ("zoo", "disks"):
+        g = np.load(os.path.join(HERE, f"hits_{name}.npz"))
+        d = os.path.join(SCN, name)
+        c = json.load(open(os.path.join(d, "config.json")))
+        c["Acceleration"] = "KDTree"
+        with open(os.path.join(d, "config_kd.json"), "w") as fh:
+            json.dump(c, fh, indent=1)
+        rays = g["rays"]
+        rp = os.path.join(d, "rays.bin")
+        with open(rp, "wb") as fh:
+            np.array([len(rays)], np.int64).tofile(fh)
+            rays.astype(np.float32).tofile(fh)
+        ref("hits", "config_kd.json", "scene.json", "rays.bin", "hits.bin", cwd=d)
+        n = len(rays)
+        raw = open(os.path.join(d, "hits.bin"), "rb").read()
+        ids = np.frombuffer(raw, np.int32, n, 0)
+        t = np.frombuffer(raw, np.float32, n, 4 * n)
+        anyh = np.frombuffer(raw, np.uint8, n, 4 * n * 8)
+        os.remove(rp)
+        os.remove(os.path.join(d, "hits.bin"))
+        np.savez_compressed(os.path.join(HERE, f"hits_{name}_kd.npz"), rays=rays, ids=ids, t=t, any=anyh)
+    print("KD-tree fixtures written to", HERE)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["direct"]:
         direct_lighting()
@@ -328,5 +369,7 @@ if __name__ == "__main__":
         textured()
     elif sys.argv[1:] == ["hdr"]:
         hdr_films()
+    elif sys.argv[1:] == ["kdtree"]:
+        kdtree()
     else:
         main()

@@ -209,3 +209,38 @@ def test_direct_lighting_vs_reference_golden(gpu_available, name, spp, tol):
 def test_direct_lighting_depth_zero_is_black(gpu_available):
     g = gpu_film(load("cornell"), 32, 32, 2, 0, integrator="direct")
     assert not g[np.isfinite(g)].any()
+
+
+@pytest.mark.parametrize("name", ["zoo", "disks"])
+def test_kdtree_hits_vs_reference_golden(gpu_available, name):
+    """The KD-tree accelerator on the GPU against the reference's KD-tree hits."""
+    print(f"case kdtree hits {name}", flush=True)
+    g = np.load(os.path.join(GOLDEN, f"hits_{name}_kd.npz"))
+    acc = c2.GpuBVH(load(name), accel="kdtree")
+    closest = g["rays"].copy()
+    closest[:, 7] = np.inf
+    ids, t = acc.intersect(closest)
+    assert np.array_equal(ids, g["ids"])
+    assert np.array_equal(t[ids >= 0], g["t"][ids >= 0])
+    assert np.array_equal(acc.intersect_test(g["rays"]), g["any"].astype(bool))
+
+
+@pytest.mark.parametrize("name,spp,depth", [("cornell", 4, 5), ("zoo", 8, 6), ("disks", 8, 6), ("textured", 8, 6)])
+@pytest.mark.parametrize("integ", ["path", "direct"])
+def test_kdtree_film_bit_identical_to_oracle(gpu_available, name, spp, depth, integ):
+    print(f"case kdtree film {name} {integ}", flush=True)
+    sc = load(name)
+    g = gpu_film(sc, 64, 64, spp, depth, "ltr", integrator=integ, accel="kdtree")
+    o, _ = pyoracle.OracleScene(sc, accel=pyoracle.ACCEL_KDTREE, integrator=integ).render(64, 64, spp, depth,
+                                                                                         arg_order="ltr")
+    assert (g == o).all(-1).mean() >= 0.999
+    assert relmse(g, o) <= 1e-6
+
+
+@pytest.mark.parametrize("name,spp,depth,integ", [("cornell", 4, 5, "path"), ("disks", 8, 6, "path"),
+                                                  ("textured", 8, 6, "direct"), ("zoo_nophong", 8, 6, "path")])
+def test_kdtree_film_vs_reference_golden(gpu_available, name, spp, depth, integ):
+    print(f"case kdtree golden {name} {integ}", flush=True)
+    g = gpu_film(load(name), 64, 64, spp, depth, "ltr", integrator=integ, accel="kdtree")
+    ref = np.load(os.path.join(GOLDEN, f"film_{name}{'_dl' if integ == 'direct' else ''}_kd.npy"))
+    assert relmse(g, ref) <= 1e-4
