@@ -100,6 +100,39 @@ def cpu_baseline(scene, width, height, depth, seconds_budget=20.0, integrator="p
             "sample": f"rows {y0}..{y0 + rows} of {width}x{height}, Halton sample 0 ({samples} samples, {s:.1f}s)"}
 
 
+def cpu_reference_baselines(width, height, depth, threads):
+    """The reference renderer itself (oracle/_ref/crystal2_ref, compiled from the reference's
+    sources) on the same sphere field written in its JSON + OBJ formats, for the path tracer
+    and the direct-lighting integrator: Halton sample 0 of the full frame, Integrator::Render
+    timed by the driver (scene parsing excluded). Returns {integrator: baseline} or None."""
+    import shutil
+    import subprocess
+    import tempfile
+    from crystal2_amd import scenes
+    ref = os.path.join(HERE, "oracle", "_ref", "crystal2_ref")
+    if not os.path.exists(ref):
+        return None
+    d = tempfile.mkdtemp(prefix="c2ref_")
+    out = {}
+    try:
+        t0 = time.time()
+        scenes.spheres_scene(d, width, height, spp=1, depth=depth, threads=threads)
+        scenes.write_config(os.path.join(d, "config_dl.json"), width, height, 1, depth, threads=threads,
+                            integrator="DirectLight")
+        log(f"cpu reference: scene files written in {time.time() - t0:.1f}s")
+        for key, cfg in (("path", "config.json"), ("direct", "config_dl.json")):
+            res = subprocess.run([ref, "render", cfg, "scene.json", "film.f32"], cwd=d, check=True,
+                                 capture_output=True, text=True, timeout=600).stdout
+            rec = json.loads(res.strip().splitlines()[-1])
+            out[key] = {"value": rec["samples"] / rec["render_s"] / 1e6, "unit": "Msamples/s",
+                        "cores": rec["threads"], "kind": "reference",
+                        "sample": f"Halton sample 0 of the full {width}x{height} frame, depth {depth} "
+                                  f"({rec['samples']} samples, {rec['render_s']:.1f}s in Integrator::Render)"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def _crop(film, W, H, win):
     """Window (x0, y0, w, h) in film coordinates (y up) of a top-down film."""
     x0, y0, w, h = win
@@ -312,7 +345,9 @@ def main():
         pt.sync()
     pt.reset()
     st0 = pt.stats()
-    pt.set_profiling(True)
+    # HIP events only around the two kernels the roofline may be quoted on: events around
+    # every launch would slow the timed region by ~4%
+    pt.set_profiling(True, kernels=["extend", "occlude"])
     pt.sync()
     t0 = time.perf_counter()
     pt.render(0, K)
@@ -328,9 +363,16 @@ def main():
     mis = st["mis_rays"] - st0["mis_rays"]
     log(f"timed {K} passes in {elapsed:.3f}s: {value:.2f} Msamples/s; rays/sample ext {ext / samples:.2f} "
         f"shadow {sh / samples:.2f} mis {mis / samples:.3f}")
-    for name, v in kt.items():
+    # per-kernel breakdown from a separate, fully instrumented render of 2 passes (not timed)
+    pt.reset()
+    pt.set_profiling(True)
+    pt.render(0, 2)
+    pt.sync()
+    for name, v in pt.kernel_times().items():
         if v["launches"]:
-            log(f"  {name:10s} {v['ms']:9.1f} ms  {v['launches']:6d} launches  {v['ms'] / v['launches']:.3f} ms avg")
+            log(f"  {name:10s} {v['ms'] / 2:9.2f} ms/pass  {v['launches'] // 2:4d} launches/pass  "
+                f"{v['ms'] / v['launches']:.3f} ms avg")
+    pt.set_profiling(False)
 
     dom = max(("extend", "occlude"), key=lambda k: kt[k]["ms"])
     rays_dom = ext if dom == "extend" else sh
@@ -355,10 +397,19 @@ def main():
             f"cache reaches plain relMSE at {eq['cache_time_to_plain_relmse_frac']} of the time; "
             f"mfma {eq['mfma']}")
 
+    refs = None
+    if not args.no_cpu_baseline:
+        try:
+            refs = cpu_reference_baselines(W, H, args.depth, max(1, min(16, len(os.sched_getaffinity(0)))))
+        except Exception as e:
+            log(f"reference renderer unavailable: {e}")
+
     dl = None
     if not args.no_direct:
         dl = direct_lighting_bench(scene, W, H, args.depth, K,
                                    cpu_seconds=0.0 if args.no_cpu_baseline else min(args.cpu_seconds, 10.0))
+        if refs:
+            dl["cpu_reference"] = refs["direct"]
         log(f"direct lighting: {dl}")
 
     bat = None
@@ -366,13 +417,18 @@ def main():
         bat = batch_bench(args.batch_scenes, depth=args.depth)
         log(f"batch: {bat}")
 
-    cpu = None
+    cpu = cpu_port = None
     if not args.no_cpu_baseline:
+        if refs:
+            cpu = refs["path"]
+            log(f"cpu baseline (reference renderer): {cpu}")
         try:
-            cpu = cpu_baseline(scene, W, H, args.depth, args.cpu_seconds)
-            log(f"cpu baseline: {cpu}")
+            cpu_port = cpu_baseline(scene, W, H, args.depth, min(args.cpu_seconds, 10.0))
+            log(f"cpu baseline (oracle port): {cpu_port}")
         except Exception as e:  # the GPU figure stands alone if the checker is absent
-            log(f"cpu baseline unavailable: {e}")
+            log(f"cpu port unavailable: {e}")
+        if cpu is None:
+            cpu = cpu_port
 
     out = {
         "metric": "Msamples/s + relMSE at equal time: plain GPU PT vs radiance-cache PT, 1 MI355X",
@@ -393,6 +449,7 @@ def main():
                                        "mis": round(mis / samples, 4)}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_port": cpu_port,
         "relmse_equal_time": eq,
         "batch": bat,
         "direct_lighting": dl,

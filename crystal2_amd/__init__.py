@@ -341,8 +341,16 @@ class WavefrontPathIntegrator:
         check(self._lib.c2_pt_get_cache_times(self._h, C.byref(t)))
         return {"infer_ms": t.infer_ms, "train_ms": t.train_ms}
 
-    def set_profiling(self, enable: bool = True):
-        check(self._lib.c2_pt_set_profiling(self._h, int(enable)))
+    def set_profiling(self, enable: bool = True, kernels=None):
+        """Time kernels with HIP events: all classes, or only those named in ``kernels``
+        (names of :data:`_abi.KERNEL_CLASSES`)."""
+        if kernels is None:
+            check(self._lib.c2_pt_set_profiling(self._h, int(enable)))
+            return
+        mask = 0
+        for k in kernels:
+            mask |= 1 << _abi.KERNEL_CLASSES.index(k)
+        check(self._lib.c2_pt_profile_kernels(self._h, mask if enable else 0))
 
     def kernel_times(self) -> dict:
         k = _abi.c2_kernel_times()

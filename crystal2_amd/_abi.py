@@ -161,6 +161,7 @@ SIGNATURES = [
     ("c2_pt_get_stats", C.c_int, [P, C.POINTER(c2_pt_stats)]),
     ("c2_pt_sync", C.c_int, [P]),
     ("c2_pt_set_profiling", C.c_int, [P, C.c_int32]),
+    ("c2_pt_profile_kernels", C.c_int, [P, C.c_uint32]),
     ("c2_pt_get_kernel_times", C.c_int, [P, C.POINTER(c2_kernel_times)]),
     ("c2_pt_capture_rays", C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, _f32p, _i32p, C.c_int64,
                                      C.POINTER(C.c_int64)]),

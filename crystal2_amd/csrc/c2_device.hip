@@ -1413,6 +1413,7 @@ int c2_accel_count_visits(const c2_accel* a, const float* rays, int64_t n, int32
 struct c2_pt {
     std::unique_ptr<c2_accel> accel;
     bool profiling = false;
+    uint32_t prof_mask = 0;  // kernel classes timed with events
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
     // ray capture request (c2_pt_capture_rays)
     int cap_bounce = -1, cap_kind = 0;
@@ -1712,13 +1713,13 @@ struct KTimer {
     hipStream_t s;
     hipEvent_t a = nullptr, b = nullptr;
     KTimer(c2_pt* p, int c, hipStream_t st) : pt(p), cls(c), s(st) {
-        if (!pt->profiling) return;
+        if (!(pt->prof_mask & (1u << c))) return;
         HIP_CHECK(hipEventCreate(&a));
         HIP_CHECK(hipEventCreate(&b));
         HIP_CHECK(hipEventRecord(a, s));
     }
     ~KTimer() {
-        if (!pt->profiling) return;
+        if (!a) return;
         (void)hipEventRecord(b, s);
         pt->events.push_back({cls, {a, b}});
     }
@@ -1946,16 +1947,20 @@ int c2_pt_get_film(c2_pt* pt, float* rgb) {
     });
 }
 
-int c2_pt_set_profiling(c2_pt* pt, int32_t enable) {
+int c2_pt_set_profiling(c2_pt* pt, int32_t enable) { return c2_pt_profile_kernels(pt, enable ? 0xffu : 0u); }
+
+int c2_pt_profile_kernels(c2_pt* pt, uint32_t mask) {
     return guard([&] {
         if (!pt) fail(C2_ERR_INVALID, "null renderer");
         HIP_CHECK(hipStreamSynchronize(pt->stream));
+        HIP_CHECK(hipStreamSynchronize(pt->side));
         for (auto& e : pt->events) {
             (void)hipEventDestroy(e.second.first);
             (void)hipEventDestroy(e.second.second);
         }
         pt->events.clear();
-        pt->profiling = enable != 0;
+        pt->prof_mask = mask & ((1u << C2_K_COUNT) - 1u);
+        pt->profiling = pt->prof_mask != 0;
     });
 }
 
@@ -2185,8 +2190,8 @@ int c2_batch_autotune(c2_batch* b, c2_batch_tuning* out) {
         hipEvent_t e0, e1;
         HIP_CHECK(hipEventCreate(&e0));
         HIP_CHECK(hipEventCreate(&e1));
-        bool saveProf = pt->profiling;
-        pt->profiling = false;
+        const uint32_t saveProf = pt->prof_mask;
+        pt->prof_mask = 0;
         float best = INFINITY;
         int32_t bestC = pt->chunk, bestQ = pt->opts.queue_blocks;
         int tried = 0;
@@ -2210,7 +2215,7 @@ int c2_batch_autotune(c2_batch* b, c2_batch_tuning* out) {
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        pt->profiling = saveProf;
+        pt->prof_mask = saveProf;
         pt->chunk = bestC;
         pt->opts.queue_blocks = bestQ;
         pt->samples = 0;
@@ -2236,8 +2241,8 @@ int c2_batch_render(c2_batch* b, int32_t use_graph) {
             enqueue_passes(pt, 0, b->max_spp, s, true);
         } else {
             if (!b->exec) {
-                bool saveProf = pt->profiling;
-                pt->profiling = false;  // no timing events inside the graph
+                const uint32_t saveProf = pt->prof_mask;
+                pt->prof_mask = 0;  // no timing events inside the graph
                 HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
                 try {
                     enqueue_film_reset(pt, s);
@@ -2246,10 +2251,10 @@ int c2_batch_render(c2_batch* b, int32_t use_graph) {
                     hipGraph_t g = nullptr;
                     (void)hipStreamEndCapture(s, &g);
                     if (g) (void)hipGraphDestroy(g);
-                    pt->profiling = saveProf;
+                    pt->prof_mask = saveProf;
                     throw;
                 }
-                pt->profiling = saveProf;
+                pt->prof_mask = saveProf;
                 HIP_CHECK(hipStreamEndCapture(s, &b->graph));
                 HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
             }

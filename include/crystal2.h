@@ -299,6 +299,10 @@ typedef struct c2_kernel_times {
     int64_t launches[C2_K_COUNT];  /* launches per class                      */
 } c2_kernel_times;
 int  c2_pt_set_profiling(c2_pt* pt, int32_t enable);   /* clears the record  */
+/* Times only the kernel classes whose bit (1 << C2_K_*) is set (0 = none); clears the
+ * record. No counterpart in the reference: it keeps the event overhead of measuring one
+ * kernel inside a timed region small. */
+int  c2_pt_profile_kernels(c2_pt* pt, uint32_t class_mask);
 int  c2_pt_get_kernel_times(c2_pt* pt, c2_kernel_times* out);  /* synchronises */
 
 /* Renders Halton sample `sample` (adding it to the film) and copies up to n rays
