@@ -204,7 +204,7 @@ struct Collapser {
             n.lo_x[s] = c.box.lo[0]; n.lo_y[s] = c.box.lo[1]; n.lo_z[s] = c.box.lo[2];
             n.hi_x[s] = c.box.hi[0]; n.hi_y[s] = c.box.hi[1]; n.hi_z[s] = c.box.hi[2];
             if (c.leaf()) {
-                n.child[s] = ~c.left;
+                n.child[s] = ~(int32_t)((uint32_t)c.left | ((uint32_t)(-c.right - 1) << kSlotBits));
                 n.count |= (uint32_t)(-c.right) << (8 * s);
             } else {
                 n.child[s] = -1;  // patched below

@@ -7,8 +7,9 @@
 //   Node4 (128 B, 128-B aligned)
 //     lo_x[4] hi_x[4] lo_y[4] hi_y[4] lo_z[4] hi_z[4]   fp32 child boxes (96 B)
 //     child[4]   >= 0 : index of an inner Node4
-//                <  0 : leaf, first primitive slot = ~child
-//     count      4 x 8-bit primitive counts of leaf children (0 for inner / empty)
+//                <  0 : leaf, ~child = first primitive slot | (count - 1) << kSlotBits
+//     count      4 x 8-bit primitive counts of leaf children (0 for inner / empty); the
+//                kernels read the count from the child word and never load this one
 //   Empty child slots carry an inverted box (+inf / -inf) and never pass the slab test.
 //
 // Child boxes are the exact fp32 unions (min/max) of the entity boxes below them,
@@ -29,7 +30,8 @@
 
 namespace c2 {
 
-constexpr int kLeafSize = 4;        // largest leaf (4-bit count field); the ray-cast accelerator's leaves
+constexpr int kLeafSize = 4;        // largest leaf (2-bit count field); the ray-cast accelerator's leaves
+constexpr int32_t kSlotBits = 28;   // leaf child word: slot in bits 0-27, count - 1 in bits 28-29
 constexpr int kLeafSizeRender = 2;  // leaves of the renderer's trees (fewer primitive tests per ray)
 constexpr int kStackSize = 64;      // per-lane traversal stack (entries)
 constexpr int kMaxBvh4Depth = 21;   // 3 * depth + 1 <= kStackSize

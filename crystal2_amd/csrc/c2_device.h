@@ -205,7 +205,6 @@ struct Hit {
 };
 
 // Stack entry: >= 0 inner node; < 0 leaf = ~(slot | (count - 1) << 28).
-constexpr int32_t kSlotBits = 28;
 
 __device__ __forceinline__ void load_prim(const DevScene& S, int slot, float4& a, float4& b, float4& c) {
     a = S.prims[3 * slot + 0];
@@ -299,7 +298,6 @@ __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, 
             float4 ny = nf[2 + negY], fy = nf[3 - negY];
             float4 nz = nf[4 + negZ], fz = nf[5 - negZ];
             int4 ch = *reinterpret_cast<const int4*>(nf + 6);
-            uint32_t cnt = *reinterpret_cast<const uint32_t*>(nf + 7);
             float lim = ANY ? tmax : fmin_ref(tmax, best);
             float k0 = INFINITY, k1 = INFINITY, k2 = INFINITY, k3 = INFINITY;
             int32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
@@ -310,9 +308,8 @@ __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, 
         float tf = fminf(fminf(fminf(lim, (fx.C - r.o.x) * r.invD.x), (fy.C - r.o.y) * r.invD.y),       \
                          (fz.C - r.o.z) * r.invD.z);                                                 \
         if (!(tf < tn)) {                                                                            \
-            uint32_t c8 = (cnt >> (8 * I)) & 0xffu;                                                  \
             KOUT = tn;                                                                               \
-            EOUT = CH >= 0 ? CH : ~((int32_t)((uint32_t)(~CH) | ((c8 - 1u) << kSlotBits)));          \
+            EOUT = CH;                                                                               \
         }                                                                                            \
     }
             C2_CHILD(0, x, ch.x, k0, e0)
