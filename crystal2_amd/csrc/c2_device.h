@@ -239,17 +239,28 @@ struct TravLds {
     float t[ANY ? 1 : TravK<ANY>::value * kTravBlock];
 };
 
+// Overflow part of a lane's stack, in private memory: the entries pushed beyond the K
+// that the LDS ring holds.
+template <bool ANY>
+struct TravOverflow {
+    static constexpr int K = TravK<ANY>::value;
+    int32_t i[kStackSize - K];
+    float t[ANY ? 1 : kStackSize - K];
+};
+// The stack proper holds only scalars and pointers, so that the stack pointer and lane
+// stay in registers (an array member would put the whole object in private memory).
 template <bool ANY>
 struct TravStack {
     static constexpr int K = TravK<ANY>::value;
     static_assert((K & (K - 1)) == 0 && K > 0 && K <= kStackSize, "LDS stack depth must be a power of two");
     int32_t* li;
     float* lt;
+    int32_t* oi;
+    float* ot;
     int lane;
     int sp;
-    int32_t oi[kStackSize - K];
-    float ot[ANY ? 1 : kStackSize - K];
-    __device__ __forceinline__ TravStack(TravLds<ANY>& L) : li(L.idx), lt(L.t), lane((int)threadIdx.x), sp(0) {}
+    __device__ __forceinline__ TravStack(TravLds<ANY>& L, TravOverflow<ANY>& O)
+        : li(L.idx), lt(L.t), oi(O.i), ot(O.t), lane((int)threadIdx.x), sp(0) {}
     __device__ __forceinline__ void push(int32_t e, float t) {
         int off = (sp & (K - 1)) * kTravBlock + lane;
         if (sp >= K) {  // evict the oldest resident entry (same ring slot) to the overflow array
@@ -281,7 +292,8 @@ __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, 
     if (root < 0) return false;
     RayPre r = make_ray(o, d);
     const int negX = r.neg[0] ? 1 : 0, negY = r.neg[1] ? 1 : 0, negZ = r.neg[2] ? 1 : 0;
-    TravStack<ANY> st(lds);
+    TravOverflow<ANY> ovf;
+    TravStack<ANY> st(lds, ovf);
     int32_t cur = root;
     float best = INFINITY;
     int32_t bestE = -1;
