@@ -291,7 +291,8 @@ __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, 
     hit.entity = -1;
     if (root < 0) return false;
     RayPre r = make_ray(o, d);
-    const int negX = r.neg[0] ? 1 : 0, negY = r.neg[1] ? 1 : 0, negZ = r.neg[2] ? 1 : 0;
+    // byte offset of each axis's near plane within its (lo, hi) pair
+    const uint32_t oX = r.neg[0] ? 16u : 0u, oY = r.neg[1] ? 16u : 0u, oZ = r.neg[2] ? 16u : 0u;
     TravOverflow<ANY> ovf;
     TravStack<ANY> st(lds, ovf);
     int32_t cur = root;
@@ -305,11 +306,17 @@ __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, 
             // picked by address (float4 index 2 * axis + neg), which is the reference's
             // swap of the two plane distances. Empty slots hold inverted boxes
             // (+inf / -inf), whose interval is empty for every direction.
-            const float4* nf = reinterpret_cast<const float4*>(S.nodes + cur);
-            float4 nx = nf[0 + negX], fx = nf[1 - negX];
-            float4 ny = nf[2 + negY], fy = nf[3 - negY];
-            float4 nz = nf[4 + negZ], fz = nf[5 - negZ];
-            int4 ch = *reinterpret_cast<const int4*>(nf + 6);
+            // 32-bit byte offsets from the array base (the node array stays below
+            // 4 GiB): the loads take a scalar base plus a 32-bit lane offset
+            const char* nb = reinterpret_cast<const char*>(S.nodes);
+            const uint32_t no = (uint32_t)cur * 128u;
+            const float4 nx = *reinterpret_cast<const float4*>(nb + (no + oX));
+            const float4 fx = *reinterpret_cast<const float4*>(nb + (no + (16u - oX)));
+            const float4 ny = *reinterpret_cast<const float4*>(nb + (no + 32u + oY));
+            const float4 fy = *reinterpret_cast<const float4*>(nb + (no + (48u - oY)));
+            const float4 nz = *reinterpret_cast<const float4*>(nb + (no + 64u + oZ));
+            const float4 fz = *reinterpret_cast<const float4*>(nb + (no + (80u - oZ)));
+            const int4 ch = *reinterpret_cast<const int4*>(nb + (no + 96u));
             float lim = ANY ? tmax : fmin_ref(tmax, best);
             float k0 = INFINITY, k1 = INFINITY, k2 = INFINITY, k3 = INFINITY;
             int32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;
@@ -330,10 +337,16 @@ __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, 
             C2_CHILD(3, w, ch.w, k3, e3)
 #undef C2_CHILD
             // sort the four (key, entry) pairs ascending; misses carry +inf keys
-#define C2_CSWAP(KA, EA, KB, EB)                       \
-    if (KB < KA) {                                     \
-        float tk = KA; KA = KB; KB = tk;               \
-        int32_t te = EA; EA = EB; EB = te;             \
+// (as selects: no branch, no exec-mask change)
+#define C2_CSWAP(KA, EA, KB, EB)                                                 \
+    {                                                                            \
+        const bool sw = KB < KA;                                                 \
+        const float lo = sw ? KB : KA, hi = sw ? KA : KB;                        \
+        const int32_t elo = sw ? EB : EA, ehi = sw ? EA : EB;                    \
+        KA = lo;                                                                 \
+        KB = hi;                                                                 \
+        EA = elo;                                                                \
+        EB = ehi;                                                                \
     }
             // near-first also for any hit: measured faster than a fixed order
             C2_CSWAP(k0, e0, k1, e1)

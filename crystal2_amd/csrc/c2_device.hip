@@ -1004,7 +1004,8 @@ struct SceneGPU {
             append_scene(descs[k], o, k == 0);
             if ((int64_t)ents.size() >= (1ll << kSlotBits) || (int64_t)prims.size() / 3 >= (1ll << kSlotBits))
                 fail(C2_ERR_UNSUPPORTED, "too many entities in the scene set");
-            if ((int64_t)nodes.size() >= 0x7fffffffll) fail(C2_ERR_UNSUPPORTED, "too many BVH nodes");
+            if ((int64_t)nodes.size() * (int64_t)sizeof(Node4) >= (1ll << 32))
+                fail(C2_ERR_UNSUPPORTED, "too many BVH nodes (the node array must stay below 4 GiB)");
         }
         dev.nodes = arena.upload(nodes);
         dev.prims = arena.upload(prims);
