@@ -56,6 +56,11 @@ constexpr int kDlChainLimit = 8;
 // on the device, so they are launched for the upper bound with grid-stride loops;
 // the cap keeps a shrinking queue from paying for tens of thousands of empty blocks.
 constexpr int kQueueBlocksDefault = 8192;
+// Path-buffer lanes: consecutive chunks (sample passes) alternate between two sets of
+// path buffers on two streams, so the long, thinly populated last bounces of one pass
+// share the GPU with the first bounces of the next. Film accumulation stays in chunk
+// order (an event chain), so the films equal those of a single lane bit for bit.
+constexpr int kLanes = 2;
 
 // ============================================================================ accelerator kernels
 
@@ -1424,7 +1429,9 @@ struct c2_pt {
     c2_render_opts opts{};
     PassParams pp{};
     DeviceArena arena;
-    PathBuffers B{};
+    PathBuffers B{};   // lane 0 (the radiance cache and ray capture run on lane 0 only)
+    PathBuffers B1{};  // lane 1: a second set of path buffers (see enqueue_passes)
+    int lanes = 1;
     int32_t chunk = 0;      // paths in flight per launch (at most chunk_cap)
     int32_t chunk_cap = 0;  // paths the buffers hold
     // views (one per scene of a batch) and the views taking part in each sample pass
@@ -1442,6 +1449,9 @@ struct c2_pt {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // MIS rays, concurrent with the shadow rays
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t stream1 = nullptr, side1 = nullptr;  // lane 1 and its MIS stream
+    hipEvent_t ev_fork1 = nullptr, ev_join1 = nullptr, ev_start = nullptr, ev_end = nullptr;
+    hipEvent_t ev_acc[2] = {nullptr, nullptr};  // last film accumulation of each lane
     int64_t launches = 0;
     int64_t samples = 0;
     // radiance cache
@@ -1451,18 +1461,18 @@ struct c2_pt {
     int64_t cache_queries = 0, cache_rows = 0;
 
     ~c2_pt() {
-        // drain both streams before the arenas release the buffers they use
-        if (side) (void)hipStreamSynchronize(side);
-        if (stream) (void)hipStreamSynchronize(stream);
+        // drain every stream before the arenas release the buffers they use
+        for (hipStream_t q : {side, stream, side1, stream1})
+            if (q) (void)hipStreamSynchronize(q);
         for (auto* v : {&events, &cache_events})
             for (auto& e : *v) {
                 (void)hipEventDestroy(e.second.first);
                 (void)hipEventDestroy(e.second.second);
             }
-        if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
-        if (side) (void)hipStreamDestroy(side);
-        if (stream) (void)hipStreamDestroy(stream);
+        for (hipEvent_t e : {ev_fork, ev_join, ev_fork1, ev_join1, ev_start, ev_end, ev_acc[0], ev_acc[1]})
+            if (e) (void)hipEventDestroy(e);
+        for (hipStream_t q : {side, stream, side1, stream1})
+            if (q) (void)hipStreamDestroy(q);
     }
 };
 
@@ -1501,6 +1511,44 @@ void check_opts(const c2_render_opts* opts) {
 
 // Builds a renderer over n scenes (one view each) sharing one concatenated scene,
 // path state sized for the largest sample pass, and one film per view.
+// Per-path buffers of one lane for `ch` paths in flight with `lmax` shadow slots each;
+// the eligible-light list and the running ray totals are shared by the lanes.
+void alloc_path_buffers(PathBuffers& B, DeviceArena& A, size_t ch, int lmax, const int32_t* elig,
+                        unsigned long long* totals) {
+    const size_t slots = ch * (size_t)lmax;
+    B.ray_o = A.alloc<float4>(ch);
+    B.ray_d = A.alloc<float4>(ch);
+    B.beta = A.alloc<float4>(ch);
+    B.beta2 = A.alloc<float4>(ch);
+    B.L = A.alloc<float4>(ch);
+    B.meta = A.alloc<int4>(ch);
+    B.vid = A.alloc<int32_t>(ch);
+    B.hit = A.alloc<float4>(ch);
+    B.vtx_p = A.alloc<float4>(ch);
+    B.vtx_n = A.alloc<float4>(ch);
+    B.vtx_du = A.alloc<float4>(ch);
+    B.vtx_d = A.alloc<float4>(ch);
+    B.vtx_i = A.alloc<int4>(ch);
+    B.misflag = A.alloc<int32_t>(ch);
+    B.mis_hit = A.alloc<int32_t>(ch);
+    B.elig = elig;
+    B.texel = A.alloc<int32_t>(ch);
+    B.mis_w = A.alloc<float4>(ch);
+    B.mis_f = A.alloc<float4>(ch);
+    B.mis_L = A.alloc<float4>(ch);
+    B.sh_o = A.alloc<float4>(slots);
+    B.sh_d = A.alloc<float4>(slots);
+    B.sh_c = A.alloc<float4>(slots);
+    B.sh_vis = A.alloc<uint8_t>(slots);
+    B.queue[0] = A.alloc<int32_t>(ch);
+    B.queue[1] = A.alloc<int32_t>(ch);
+    B.sh_list = A.alloc<int32_t>(slots);
+    B.mis_list = A.alloc<int32_t>(ch);
+    B.counters = A.alloc<int32_t>(8);
+    B.totals = totals;
+    HIP_CHECK(hipMemset(B.counters, 0, 8 * sizeof(int32_t)));
+}
+
 void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, int n, const c2_render_opts* opts,
               bool per_sample_passes) {
     check_opts(opts);
@@ -1618,40 +1666,13 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     View* dviews = A.upload(pt->views);
     pt->pv = A.upload(pv);
     P.views = dviews;
-    size_t ch = (size_t)chunk, slots = (size_t)chunk * P.lmax;
-    B.ray_o = A.alloc<float4>(ch);
-    B.ray_d = A.alloc<float4>(ch);
-    B.beta = A.alloc<float4>(ch);
-    B.beta2 = A.alloc<float4>(ch);
-    B.L = A.alloc<float4>(ch);
-    B.meta = A.alloc<int4>(ch);
-    B.vid = A.alloc<int32_t>(ch);
-    B.hit = A.alloc<float4>(ch);
-    B.vtx_p = A.alloc<float4>(ch);
-    B.vtx_n = A.alloc<float4>(ch);
-    B.vtx_du = A.alloc<float4>(ch);
-    B.vtx_d = A.alloc<float4>(ch);
-    B.vtx_i = A.alloc<int4>(ch);
-    B.misflag = A.alloc<int32_t>(ch);
-    B.mis_hit = A.alloc<int32_t>(ch);
     if (elig.empty()) elig.push_back(-1);
-    B.elig = A.upload(elig);
-    B.texel = A.alloc<int32_t>(ch);
-    B.mis_w = A.alloc<float4>(ch);
-    B.mis_f = A.alloc<float4>(ch);
-    B.mis_L = A.alloc<float4>(ch);
-    B.sh_o = A.alloc<float4>(slots);
-    B.sh_d = A.alloc<float4>(slots);
-    B.sh_c = A.alloc<float4>(slots);
-    B.sh_vis = A.alloc<uint8_t>(slots);
-    B.queue[0] = A.alloc<int32_t>(ch);
-    B.queue[1] = A.alloc<int32_t>(ch);
-    B.sh_list = A.alloc<int32_t>(slots);
-    B.mis_list = A.alloc<int32_t>(ch);
-    B.counters = A.alloc<int32_t>(8);
-    B.totals = A.alloc<unsigned long long>(4);
-    HIP_CHECK(hipMemset(B.counters, 0, 8 * sizeof(int32_t)));
-    HIP_CHECK(hipMemset(B.totals, 0, 4 * sizeof(unsigned long long)));
+    const int32_t* delig = A.upload(elig);
+    unsigned long long* totals = A.alloc<unsigned long long>(4);
+    HIP_CHECK(hipMemset(totals, 0, 4 * sizeof(unsigned long long)));
+    pt->lanes = kLanes;
+    alloc_path_buffers(B, A, (size_t)chunk, P.lmax, delig, totals);
+    if (pt->lanes > 1) alloc_path_buffers(pt->B1, A, (size_t)chunk, P.lmax, delig, totals);
     // Every view's film covers its whole frame: k_accumulate, the reset and the film
     // read-back index it by full-frame pixel (y * width + x), also when a pixel
     // window restricts which pixels receive samples.
@@ -1665,6 +1686,12 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     HIP_CHECK(hipStreamCreateWithFlags(&pt->side, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreateWithFlags(&pt->ev_fork, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&pt->ev_join, hipEventDisableTiming));
+    if (pt->lanes > 1) {
+        HIP_CHECK(hipStreamCreateWithFlags(&pt->stream1, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&pt->side1, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&pt->ev_fork1, &pt->ev_join1, &pt->ev_start, &pt->ev_end, &pt->ev_acc[0], &pt->ev_acc[1]})
+            HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    }
 }
 
 // Zeroes every film with kernels (also valid inside a captured graph).
@@ -1763,10 +1790,20 @@ namespace {
 // Enqueues Halton sample passes [sample_begin, + sample_count) on stream s. A single
 // renderer uses one pass table for every sample; a batch uses table k for sample k
 // (the views with spp > k).
-void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipStream_t s, bool per_sample_passes) {
+void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipStream_t s0, bool per_sample_passes) {
         PassParams P = pt->pp;
         DevScene S = pt->accel->scene.dev;
         const bool SGkd = pt->accel->scene.kd;
+        // lanes: chunk c runs on lane c % L (lane 0 = the caller's stream s0); lane 1
+        // starts after the work already queued on s0, and s0 waits for it at the end.
+        // One lane with the radiance cache (training between chunks), ray capture and
+        // batches (per-sample pass tables, also captured into a graph).
+        const int L = (pt->lanes > 1 && P.cache_bounce < 0 && pt->cap_bounce < 0 && !per_sample_passes) ? pt->lanes : 1;
+        if (L > 1) {
+            HIP_CHECK(hipEventRecord(pt->ev_start, s0));
+            HIP_CHECK(hipStreamWaitEvent(pt->stream1, pt->ev_start, 0));
+        }
+        int64_t nchunk = 0;
         for (int k = sample_begin; k < sample_begin + sample_count; k++) {
             const c2_pt::PassTab& tab = pt->passes[per_sample_passes ? k : 0];
             const int64_t npix = tab.npix;
@@ -1777,12 +1814,16 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
             for (int64_t beg = 0; beg < npix; beg += pt->chunk) {
                 P.pix_begin = (int32_t)beg;
                 P.pix_count = (int32_t)std::min<int64_t>(pt->chunk, npix - beg);
+                const int lane = (int)(nchunk % L);
+                const hipStream_t s = lane ? pt->stream1 : s0, side = lane ? pt->side1 : pt->side;
+                const hipEvent_t evf = lane ? pt->ev_fork1 : pt->ev_fork, evj = lane ? pt->ev_join1 : pt->ev_join;
+                PathBuffers& LB = lane ? pt->B1 : pt->B;
                 const int qcap = pt->opts.queue_blocks > 0 ? pt->opts.queue_blocks : kQueueBlocksDefault;
                 int g = std::min(grid_for(P.pix_count), qcap);
-                if (P.cache_bounce >= 0) hipLaunchKernelGGL(k_chunk_reset, dim3(1), dim3(1), 0, s, pt->B);
+                if (P.cache_bounce >= 0) hipLaunchKernelGGL(k_chunk_reset, dim3(1), dim3(1), 0, s, LB);
                 {
                     KTimer kt(pt, C2_K_GENERATE, s);
-                    hipLaunchKernelGGL(k_generate, dim3(g), dim3(kBlock), 0, s, S, pt->B, P);
+                    hipLaunchKernelGGL(k_generate, dim3(g), dim3(kBlock), 0, s, S, LB, P);
                 }
                 pt->launches++;
                 int q = 0;
@@ -1794,74 +1835,74 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                     const int steps = P.integrator == C2_INTEGRATOR_DIRECT ? (P.max_depth > 0 ? kDlChainLimit : 0)
                                                                           : P.max_depth;
                     for (int b = 0; b < steps; b++) {
-                        hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, pt->B, q ^ 1);
+                        hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, LB, q ^ 1);
                         if (pt->cap_bounce == b && pt->cap_kind == 0) {
-                            hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, pt->B, 0, q,
+                            hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, LB, 0, q,
                                                P.lmax, pt->cap_n, pt->cap_rays, pt->cap_ignore,
-                                               pt->B.counters + 6);
+                                               LB.counters + 6);
                         }
                         {
                             KTimer kt(pt, C2_K_EXTEND, s);
                             if (SGkd)
-                                hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
+                                hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
                             else
-                                hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
+                                hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
                         }
                         if (S.tex) {
                             KTimer kt(pt, C2_K_SHADE, s);
-                            hipLaunchKernelGGL(k_surface, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
+                            hipLaunchKernelGGL(k_surface, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
                             pt->launches++;
                         }
                         {
                             KTimer kt(pt, C2_K_SHADE, s);
-                            hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q, b);
+                            hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, s, S, LB, P, q, b);
                         }
                         if (P.cache_bounce == b) {
                             hipEvent_t e0, e1;
                             HIP_CHECK(hipEventCreate(&e0));
                             HIP_CHECK(hipEventCreate(&e1));
                             HIP_CHECK(hipEventRecord(e0, s));
-                            nrc_launch_infer(pt->nrc, pt->B.nrc_list, pt->B.counters + 4, P.pix_count, pt->B.nrc_feat,
-                                             pt->B.L, pt->B.beta, s);
+                            nrc_launch_infer(pt->nrc, LB.nrc_list, LB.counters + 4, P.pix_count, LB.nrc_feat,
+                                             LB.L, LB.beta, s);
                             HIP_CHECK(hipEventRecord(e1, s));
                             pt->cache_events.push_back({0, {e0, e1}});
                         }
                         {
                             KTimer kt(pt, C2_K_NEE, s);
-                            hipLaunchKernelGGL(k_nee, dim3(g), dim3(kBlock), 0, s, S, pt->B, P, q);
+                            hipLaunchKernelGGL(k_nee, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
                         }
                         if (pt->cap_bounce == b && pt->cap_kind == 1) {
-                            hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, pt->B, 1, q,
+                            hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, LB, 1, q,
                                                P.lmax, pt->cap_n, pt->cap_rays, pt->cap_ignore,
-                                               pt->B.counters + 6);
+                                               LB.counters + 6);
                         }
                         // The rare MIS rays (closest hit) run on the side stream, concurrently
                         // with the shadow rays; both only read what k_nee wrote, and k_resolve
                         // waits for both.
-                        HIP_CHECK(hipEventRecord(pt->ev_fork, s));
-                        HIP_CHECK(hipStreamWaitEvent(pt->side, pt->ev_fork, 0));
+                        HIP_CHECK(hipEventRecord(evf, s));
+                        HIP_CHECK(hipStreamWaitEvent(side, evf, 0));
                         {
-                            KTimer kt(pt, C2_K_MIS, pt->side);
+                            KTimer kt(pt, C2_K_MIS, side);
                             if (SGkd)
                                 hipLaunchKernelGGL(k_mis<true>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0,
-                                                   pt->side, S, pt->B, P);
+                                                   side, S, LB, P);
                             else
                                 hipLaunchKernelGGL(k_mis<false>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0,
-                                                   pt->side, S, pt->B, P);
+                                                   side, S, LB, P);
                         }
-                        HIP_CHECK(hipEventRecord(pt->ev_join, pt->side));
+                        HIP_CHECK(hipEventRecord(evj, side));
                         int gs = std::min(grid_for((int64_t)P.pix_count * P.lmax), qcap);
                         {
                             KTimer kt(pt, C2_K_OCCLUDE, s);
                             if (SGkd)
-                                hipLaunchKernelGGL(k_occlude<true>, dim3(gs), dim3(kBlock), 0, s, S, pt->B);
+                                hipLaunchKernelGGL(k_occlude<true>, dim3(gs), dim3(kBlock), 0, s, S, LB);
                             else
-                                hipLaunchKernelGGL(k_occlude<false>, dim3(gs), dim3(kBlock), 0, s, S, pt->B);
+                                hipLaunchKernelGGL(k_occlude<false>, dim3(gs), dim3(kBlock), 0, s, S, LB);
                         }
-                        HIP_CHECK(hipStreamWaitEvent(s, pt->ev_join, 0));
+                        HIP_CHECK(hipStreamWaitEvent(s, evj, 0));
                         {
                             KTimer kt(pt, C2_K_RESOLVE, s);
-                            hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kBlock), 0, s, pt->B, P, q);
+                            hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kBlock), 0, s, LB, P, q);
                         }
                         pt->launches += 7;
                         q ^= 1;
@@ -1872,20 +1913,28 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                     HIP_CHECK(hipEventCreate(&e0));
                     HIP_CHECK(hipEventCreate(&e1));
                     HIP_CHECK(hipEventRecord(e0, s));
-                    hipLaunchKernelGGL(k_cache_collect, dim3(grid_for(P.train_paths)), dim3(kBlock), 0, s, pt->B, P);
-                    nrc_launch_train(pt->nrc, pt->B.bfeat, pt->B.btarget, pt->B.counters + 7, P.train_cap, s, true);
+                    hipLaunchKernelGGL(k_cache_collect, dim3(grid_for(P.train_paths)), dim3(kBlock), 0, s, LB, P);
+                    nrc_launch_train(pt->nrc, LB.bfeat, LB.btarget, LB.counters + 7, P.train_cap, s, true);
                     HIP_CHECK(hipEventRecord(e1, s));
                     pt->cache_events.push_back({1, {e0, e1}});
                 }
+                // films accumulate in chunk order: after the previous chunk's (other lane)
+                if (L > 1 && nchunk > 0) HIP_CHECK(hipStreamWaitEvent(s, pt->ev_acc[lane ^ 1], 0));
                 {
                     KTimer kt(pt, C2_K_ACCUMULATE, s);
-                    hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, s, pt->B, P, pt->opts.film_mode,
+                    hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, s, LB, P, pt->opts.film_mode,
                                        pt->film, pt->comp, pt->film64);
                 }
+                if (L > 1) HIP_CHECK(hipEventRecord(pt->ev_acc[lane], s));
                 pt->launches++;
                 HIP_CHECK(hipGetLastError());
+                nchunk++;
             }
             pt->samples += npix;
+        }
+        if (L > 1) {
+            HIP_CHECK(hipEventRecord(pt->ev_end, pt->stream1));
+            HIP_CHECK(hipStreamWaitEvent(s0, pt->ev_end, 0));
         }
 }
 
@@ -1953,8 +2002,7 @@ int c2_pt_set_profiling(c2_pt* pt, int32_t enable) { return c2_pt_profile_kernel
 int c2_pt_profile_kernels(c2_pt* pt, uint32_t mask) {
     return guard([&] {
         if (!pt) fail(C2_ERR_INVALID, "null renderer");
-        HIP_CHECK(hipStreamSynchronize(pt->stream));
-        HIP_CHECK(hipStreamSynchronize(pt->side));
+        HIP_CHECK(hipDeviceSynchronize());
         for (auto& e : pt->events) {
             (void)hipEventDestroy(e.second.first);
             (void)hipEventDestroy(e.second.second);
