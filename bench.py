@@ -160,13 +160,21 @@ def equal_time(scene, W, H, depth, seconds, judge_spp=4096, crop=256, cache_boun
     log(f"judge: {judge_spp} spp FP64 on {crop}x{crop} in {time.perf_counter() - t0:.1f}s")
 
     def run(pt, budget, checkpoints, start=0):
-        used, k, curve = 0.0, start, []
+        # passes rendered in one call up to the next checkpoint (a renderer overlaps the
+        # passes of one call), fewer when the time left would not hold them
+        used, k, curve, per = 0.0, start, [], None
         while used < budget:
+            done = k - start
+            n = min((c for c in checkpoints if c > done), default=done * 2) - done
+            if per:
+                n = max(1, min(n, int((budget - used) / per)))
             a = time.perf_counter()
-            pt.render(k, 1)
+            pt.render(k, n)
             pt.sync()
-            used += time.perf_counter() - a
-            k += 1
+            dt = time.perf_counter() - a
+            used += dt
+            per = dt / n
+            k += n
             if (k - start) in checkpoints or used >= budget:
                 curve.append((round(used, 4), k - start, _relmse(_crop(pt.film(), W, H, win), ref)))
         return curve

@@ -60,7 +60,23 @@ constexpr int kQueueBlocksDefault = 8192;
 // path buffers on two streams, so the long, thinly populated last bounces of one pass
 // share the GPU with the first bounces of the next. Film accumulation stays in chunk
 // order (an event chain), so the films equal those of a single lane bit for bit.
-constexpr int kLanes = 2;
+#ifndef C2_LANES
+#define C2_LANES 2
+#endif
+#ifndef C2_MIS_SIDE
+#define C2_MIS_SIDE 1
+#endif
+constexpr int kLanes = C2_LANES;
+static_assert(kLanes >= 1 && kLanes <= 4, "1 to 4 lanes");
+// Samples per chunk of a single renderer: a launch carries this many consecutive samples
+// of every pixel (more rays per launch in the thinly populated late bounces).
+#ifndef C2_SAMPLES_PER_CHUNK
+#define C2_SAMPLES_PER_CHUNK 4
+#endif
+constexpr int kSamplesPerChunk = C2_SAMPLES_PER_CHUNK;
+// MIS rays on a side stream of each lane, concurrent with its shadow rays (else in
+// line on the lane's stream)
+constexpr bool kMisSide = C2_MIS_SIDE != 0;
 
 // ============================================================================ accelerator kernels
 
@@ -176,8 +192,10 @@ struct View {
 };
 
 struct PassParams {
-    int32_t sample;          // Halton sample number k
+    int32_t sample;          // Halton sample number k of the chunk's first sample
     int32_t pix_begin, pix_count;  // linear pass indices of this chunk
+    int32_t nsamp;           // samples k .. k + nsamp - 1 of those indices: path i is sample
+                             // k + i / pix_count of linear index pix_begin + i % pix_count
     int32_t max_depth;
     int32_t arg_order;
     int32_t lmax;            // shadow slots per path: the largest eligible-light count (>= 1)
@@ -292,10 +310,12 @@ __device__ __forceinline__ void block_append_lights(int32_t* counter, uint32_t b
 __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, PassParams P) {
     __shared__ int32_t sm[kWaves + 1];
     int stride = gridDim.x * blockDim.x;
-    for (int base = blockIdx.x * blockDim.x; base < P.pix_count; base += stride) {
+    const int npath = P.nsamp * P.pix_count;
+    for (int base = blockIdx.x * blockDim.x; base < npath; base += stride) {
         int i = base + (int)threadIdx.x;
-        bool live = i < P.pix_count;
-        int lin = P.pix_begin + i;
+        bool live = i < npath;
+        const int js = live ? i / P.pix_count : 0;
+        int lin = P.pix_begin + (i - js * P.pix_count);
         // the view owning linear index lin: last entry with first index <= lin
         int lo = 0, hi = P.npv - 1;
         while (lo < hi) {
@@ -321,7 +341,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, 
         if (tp >= P.train_paths) train = false;  // beyond the record capacity: an ordinary path
         if (!live) continue;
         HaltonDev hd{V.exp0, V.scale1};
-        int32_t idx = halton_index_dev(V, x, y, P.sample);
+        int32_t idx = halton_index_dev(V, x, y, P.sample + js);
         float s0 = halton_value(S, hd, idx, 0), s1 = halton_value(S, hd, idx, 1);
         float px = (float)x + s0, py = (float)y + s1;
         px = px / V.width;
@@ -345,7 +365,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, 
         B.vid[i] = v;
         B.queue[0][i] = i;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) B.counters[0] = P.pix_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) B.counters[0] = npath;
 }
 
 __global__ void k_chunk_reset(PathBuffers B) {
@@ -838,34 +858,39 @@ __global__ void __launch_bounds__(kBlock) k_resolve(PathBuffers B, PassParams P,
 }
 
 // FrameBuffer::AddSample, FrameBuffer.cpp:20-32 (weight 1, in sample order)
+// The film adds of a chunk's samples of one pixel run in one thread in sample order, so
+// every pixel's sum is formed in the reference's order (sample 0, 1, ...).
 __global__ void __launch_bounds__(kBlock) k_accumulate(PathBuffers B, PassParams P, int mode, float4* film,
                                                        float4* comp, double4* film64) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < P.pix_count; i += gridDim.x * blockDim.x) {
-        float4 L4 = B.L[i];
-        V3 c{L4.x, L4.y, L4.z};
-        if (any_nan(c)) c = V3{0, 0, 0};
-        if (any_inf(c)) c = V3{0, 0, 0};
-        int pixel = B.meta[i].x;
-        if (mode == C2_FILM_F64) {
-            double4 f = film64[pixel];
-            f.x += (double)c.x;
-            f.y += (double)c.y;
-            f.z += (double)c.z;
-            f.w += 1.0;
-            film64[pixel] = f;
-        } else if (mode == C2_FILM_F32_KAHAN) {
-            float4 f = film[pixel], cp = comp[pixel];
-            float y, t;
-            y = c.x - cp.x; t = f.x + y; cp.x = (t - f.x) - y; f.x = t;
-            y = c.y - cp.y; t = f.y + y; cp.y = (t - f.y) - y; f.y = t;
-            y = c.z - cp.z; t = f.z + y; cp.z = (t - f.z) - y; f.z = t;
-            f.w += 1.f;
-            film[pixel] = f;
-            comp[pixel] = cp;
-        } else {
-            float4 f = film[pixel];
-            V3 acc = V3{f.x, f.y, f.z} + c * 1.f;
-            film[pixel] = make_float4(acc.x, acc.y, acc.z, f.w + 1.f);
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P.pix_count; t += gridDim.x * blockDim.x) {
+        for (int js = 0; js < P.nsamp; js++) {
+            const int i = js * P.pix_count + t;
+            float4 L4 = B.L[i];
+            V3 c{L4.x, L4.y, L4.z};
+            if (any_nan(c)) c = V3{0, 0, 0};
+            if (any_inf(c)) c = V3{0, 0, 0};
+            int pixel = B.meta[i].x;
+            if (mode == C2_FILM_F64) {
+                double4 f = film64[pixel];
+                f.x += (double)c.x;
+                f.y += (double)c.y;
+                f.z += (double)c.z;
+                f.w += 1.0;
+                film64[pixel] = f;
+            } else if (mode == C2_FILM_F32_KAHAN) {
+                float4 f = film[pixel], cp = comp[pixel];
+                float y, tt;
+                y = c.x - cp.x; tt = f.x + y; cp.x = (tt - f.x) - y; f.x = tt;
+                y = c.y - cp.y; tt = f.y + y; cp.y = (tt - f.y) - y; f.y = tt;
+                y = c.z - cp.z; tt = f.z + y; cp.z = (tt - f.z) - y; f.z = tt;
+                f.w += 1.f;
+                film[pixel] = f;
+                comp[pixel] = cp;
+            } else {
+                float4 f = film[pixel];
+                V3 acc = V3{f.x, f.y, f.z} + c * 1.f;
+                film[pixel] = make_float4(acc.x, acc.y, acc.z, f.w + 1.f);
+            }
         }
     }
 }
@@ -1429,8 +1454,8 @@ struct c2_pt {
     c2_render_opts opts{};
     PassParams pp{};
     DeviceArena arena;
-    PathBuffers B{};   // lane 0 (the radiance cache and ray capture run on lane 0 only)
-    PathBuffers B1{};  // lane 1: a second set of path buffers (see enqueue_passes)
+    PathBuffers B{};  // lane 0 (the radiance cache and ray capture run on lane 0 only)
+    PathBuffers BL[3]{};  // lanes 1..3: further sets of path buffers (see enqueue_passes)
     int lanes = 1;
     int32_t chunk = 0;      // paths in flight per launch (at most chunk_cap)
     int32_t chunk_cap = 0;  // paths the buffers hold
@@ -1449,9 +1474,12 @@ struct c2_pt {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // MIS rays, concurrent with the shadow rays
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    hipStream_t stream1 = nullptr, side1 = nullptr;  // lane 1 and its MIS stream
-    hipEvent_t ev_fork1 = nullptr, ev_join1 = nullptr, ev_start = nullptr, ev_end = nullptr;
-    hipEvent_t ev_acc[2] = {nullptr, nullptr};  // last film accumulation of each lane
+    // lanes 1..3: stream, MIS stream, fork / join events; per lane: end of its work and
+    // its last film accumulation
+    hipStream_t lstream[3] = {}, lside[3] = {};
+    hipEvent_t lfork[3] = {}, ljoin[3] = {}, lend[3] = {};
+    hipEvent_t ev_start = nullptr;
+    hipEvent_t ev_acc[4] = {};
     int64_t launches = 0;
     int64_t samples = 0;
     // radiance cache
@@ -1462,16 +1490,19 @@ struct c2_pt {
 
     ~c2_pt() {
         // drain every stream before the arenas release the buffers they use
-        for (hipStream_t q : {side, stream, side1, stream1})
+        for (hipStream_t q : {side, stream, lside[0], lstream[0], lside[1], lstream[1], lside[2], lstream[2]})
             if (q) (void)hipStreamSynchronize(q);
         for (auto* v : {&events, &cache_events})
             for (auto& e : *v) {
                 (void)hipEventDestroy(e.second.first);
                 (void)hipEventDestroy(e.second.second);
             }
-        for (hipEvent_t e : {ev_fork, ev_join, ev_fork1, ev_join1, ev_start, ev_end, ev_acc[0], ev_acc[1]})
+        for (hipEvent_t e : {ev_fork, ev_join, ev_start, ev_acc[0], ev_acc[1], ev_acc[2], ev_acc[3]})
             if (e) (void)hipEventDestroy(e);
-        for (hipStream_t q : {side, stream, side1, stream1})
+        for (int l = 0; l < 3; l++)
+            for (hipEvent_t e : {lfork[l], ljoin[l], lend[l]})
+                if (e) (void)hipEventDestroy(e);
+        for (hipStream_t q : {side, stream, lside[0], lstream[0], lside[1], lstream[1], lside[2], lstream[2]})
             if (q) (void)hipStreamDestroy(q);
     }
 };
@@ -1571,6 +1602,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     P.arg_order = opts->arg_order;
     P.integrator = opts->integrator;
     P.cache_bounce = -1;
+    P.nsamp = 1;
     P.train_ratio = 1;
     P.train_phase = 0;
     P.train_cap = 0;
@@ -1655,11 +1687,13 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     }
     int64_t maxPass = 0;
     for (auto& t : pt->passes) maxPass = std::max(maxPass, t.npix);
-    int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : maxPass;
-    // keep the per-path shadow slots within ~6 GiB
+    // default: kSamplesPerChunk samples of a pass per launch (a batch: one pass)
+    const int64_t dflt = per_sample_passes ? maxPass : maxPass * kSamplesPerChunk;
+    int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : dflt;
+    // keep each lane's path buffers within ~8 GiB
     int64_t perPath = (int64_t)P.lmax * (3 * 16 + 1 + 4) + 16 * 14 + 16;
-    int64_t maxChunk = std::max<int64_t>(4096, ((int64_t)6 << 30) / perPath);
-    chunk = std::max<int64_t>(1, std::min(std::min(chunk, maxPass), maxChunk));
+    int64_t maxChunk = std::max<int64_t>(4096, ((int64_t)8 << 30) / perPath);
+    chunk = std::max<int64_t>(1, std::min(std::min(chunk, std::max(maxPass, dflt)), maxChunk));
     pt->chunk = pt->chunk_cap = (int32_t)chunk;
     PathBuffers& B = pt->B;
     DeviceArena& A = pt->arena;
@@ -1672,7 +1706,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     HIP_CHECK(hipMemset(totals, 0, 4 * sizeof(unsigned long long)));
     pt->lanes = kLanes;
     alloc_path_buffers(B, A, (size_t)chunk, P.lmax, delig, totals);
-    if (pt->lanes > 1) alloc_path_buffers(pt->B1, A, (size_t)chunk, P.lmax, delig, totals);
+    for (int l = 1; l < pt->lanes; l++) alloc_path_buffers(pt->BL[l - 1], A, (size_t)chunk, P.lmax, delig, totals);
     // Every view's film covers its whole frame: k_accumulate, the reset and the film
     // read-back index it by full-frame pixel (y * width + x), also when a pixel
     // window restricts which pixels receive samples.
@@ -1687,10 +1721,14 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     HIP_CHECK(hipEventCreateWithFlags(&pt->ev_fork, hipEventDisableTiming));
     HIP_CHECK(hipEventCreateWithFlags(&pt->ev_join, hipEventDisableTiming));
     if (pt->lanes > 1) {
-        HIP_CHECK(hipStreamCreateWithFlags(&pt->stream1, hipStreamNonBlocking));
-        HIP_CHECK(hipStreamCreateWithFlags(&pt->side1, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&pt->ev_fork1, &pt->ev_join1, &pt->ev_start, &pt->ev_end, &pt->ev_acc[0], &pt->ev_acc[1]})
-            HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreateWithFlags(&pt->ev_start, hipEventDisableTiming));
+        for (int l = 0; l < pt->lanes; l++) HIP_CHECK(hipEventCreateWithFlags(&pt->ev_acc[l], hipEventDisableTiming));
+        for (int l = 0; l < pt->lanes - 1; l++) {
+            HIP_CHECK(hipStreamCreateWithFlags(&pt->lstream[l], hipStreamNonBlocking));
+            if (kMisSide) HIP_CHECK(hipStreamCreateWithFlags(&pt->lside[l], hipStreamNonBlocking));
+            for (hipEvent_t* e : {&pt->lfork[l], &pt->ljoin[l], &pt->lend[l]})
+                HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        }
     }
 }
 
@@ -1801,25 +1839,36 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
         const int L = (pt->lanes > 1 && P.cache_bounce < 0 && pt->cap_bounce < 0 && !per_sample_passes) ? pt->lanes : 1;
         if (L > 1) {
             HIP_CHECK(hipEventRecord(pt->ev_start, s0));
-            HIP_CHECK(hipStreamWaitEvent(pt->stream1, pt->ev_start, 0));
+            for (int l = 0; l < L - 1; l++) HIP_CHECK(hipStreamWaitEvent(pt->lstream[l], pt->ev_start, 0));
         }
         int64_t nchunk = 0;
-        for (int k = sample_begin; k < sample_begin + sample_count; k++) {
+        const int32_t sample_end = sample_begin + sample_count;
+        for (int k = sample_begin; k < sample_end;) {
             const c2_pt::PassTab& tab = pt->passes[per_sample_passes ? k : 0];
             const int64_t npix = tab.npix;
+            // samples per chunk: several whole passes when the buffers hold them (a single
+            // renderer without the radiance cache or ray capture), else one pass in chunks
+            int ns = 1;
+            if (!per_sample_passes && P.cache_bounce < 0 && pt->cap_bounce < 0 && npix > 0)
+                ns = (int)std::max<int64_t>(1, std::min<int64_t>(pt->chunk / npix, sample_end - k));
+            const int64_t step = ns > 1 ? npix : pt->chunk;
             P.pv = pt->pv + tab.off;
             P.npv = tab.npv;
             P.sample = k;
+            P.nsamp = ns;
             if (P.cache_bounce >= 0) P.train_phase = (int32_t)(((uint32_t)k * 2654435761u) % (uint32_t)P.train_ratio);
-            for (int64_t beg = 0; beg < npix; beg += pt->chunk) {
+            for (int64_t beg = 0; beg < npix; beg += step) {
                 P.pix_begin = (int32_t)beg;
-                P.pix_count = (int32_t)std::min<int64_t>(pt->chunk, npix - beg);
+                P.pix_count = (int32_t)std::min<int64_t>(step, npix - beg);
+                const int64_t npath = (int64_t)ns * P.pix_count;
                 const int lane = (int)(nchunk % L);
-                const hipStream_t s = lane ? pt->stream1 : s0, side = lane ? pt->side1 : pt->side;
-                const hipEvent_t evf = lane ? pt->ev_fork1 : pt->ev_fork, evj = lane ? pt->ev_join1 : pt->ev_join;
-                PathBuffers& LB = lane ? pt->B1 : pt->B;
+                const hipStream_t s = lane ? pt->lstream[lane - 1] : s0;
+                const hipStream_t side = !kMisSide ? s : lane ? pt->lside[lane - 1] : pt->side;
+                const hipEvent_t evf = lane ? pt->lfork[lane - 1] : pt->ev_fork;
+                const hipEvent_t evj = lane ? pt->ljoin[lane - 1] : pt->ev_join;
+                PathBuffers& LB = lane ? pt->BL[lane - 1] : pt->B;
                 const int qcap = pt->opts.queue_blocks > 0 ? pt->opts.queue_blocks : kQueueBlocksDefault;
-                int g = std::min(grid_for(P.pix_count), qcap);
+                int g = std::min(grid_for(npath), qcap);
                 if (P.cache_bounce >= 0) hipLaunchKernelGGL(k_chunk_reset, dim3(1), dim3(1), 0, s, LB);
                 {
                     KTimer kt(pt, C2_K_GENERATE, s);
@@ -1891,7 +1940,7 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                                                    side, S, LB, P);
                         }
                         HIP_CHECK(hipEventRecord(evj, side));
-                        int gs = std::min(grid_for((int64_t)P.pix_count * P.lmax), qcap);
+                        int gs = std::min(grid_for(npath * P.lmax), qcap);
                         {
                             KTimer kt(pt, C2_K_OCCLUDE, s);
                             if (SGkd)
@@ -1919,7 +1968,7 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                     pt->cache_events.push_back({1, {e0, e1}});
                 }
                 // films accumulate in chunk order: after the previous chunk's (other lane)
-                if (L > 1 && nchunk > 0) HIP_CHECK(hipStreamWaitEvent(s, pt->ev_acc[lane ^ 1], 0));
+                if (L > 1 && nchunk > 0) HIP_CHECK(hipStreamWaitEvent(s, pt->ev_acc[(lane + L - 1) % L], 0));
                 {
                     KTimer kt(pt, C2_K_ACCUMULATE, s);
                     hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, s, LB, P, pt->opts.film_mode,
@@ -1930,11 +1979,12 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                 HIP_CHECK(hipGetLastError());
                 nchunk++;
             }
-            pt->samples += npix;
+            pt->samples += npix * ns;
+            k += ns;
         }
-        if (L > 1) {
-            HIP_CHECK(hipEventRecord(pt->ev_end, pt->stream1));
-            HIP_CHECK(hipStreamWaitEvent(s0, pt->ev_end, 0));
+        for (int l = 0; l < L - 1; l++) {
+            HIP_CHECK(hipEventRecord(pt->lend[l], pt->lstream[l]));
+            HIP_CHECK(hipStreamWaitEvent(s0, pt->lend[l], 0));
         }
 }
 

@@ -131,6 +131,12 @@ def test_sample_ranges_and_chunks_are_exact(gpu_available):
     pt.render(1, 3)
     pt.sync()
     assert np.array_equal(pt.film(), full)
+    # default chunks carry two samples of every pixel: ranges split into 1, then 2 + 1
+    pt = c2.WavefrontPathIntegrator(sc, 64, 64, 4, 5)
+    pt.render(0, 1)
+    pt.render(1, 3)
+    pt.sync()
+    assert np.array_equal(pt.film(), full)
 
 
 def test_film_modes(gpu_available):
