@@ -74,6 +74,12 @@ static_assert(kLanes >= 1 && kLanes <= 4, "1 to 4 lanes");
 #define C2_SAMPLES_PER_CHUNK 4
 #endif
 constexpr int kSamplesPerChunk = C2_SAMPLES_PER_CHUNK;
+// Shading classes (k_class_count) and the first bounce whose queue is grouped by them
+constexpr int kClasses = 32;
+#ifndef C2_SORT_FROM
+#define C2_SORT_FROM 1
+#endif
+constexpr int kSortFrom = C2_SORT_FROM;
 // MIS rays on a side stream of each lane, concurrent with its shadow rays (else in
 // line on the lane's stream)
 constexpr bool kMisSide = C2_MIS_SIDE != 0;
@@ -169,6 +175,9 @@ struct PathBuffers {
     float4* btarget;    // training batch targets [cap]
     int32_t* sh_list;
     int32_t* mis_list;
+    int32_t* queue_s;   // this bounce's queue grouped by shading class (see k_class_count)
+    uint8_t* cls;       // shading class per queue index
+    int32_t* ccount;    // [kClasses] paths per class, [kClasses] scatter cursors
     int32_t* counters;  // [0..1] queue counts, [2] shadow, [3] mis, [4] cache queries, [5] training
                         // paths, [6] captured rays, [7] training rows
     unsigned long long* totals;  // extension, shadow, mis rays (64-bit running totals)
@@ -378,6 +387,63 @@ __global__ void k_reset(PathBuffers B, int next_q) {
     B.counters[next_q] = 0;
     B.counters[2] = 0;
     B.counters[3] = 0;
+    for (int c = 0; c < 2 * kClasses; c++) B.ccount[c] = 0;
+}
+
+// Shading class of a hit: material type and its distribution (or, for glass, whether it
+// is rough); the last two classes are "no material" and "no hit".
+__device__ __forceinline__ int shade_class(const DevScene& S, int ent) {
+    if (ent < 0) return kClasses - 1;
+    const int m = S.entities[ent].z;
+    if (m < 0) return kClasses - 2;
+    const c2_material& M = S.materials[m];
+    const int sub = M.type == C2_MAT_GLASS ? (M.roughness[0] != 0.f || M.roughness[1] != 0.f ? 1 : 0) : (M.ndf & 3);
+    return min(M.type, 6) * 4 + sub;
+}
+
+// Groups a bounce's queue by shading class before the vertex kernels (k_shade, k_nee,
+// k_resolve), so that a wavefront's paths mostly run one material's code instead of
+// the code of every material its paths hit. Each path's result does not depend on the
+// order. Counting pass: per-block histogram in LDS, one atomic per class and block.
+__global__ void __launch_bounds__(kBlock) k_class_count(DevScene S, PathBuffers B, int q) {
+    __shared__ int32_t h[kClasses];
+    if (threadIdx.x < kClasses) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int n = B.counters[q];
+    const int32_t* queue = B.queue[q];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int c = shade_class(S, __float_as_int(B.hit[queue[i]].w));
+        B.cls[i] = (uint8_t)c;
+        atomicAdd(&h[c], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kClasses && h[threadIdx.x]) atomicAdd(&B.ccount[threadIdx.x], h[threadIdx.x]);
+}
+
+// Scatter pass: class c starts at the sum of the counts below it; a block reserves its
+// slots of each class with one atomic per class.
+__global__ void __launch_bounds__(kBlock) k_class_scatter(PathBuffers B, int q) {
+    __shared__ int32_t base[kClasses], h[kClasses], off[kClasses];
+    if (threadIdx.x < kClasses) {
+        int b = 0;
+        for (int c = 0; c < (int)threadIdx.x; c++) b += B.ccount[c];
+        base[threadIdx.x] = b;
+    }
+    const int n = B.counters[q];
+    const int32_t* queue = B.queue[q];
+    for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
+        if (threadIdx.x < kClasses) h[threadIdx.x] = 0;
+        __syncthreads();
+        const int i = i0 + (int)threadIdx.x;
+        const bool live = i < n;
+        const int c = live ? B.cls[i] : 0;
+        const int rank = live ? atomicAdd(&h[c], 1) : 0;
+        __syncthreads();
+        if (threadIdx.x < kClasses) off[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&B.ccount[kClasses + threadIdx.x], h[threadIdx.x]) : 0;
+        __syncthreads();
+        if (live) B.queue_s[base[c] + off[c] + rank] = queue[i];
+        __syncthreads();
+    }
 }
 
 template <bool KD>
@@ -1575,9 +1641,13 @@ void alloc_path_buffers(PathBuffers& B, DeviceArena& A, size_t ch, int lmax, con
     B.queue[1] = A.alloc<int32_t>(ch);
     B.sh_list = A.alloc<int32_t>(slots);
     B.mis_list = A.alloc<int32_t>(ch);
+    B.queue_s = A.alloc<int32_t>(ch);
+    B.cls = A.alloc<uint8_t>(ch);
+    B.ccount = A.alloc<int32_t>(2 * kClasses);
     B.counters = A.alloc<int32_t>(8);
     B.totals = totals;
     HIP_CHECK(hipMemset(B.counters, 0, 8 * sizeof(int32_t)));
+    HIP_CHECK(hipMemset(B.ccount, 0, 2 * kClasses * sizeof(int32_t)));
 }
 
 void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, int n, const c2_render_opts* opts,
@@ -1897,14 +1967,23 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                             else
                                 hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
                         }
+                        // the vertex kernels read this bounce's queue grouped by shading class
+                        PathBuffers SB = LB;
+                        if (b >= kSortFrom) {
+                            KTimer kt(pt, C2_K_SHADE, s);
+                            hipLaunchKernelGGL(k_class_count, dim3(g), dim3(kBlock), 0, s, S, LB, q);
+                            hipLaunchKernelGGL(k_class_scatter, dim3(g), dim3(kBlock), 0, s, LB, q);
+                            SB.queue[q] = LB.queue_s;
+                            pt->launches += 2;
+                        }
                         if (S.tex) {
                             KTimer kt(pt, C2_K_SHADE, s);
-                            hipLaunchKernelGGL(k_surface, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
+                            hipLaunchKernelGGL(k_surface, dim3(g), dim3(kBlock), 0, s, S, SB, P, q);
                             pt->launches++;
                         }
                         {
                             KTimer kt(pt, C2_K_SHADE, s);
-                            hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, s, S, LB, P, q, b);
+                            hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, s, S, SB, P, q, b);
                         }
                         if (P.cache_bounce == b) {
                             hipEvent_t e0, e1;
@@ -1918,7 +1997,7 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         }
                         {
                             KTimer kt(pt, C2_K_NEE, s);
-                            hipLaunchKernelGGL(k_nee, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
+                            hipLaunchKernelGGL(k_nee, dim3(g), dim3(kBlock), 0, s, S, SB, P, q);
                         }
                         if (pt->cap_bounce == b && pt->cap_kind == 1) {
                             hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, LB, 1, q,
@@ -1951,7 +2030,7 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         HIP_CHECK(hipStreamWaitEvent(s, evj, 0));
                         {
                             KTimer kt(pt, C2_K_RESOLVE, s);
-                            hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kBlock), 0, s, LB, P, q);
+                            hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kBlock), 0, s, SB, P, q);
                         }
                         pt->launches += 7;
                         q ^= 1;
