@@ -68,8 +68,8 @@ constexpr int kQueueBlocksDefault = 8192;
 #endif
 constexpr int kLanes = C2_LANES;
 static_assert(kLanes >= 1 && kLanes <= 4, "1 to 4 lanes");
-// Samples per chunk of a single renderer: a launch carries this many consecutive samples
-// of every pixel (more rays per launch in the thinly populated late bounces).
+// Samples per chunk: a launch carries up to this many consecutive samples of every pixel
+// (more rays per launch in the thinly populated late bounces).
 #ifndef C2_SAMPLES_PER_CHUNK
 #define C2_SAMPLES_PER_CHUNK 4
 #endif
@@ -1757,8 +1757,8 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     }
     int64_t maxPass = 0;
     for (auto& t : pt->passes) maxPass = std::max(maxPass, t.npix);
-    // default: kSamplesPerChunk samples of a pass per launch (a batch: one pass)
-    const int64_t dflt = per_sample_passes ? maxPass : maxPass * kSamplesPerChunk;
+    // default: kSamplesPerChunk samples of the largest pass per launch
+    const int64_t dflt = maxPass * kSamplesPerChunk;
     int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : dflt;
     // keep each lane's path buffers within ~8 GiB
     int64_t perPath = (int64_t)P.lmax * (3 * 16 + 1 + 4) + 16 * 14 + 16;
@@ -1774,7 +1774,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     const int32_t* delig = A.upload(elig);
     unsigned long long* totals = A.alloc<unsigned long long>(4);
     HIP_CHECK(hipMemset(totals, 0, 4 * sizeof(unsigned long long)));
-    pt->lanes = kLanes;
+    pt->lanes = per_sample_passes ? 1 : kLanes;  // batches keep one lane
     alloc_path_buffers(B, A, (size_t)chunk, P.lmax, delig, totals);
     for (int l = 1; l < pt->lanes; l++) alloc_path_buffers(pt->BL[l - 1], A, (size_t)chunk, P.lmax, delig, totals);
     // Every view's film covers its whole frame: k_accumulate, the reset and the film
@@ -1916,11 +1916,21 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
         for (int k = sample_begin; k < sample_end;) {
             const c2_pt::PassTab& tab = pt->passes[per_sample_passes ? k : 0];
             const int64_t npix = tab.npix;
-            // samples per chunk: several whole passes when the buffers hold them (a single
-            // renderer without the radiance cache or ray capture), else one pass in chunks
+            // samples per chunk: several whole passes when the buffers hold them (without
+            // the radiance cache or ray capture), else one pass in chunks. A batch's passes
+            // k .. k + ns - 1 must have the same views: the sets shrink as k passes each
+            // view's spp, so an equal view count means the same set.
             int ns = 1;
-            if (!per_sample_passes && P.cache_bounce < 0 && pt->cap_bounce < 0 && npix > 0)
-                ns = (int)std::max<int64_t>(1, std::min<int64_t>(pt->chunk / npix, sample_end - k));
+            if (P.cache_bounce < 0 && pt->cap_bounce < 0 && npix > 0) {
+                int64_t lim = std::min<int64_t>({pt->chunk / npix, (int64_t)sample_end - k, (int64_t)kSamplesPerChunk});
+                if (per_sample_passes)
+                    for (int64_t j = 1; j < lim; j++)
+                        if (pt->passes[k + j].npv != tab.npv) {
+                            lim = j;
+                            break;
+                        }
+                ns = (int)std::max<int64_t>(1, lim);
+            }
             const int64_t step = ns > 1 ? npix : pt->chunk;
             P.pv = pt->pv + tab.off;
             P.npv = tab.npv;
