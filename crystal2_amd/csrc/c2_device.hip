@@ -421,9 +421,14 @@ __global__ void __launch_bounds__(kBlock) k_class_count(DevScene S, PathBuffers 
 }
 
 // Scatter pass: class c starts at the sum of the counts below it; a block reserves its
-// slots of each class with one atomic per class.
+// slots of each class with one atomic per class. Within a block the order is kept (a
+// lane's rank among the same-class lanes before it, from five ballots), so a class's
+// paths stay in queue order, which is close to image order: the shadow and extension
+// rays they spawn stay spatially coherent.
 __global__ void __launch_bounds__(kBlock) k_class_scatter(PathBuffers B, int q) {
-    __shared__ int32_t base[kClasses], h[kClasses], off[kClasses];
+    static_assert(kClasses == 32, "five class bits");
+    __shared__ int32_t base[kClasses], wcnt[kWaves][kClasses], off[kClasses];
+    const int lane = __lane_id(), w = (int)threadIdx.x >> 6;
     if (threadIdx.x < kClasses) {
         int b = 0;
         for (int c = 0; c < (int)threadIdx.x; c++) b += B.ccount[c];
@@ -431,17 +436,33 @@ __global__ void __launch_bounds__(kBlock) k_class_scatter(PathBuffers B, int q) 
     }
     const int n = B.counters[q];
     const int32_t* queue = B.queue[q];
+    const uint64_t lower = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
     for (int i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x) {
-        if (threadIdx.x < kClasses) h[threadIdx.x] = 0;
+        for (int k = (int)threadIdx.x; k < kWaves * kClasses; k += blockDim.x) (&wcnt[0][0])[k] = 0;
         __syncthreads();
         const int i = i0 + (int)threadIdx.x;
         const bool live = i < n;
         const int c = live ? B.cls[i] : 0;
-        const int rank = live ? atomicAdd(&h[c], 1) : 0;
+        uint64_t same = __ballot(live);
+        for (int bit = 0; bit < 5; bit++) {
+            const uint64_t m = __ballot(live && ((c >> bit) & 1));
+            same &= ((c >> bit) & 1) ? m : ~m;
+        }
+        const int rank = __popcll(same & lower);
+        if (live && rank == 0) wcnt[w][c] = __popcll(same);
         __syncthreads();
-        if (threadIdx.x < kClasses) off[threadIdx.x] = h[threadIdx.x] ? atomicAdd(&B.ccount[kClasses + threadIdx.x], h[threadIdx.x]) : 0;
+        if (threadIdx.x < kClasses) {
+            const int cc = (int)threadIdx.x;
+            int t = 0;
+            for (int k = 0; k < kWaves; k++) {
+                const int v = wcnt[k][cc];
+                wcnt[k][cc] = t;  // exclusive prefix over the block's waves
+                t += v;
+            }
+            off[cc] = t ? atomicAdd(&B.ccount[kClasses + cc], t) : 0;
+        }
         __syncthreads();
-        if (live) B.queue_s[base[c] + off[c] + rank] = queue[i];
+        if (live) B.queue_s[base[c] + off[c] + wcnt[w][c] + rank] = queue[i];
         __syncthreads();
     }
 }
