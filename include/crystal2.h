@@ -262,7 +262,9 @@ typedef struct c2_render_opts {
     int32_t arg_order;        /* C2_ARGORDER_*: order of Get1D/Get2D at
                                  PathTracingIntegrator.cpp:79 (RTL = GCC/MSVC)       */
     int32_t film_mode;        /* C2_FILM_*                                             */
-    int32_t paths_in_flight;  /* 0 = all pixels of one sample pass at once             */
+    int32_t paths_in_flight;  /* paths per launch; 0 = four samples of every pixel
+                                 (a launch then carries up to four consecutive samples;
+                                 below one pass, a pass is split into chunks)        */
     int32_t queue_blocks;     /* grid cap (blocks) of the per-bounce queue kernels,
                                  which loop grid-stride; 0 = the library default     */
     int32_t win_x0, win_y0;   /* pixel window (film coordinates, y = 0 at the bottom); */
@@ -283,6 +285,10 @@ typedef struct c2_pt_stats {
 int  c2_pt_create(const c2_scene_desc* scene, const c2_render_opts* opts, c2_pt** out);
 void c2_pt_destroy(c2_pt* pt);
 int  c2_pt_reset(c2_pt* pt);
+/* Enqueues samples [sample_begin, + sample_count) on `stream` (NULL: the renderer's own).
+ * Consecutive chunks of one call alternate between two sets of path buffers on two
+ * internal streams; the call's work starts after what `stream` already holds, and
+ * `stream` waits for all of it at the end. Films are the same for any chunking. */
 int  c2_pt_render(c2_pt* pt, int32_t sample_begin, int32_t sample_count, void* stream);
 /* Film as FrameBuffer::GetImageDataRGB32F: W*H*3 float, rows top-down (host memory). */
 int  c2_pt_get_film(c2_pt* pt, float* rgb);
@@ -290,8 +296,10 @@ int  c2_pt_get_stats(const c2_pt* pt, c2_pt_stats* out);
 /* Synchronise the renderer's stream; returns the first asynchronous error. */
 int  c2_pt_sync(c2_pt* pt);
 
-/* Per-kernel-class timing with HIP events recorded on the renderer's stream
- * around every launch while profiling is enabled (adds no synchronisation). */
+/* Per-kernel-class timing with HIP events recorded on the launching stream around
+ * every launch while profiling is enabled (adds no synchronisation). With two lanes the
+ * times are wall times while the other lane's kernels share the GPU. C2_K_SHADE also
+ * covers the shading-class grouping kernels and k_surface. */
 enum { C2_K_GENERATE = 0, C2_K_EXTEND = 1, C2_K_SHADE = 2, C2_K_OCCLUDE = 3, C2_K_MIS = 4,
        C2_K_RESOLVE = 5, C2_K_ACCUMULATE = 6, C2_K_NEE = 7, C2_K_COUNT = 8 };
 typedef struct c2_kernel_times {
