@@ -1927,7 +1927,13 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
         // starts after the work already queued on s0, and s0 waits for it at the end.
         // One lane with the radiance cache (training between chunks), ray capture and
         // batches (per-sample pass tables, also captured into a graph).
-        const int L = (pt->lanes > 1 && P.cache_bounce < 0 && pt->cap_bounce < 0 && !per_sample_passes) ? pt->lanes : 1;
+        // A stream under graph capture also keeps one lane (no second branch in the graph).
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        HIP_CHECK(hipStreamIsCapturing(s0, &cap));
+        const int L = (pt->lanes > 1 && P.cache_bounce < 0 && pt->cap_bounce < 0 && !per_sample_passes &&
+                       cap == hipStreamCaptureStatusNone)
+                          ? pt->lanes
+                          : 1;
         if (L > 1) {
             HIP_CHECK(hipEventRecord(pt->ev_start, s0));
             for (int l = 0; l < L - 1; l++) HIP_CHECK(hipStreamWaitEvent(pt->lstream[l], pt->ev_start, 0));
