@@ -133,6 +133,18 @@ def cpu_reference_baselines(width, height, depth, threads):
     return out
 
 
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry), or None."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def _crop(film, W, H, win):
     """Window (x0, y0, w, h) in film coordinates (y up) of a top-down film."""
     x0, y0, w, h = win
@@ -371,16 +383,28 @@ def main():
     mis = st["mis_rays"] - st0["mis_rays"]
     log(f"timed {K} passes in {elapsed:.3f}s: {value:.2f} Msamples/s; rays/sample ext {ext / samples:.2f} "
         f"shadow {sh / samples:.2f} mis {mis / samples:.3f}")
-    # per-kernel breakdown from a separate, fully instrumented render of 2 passes (not timed)
+    # per-kernel breakdown from a separate, fully instrumented render of 4 passes (not timed)
     pt.reset()
     pt.set_profiling(True)
-    pt.render(0, 2)
+    sb = pt.stats()
+    pt.render(0, 4)
     pt.sync()
-    for name, v in pt.kernel_times().items():
+    sa = pt.stats()
+    kti = pt.kernel_times()
+    for name, v in kti.items():
         if v["launches"]:
-            log(f"  {name:10s} {v['ms'] / 2:9.2f} ms/pass  {v['launches'] // 2:4d} launches/pass  "
+            log(f"  {name:10s} {v['ms'] / 4:9.2f} ms/pass  {v['launches'] / 4:6.2f} launches/pass  "
                 f"{v['ms'] / v['launches']:.3f} ms avg")
     pt.set_profiling(False)
+    # rays per second of each tracing stage over its own launches (wall time of those
+    # launches; with two lanes another lane's kernels share the GPU meanwhile)
+    stages = {}
+    for stage, key, kern in (("extend", "extension_rays", "extend"), ("occlude", "shadow_rays", "occlude"),
+                             ("mis", "mis_rays", "mis")):
+        rays = sa[key] - sb[key]
+        ms = kti[kern]["ms"]
+        stages[stage] = {"rays": int(rays), "ms": round(ms, 3),
+                         "mrays_s": round(rays / (ms / 1e3) / 1e6, 1) if ms > 0 else None}
 
     dom = max(("extend", "occlude"), key=lambda k: kt[k]["ms"])
     rays_dom = ext if dom == "extend" else sh
@@ -437,6 +461,8 @@ def main():
             log(f"cpu port unavailable: {e}")
         if cpu is None:
             cpu = cpu_port
+        if cpu is not None:
+            cpu["cpu_model"] = cpu_model()
 
     out = {
         "metric": "Msamples/s + relMSE at equal time: plain GPU PT vs radiance-cache PT, 1 MI355X",
@@ -456,6 +482,7 @@ def main():
                    "rays_per_sample": {"extension": round(ext / samples, 3), "shadow": round(sh / samples, 3),
                                        "mis": round(mis / samples, 4)}},
         "roofline": roofline,
+        "stages": stages,
         "cpu_baseline": cpu,
         "cpu_port": cpu_port,
         "relmse_equal_time": eq,
