@@ -133,6 +133,71 @@ def cpu_reference_baselines(width, height, depth, threads):
     return out
 
 
+def extras_bench(W, H, depth, steps=16, threads=16, cpu=True):
+    """Throughput of the rows built beside the path tracer, each with the reference
+    renderer timed on the same scene and frame (Halton sample 0, `threads` cores):
+    the KD-tree accelerator (and the BVH on the same scene) on a 100-sphere field
+    (512,010 entities), image textures with the cubemap skybox, Disk shapes, and the
+    Oren-Nayar / Plastic / Beckmann materials. GPU: `steps` passes timed after 2 warm-up
+    passes, two lanes and four samples per launch as in the main measurement."""
+    import shutil
+    import subprocess
+    import tempfile
+    import crystal2_amd as c2
+    from crystal2_amd import scenes
+    ref = os.path.join(HERE, "oracle", "_ref", "crystal2_ref")
+    golden = os.path.join(HERE, "tests", "golden", "scenes")
+    tmp = tempfile.mkdtemp(prefix="c2extras_")
+    out = {}
+    try:
+        field_dir = os.path.join(tmp, "field100")
+        scenes.spheres_scene(field_dir, W, H, spp=1, depth=depth, n_spheres=100, threads=threads)
+        field = scenes.build_spheres_scene(100, 4, 2024)
+        cases = [
+            ("kdtree", field, "kdtree", field_dir, "KDTree"),
+            ("bvh_same_scene", field, "bvh", field_dir, "BVH"),
+            ("textures_skybox", c2.Scene.load(os.path.join(golden, "textured", "scene.json"),
+                                              os.path.join(golden, "textured")), "bvh",
+             os.path.join(golden, "textured"), "BVH"),
+            ("disks", c2.Scene.load(os.path.join(golden, "disks", "scene.json"), os.path.join(golden, "disks")),
+             "bvh", os.path.join(golden, "disks"), "BVH"),
+            ("orennayar_plastic_beckmann", c2.Scene.load(os.path.join(golden, "zoo_extra", "scene.json"),
+                                                         os.path.join(golden, "zoo_extra")), "bvh",
+             os.path.join(golden, "zoo_extra"), "BVH"),
+        ]
+        for name, sc, accel, sdir, refaccel in cases:
+            t0 = time.perf_counter()
+            pt = c2.WavefrontPathIntegrator(sc, W, H, spp=steps + 2, max_depth=depth, arg_order="rtl", film="f32",
+                                            accel=accel)
+            build_s = time.perf_counter() - t0
+            pt.render(0, 2)
+            pt.sync()
+            pt.reset()
+            a = time.perf_counter()
+            pt.render(0, steps)
+            pt.sync()
+            el = time.perf_counter() - a
+            pt.close()
+            rec = {"msamples_s": round(W * H * steps / el / 1e6, 2), "entities": sc.num_entities,
+                   "accel": accel, "renderer_build_s": round(build_s, 2)}
+            if cpu and os.path.exists(ref) and name != "bvh_same_scene":
+                cfg = os.path.join(tmp, f"config_{name}.json")
+                scenes.write_config(cfg, W, H, 1, depth, threads=threads, accel=refaccel)
+                res = subprocess.run([ref, "render", cfg, "scene.json", os.path.join(tmp, "film.f32")], cwd=sdir,
+                                     check=True, capture_output=True, text=True, timeout=600).stdout
+                r = json.loads(res.strip().splitlines()[-1])
+                v = r["samples"] / r["render_s"] / 1e6
+                rec["cpu_reference"] = {"value": round(v, 3), "unit": "Msamples/s", "cores": r["threads"],
+                                        "kind": "reference", "accel": refaccel,
+                                        "sample": f"Halton sample 0 of the full {W}x{H} frame, depth {depth}"}
+                rec["vs_cpu_reference"] = round(rec["msamples_s"] / v, 1)
+            out[name] = rec
+            log(f"extras {name}: {rec}")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
+
+
 def cpu_model():
     """The host CPU's model name (first /proc/cpuinfo entry), or None."""
     try:
@@ -327,6 +392,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-render comparison")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-direct", action="store_true", help="skip the DirectLighting integrator line")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the KD-tree / textures / disk / material-model throughput lines")
     ap.add_argument("--equal-seconds", type=float, default=0.0, help="equal-time budget (0: the timed run's)")
     ap.add_argument("--judge-spp", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -444,6 +511,14 @@ def main():
             dl["cpu_reference"] = refs["direct"]
         log(f"direct lighting: {dl}")
 
+    ext = None
+    if not args.no_extras:
+        try:
+            ext = extras_bench(W, H, args.depth, threads=max(1, min(16, len(os.sched_getaffinity(0)))),
+                               cpu=not args.no_cpu_baseline)
+        except Exception as e:
+            log(f"extras unavailable: {e}")
+
     bat = None
     if not args.no_batch:
         bat = batch_bench(args.batch_scenes, depth=args.depth)
@@ -488,6 +563,7 @@ def main():
         "relmse_equal_time": eq,
         "batch": bat,
         "direct_lighting": dl,
+        "extras": ext,
     }
     print(json.dumps(out), flush=True)
 

@@ -13,6 +13,7 @@
 //                    strict t with ties to the lowest entity index, any-hit over the
 //                    box-clipped interval, the `ignore` entity skipped.
 #pragma once
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include "c2_bvh.h"
@@ -484,6 +485,59 @@ C2_HD bool ray_box_ref(V3 o, V3 d, V3 inv, const float* lo, const float* hi, flo
 
 constexpr int kKdStack = 24;  // > the tree's depth limit of 20: at most one entry per level
 
+// The KD traversal's stack as the BVH's: the top kKdRing entries (node, window start,
+// window end) of each lane in an LDS ring at [slot][threadIdx], deeper entries in a
+// private overflow array. A register array indexed by the stack pointer would hold all
+// kKdStack entries in VGPRs (one wavefront per SIMD).
+constexpr int kKdRing = 8;
+struct KdLds {  // 24 KB per block; declare as `__shared__ KdLds lds;`
+    int32_t n[kKdRing * kTravBlock];
+    float a[kKdRing * kTravBlock], b[kKdRing * kTravBlock];
+};
+struct KdOverflow {
+    int32_t n[kKdStack - kKdRing];
+    float a[kKdStack - kKdRing], b[kKdStack - kKdRing];
+};
+struct KdStack {
+    int32_t* ln;
+    float* la;
+    float* lb;
+    int32_t* on;
+    float* oa;
+    float* ob;
+    int lane;
+    int sp;
+    __device__ __forceinline__ KdStack(KdLds& L, KdOverflow& O)
+        : ln(L.n), la(L.a), lb(L.b), on(O.n), oa(O.a), ob(O.b), lane((int)threadIdx.x), sp(0) {}
+    __device__ __forceinline__ void push(int32_t n, float a, float b) {
+        const int off = (sp & (kKdRing - 1)) * kTravBlock + lane;
+        if (sp >= kKdRing) {  // evict the oldest resident entry (same ring slot)
+            on[sp - kKdRing] = ln[off];
+            oa[sp - kKdRing] = la[off];
+            ob[sp - kKdRing] = lb[off];
+        }
+        ln[off] = n;
+        la[off] = a;
+        lb[off] = b;
+        sp++;
+    }
+    __device__ __forceinline__ void pop(int32_t& n, float& a, float& b) {
+        sp--;
+        const int off = (sp & (kKdRing - 1)) * kTravBlock + lane;
+        n = ln[off];
+        a = la[off];
+        b = lb[off];
+        if (sp >= kKdRing) {  // refill the freed slot with the entry kKdRing below the new top
+            ln[off] = on[sp - kKdRing];
+            la[off] = oa[sp - kKdRing];
+            lb[off] = ob[sp - kKdRing];
+        }
+    }
+};
+// LDS of a traversal kernel: the KD-tree's stack or the BVH's
+template <bool KD, bool ANY>
+using TravLdsFor = std::conditional_t<KD, KdLds, TravLds<ANY>>;
+
 // KD-tree traversal with the reference's windows (KDTree.cpp:73-81, 195-266), recursion
 // turned into an explicit stack in the same visiting order:
 //  * inner node: t_split = (split - o[axis]) / d[axis]; the near child (below, or above
@@ -497,16 +551,15 @@ constexpr int kKdStack = 24;  // > the tree's depth limit of 20: at most one ent
 //    window [tMin, tMax], skipping the ignored entity; the first hit ends the search.
 template <bool ANY>
 __device__ __forceinline__ bool kd_traverse(const DevScene& S, int32_t root, const float* mlo, const float* mhi,
-                                            V3 o, V3 d, float tMin, float tMax, int ignore, Hit& hit) {
+                                            V3 o, V3 d, float tMin, float tMax, int ignore, Hit& hit, KdLds& lds) {
     hit.entity = -1;
     if (root <= 0) return false;
     const V3 inv{1.f / d.x, 1.f / d.y, 1.f / d.z};
     if (!ANY && !ray_box_ref(o, d, inv, mlo, mhi, tMin, tMax)) return false;
     float best = INFINITY, bestU = 0.f, bestV = 0.f;
     int32_t bestE = -1;
-    int32_t sn[kKdStack];
-    float s0[kKdStack], s1[kKdStack];
-    int sp = 0;
+    KdOverflow ovf;
+    KdStack st(lds, ovf);
     int32_t p = root;
     float t0 = tMin, t1 = tMax;
     while (true) {
@@ -566,12 +619,7 @@ __device__ __forceinline__ bool kd_traverse(const DevScene& S, int32_t root, con
                 const float ts = (split - oa) / da;
                 const bool neg = da < 0;
                 const int32_t nearC = neg ? nd.w : nd.z, farC = neg ? nd.z : nd.w;
-                if (ts <= t1 && sp < kKdStack) {  // visited after the near subtree
-                    sn[sp] = farC;
-                    s0[sp] = fmax_ref(ts, 0.f);
-                    s1[sp] = t1;
-                    sp++;
-                }
+                if (ts <= t1 && st.sp < kKdStack) st.push(farC, fmax_ref(ts, 0.f), t1);  // after the near subtree
                 if (ts >= t0) {
                     p = nearC;
                     t1 = ANY ? fmin_ref(ts, t1) : fmin_ref(fmin_ref(ts, t1), best);
@@ -579,11 +627,10 @@ __device__ __forceinline__ bool kd_traverse(const DevScene& S, int32_t root, con
                 }
             }
         }
-        if (sp == 0) break;
-        sp--;
-        p = sn[sp];
-        t0 = s0[sp];
-        t1 = ANY ? s1[sp] : fmin_ref(s1[sp], best);
+        if (st.sp == 0) break;
+        float w1;
+        st.pop(p, t0, w1);
+        t1 = ANY ? w1 : fmin_ref(w1, best);
     }
     if (ANY) return false;
     hit.entity = bestE;

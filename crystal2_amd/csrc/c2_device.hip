@@ -95,14 +95,14 @@ struct KdMaster {
 template <bool KD>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, const float* __restrict__ rays, int64_t n,
                                                       int32_t* __restrict__ prim, float* __restrict__ tout, KdMaster M) {
-    __shared__ TravLds<false> lds;
+    __shared__ TravLdsFor<KD, false> lds;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
         float4 a = r4[0], b = r4[1];
         Hit h;
         bool hit;
         if constexpr (KD)
-            hit = kd_traverse<false>(S, M.root, M.lo, M.hi, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h);
+            hit = kd_traverse<false>(S, M.root, M.lo, M.hi, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h, lds);
         else
             hit = traverse<false>(S, 0, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h, lds);
         prim[i] = hit ? h.entity : -1;
@@ -114,7 +114,7 @@ template <bool KD>
 __global__ void __launch_bounds__(kBlock) k_intersect_test(DevScene S, const float* __restrict__ rays,
                                                            const int32_t* __restrict__ ignore, int64_t n,
                                                            uint8_t* __restrict__ out, KdMaster M) {
-    __shared__ TravLds<true> lds;
+    __shared__ TravLdsFor<KD, true> lds;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
         float4 a = r4[0], b = r4[1];
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(kBlock) k_intersect_test(DevScene S, const flo
         const int ig = ignore ? ignore[i] : -1;
         bool hit;
         if constexpr (KD)
-            hit = kd_traverse<true>(S, M.root, M.lo, M.hi, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ig, h);
+            hit = kd_traverse<true>(S, M.root, M.lo, M.hi, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ig, h, lds);
         else
             hit = traverse<true>(S, 0, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ig, h, lds);
         out[i] = hit ? 1 : 0;
@@ -469,7 +469,7 @@ __global__ void __launch_bounds__(kBlock) k_class_scatter(PathBuffers B, int q) 
 
 template <bool KD>
 __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, PassParams P, int q) {
-    __shared__ TravLds<false> lds;
+    __shared__ TravLdsFor<KD, false> lds;
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[0], (unsigned long long)n);
@@ -481,7 +481,7 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, Pa
         bool hit;
         if constexpr (KD)
             hit = kd_traverse<false>(S, V.root, V.kd_lo, V.kd_hi, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY,
-                                     -1, h);
+                                     -1, h, lds);
         else
             hit = traverse<false>(S, V.root, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY, -1, h, lds);
         B.hit[p] = make_float4(h.t, h.u, h.v, __int_as_float(hit ? h.entity : -1));
@@ -830,7 +830,7 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
 
 template <bool KD>
 __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
-    __shared__ TravLds<true> lds;
+    __shared__ TravLdsFor<KD, true> lds;
     int n = B.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[1], (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -840,7 +840,7 @@ __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
         bool occluded;
         if constexpr (KD)
             occluded = kd_traverse<true>(S, __float_as_int(d.w), nullptr, nullptr, V3{o.x, o.y, o.z},
-                                         V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f, __float_as_int(o.w), h);
+                                         V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f, __float_as_int(o.w), h, lds);
         else
             occluded = traverse<true>(S, __float_as_int(d.w), V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f,
                                       __float_as_int(o.w), h, lds);
@@ -852,7 +852,7 @@ __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
 // hit entity can receive f * Le * w / pdf_bsdf, :255-273
 template <bool KD>
 __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassParams P) {
-    __shared__ TravLds<false> lds;
+    __shared__ TravLdsFor<KD, false> lds;
     int n = B.counters[3];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[2], (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -868,7 +868,7 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassP
         const View& V = P.views[B.vid[p]];
         bool found;
         if constexpr (KD)
-            found = kd_traverse<false>(S, V.root, V.kd_lo, V.kd_hi, o, wi, 0.f, INFINITY, -1, h);
+            found = kd_traverse<false>(S, V.root, V.kd_lo, V.kd_hi, o, wi, 0.f, INFINITY, -1, h, lds);
         else
             found = traverse<false>(S, V.root, o, wi, 0.f, INFINITY, -1, h, lds);
         if (found) {
