@@ -511,11 +511,11 @@ def main():
             dl["cpu_reference"] = refs["direct"]
         log(f"direct lighting: {dl}")
 
-    ext = None
+    extras = None
     if not args.no_extras:
         try:
-            ext = extras_bench(W, H, args.depth, threads=max(1, min(16, len(os.sched_getaffinity(0)))),
-                               cpu=not args.no_cpu_baseline)
+            extras = extras_bench(W, H, args.depth, threads=max(1, min(16, len(os.sched_getaffinity(0)))),
+                                  cpu=not args.no_cpu_baseline)
         except Exception as e:
             log(f"extras unavailable: {e}")
 
@@ -563,7 +563,7 @@ def main():
         "relmse_equal_time": eq,
         "batch": bat,
         "direct_lighting": dl,
-        "extras": ext,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
 
