@@ -48,9 +48,9 @@ struct DevScene {
     const int4* tex;           // texture id k: tex[k - 1] = (width, height, first texel, 0)
     const float* lin;          // [256]: (b / 255)^2.2, the texel linearisation
     const float* tri_uv;       // [ntri][6] vertex uv (allocated with textures only)
-    const int4* kd_nodes;      // KD-tree accelerator (null with the BVH): nodes, leaf lists,
-    const int32_t* kd_objs;    // and per-entity boxes (lo, hi)
-    const float4* kd_box;
+    const int4* kd_nodes;      // KD-tree accelerator (null with the BVH): nodes and, per leaf
+    const float4* kd_rec;      // entry, 5 float4: (box lo, entity), (box hi, kind), then the
+                               // triangle's 9 vertex floats or (sphere / disk record index)
     int32_t num_lights;
     int32_t num_entities;
 };
@@ -567,16 +567,16 @@ __device__ __forceinline__ bool kd_traverse(const DevScene& S, int32_t root, con
             const int4 nd = S.kd_nodes[p];
             if (nd.x < 0) {
                 for (int k = 0; k < nd.w; k++) {
-                    const int32_t e = S.kd_objs[nd.z + k];
-                    const float4 blo = S.kd_box[2 * e], bhi = S.kd_box[2 * e + 1];
+                    // one record per leaf entry: its five loads are independent
+                    const float4* rec = S.kd_rec + (size_t)(nd.z + k) * 5;
+                    const float4 blo = rec[0], bhi = rec[1], c0 = rec[2], c1 = rec[3], c2 = rec[4];
+                    const int32_t e = __float_as_int(blo.w), kind = __float_as_int(bhi.w);
                     const float lo[3] = {blo.x, blo.y, blo.z}, hi[3] = {bhi.x, bhi.y, bhi.z};
                     float a = t0, b = ANY ? t1 : fmin_ref(t1, best);
                     if (!ray_box_ref(o, d, inv, lo, hi, a, b)) continue;
                     if (ANY && e == ignore) continue;
-                    const int4 en = S.entities[e];
-                    if (en.x == C2_SHAPE_TRIANGLE) {
-                        const float* tp = S.tri_pos + (size_t)en.y * 9;
-                        V3 v0{tp[0], tp[1], tp[2]}, v1{tp[3], tp[4], tp[5]}, v2{tp[6], tp[7], tp[8]};
+                    if (kind == C2_SHAPE_TRIANGLE) {
+                        V3 v0{c0.x, c0.y, c0.z}, v1{c0.w, c1.x, c1.y}, v2{c1.z, c1.w, c2.x};
                         if (ANY) {
                             if (tri_test(o, d, v0, v1, v2, t0, t1)) {
                                 hit.entity = e;
@@ -592,8 +592,8 @@ __device__ __forceinline__ bool kd_traverse(const DevScene& S, int32_t root, con
                             }
                         }
                     } else {
-                        const float* sp3 = S.spheres + (size_t)en.y * 22;
-                        const bool disk = en.x == C2_SHAPE_DISK;
+                        const float* sp3 = S.spheres + (size_t)__float_as_int(c0.x) * 22;
+                        const bool disk = kind == C2_SHAPE_DISK;
                         if (ANY) {
                             if (disk ? disk_test(sp3, o, d, t0, t1) : sphere_test(sp3, o, d, t0, t1)) {
                                 hit.entity = e;

@@ -1091,8 +1091,8 @@ struct SceneGPU {
     // KD-tree accelerator (built instead of being traversed by the BVH when kd is set)
     bool kd = false;
     std::vector<int4> kd_nodes{make_int4(-1, 0, 0, 0)};  // node 0: "none"
-    std::vector<int32_t> kd_objs;
-    std::vector<float4> kd_box;
+    std::vector<float4> kd_rec;  // per leaf entry: 5 float4 (see DevScene::kd_rec)
+    int64_t kd_refs = 0;
     std::vector<KdMaster> kdm;  // per scene: root and union box
     double build_seconds = 0;
 
@@ -1136,8 +1136,7 @@ struct SceneGPU {
         dev.perm_off = arena.upload(halton_perm_offsets());
         if (kd) {
             dev.kd_nodes = arena.upload(kd_nodes);
-            dev.kd_objs = arena.upload(kd_objs.empty() ? std::vector<int32_t>{0} : kd_objs);
-            dev.kd_box = arena.upload(kd_box.empty() ? std::vector<float4>{make_float4(0, 0, 0, 0)} : kd_box);
+            dev.kd_rec = arena.upload(kd_rec.empty() ? std::vector<float4>(5, make_float4(0, 0, 0, 0)) : kd_rec);
         }
         if (!tex.empty()) {
             dev.texels = arena.upload(texels);
@@ -1215,7 +1214,7 @@ struct SceneGPU {
             build_kdtree(boxes, kt);
             build_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - k0).count();
             if (kt.depth > 20) fail(C2_ERR_INVALID, "KD-tree deeper than its limit");
-            const int32_t nb = (int32_t)kd_nodes.size() - 1, ob = (int32_t)kd_objs.size();  // node 0 is shared
+            const int32_t nb = (int32_t)kd_nodes.size() - 1, ob = (int32_t)kd_refs;  // node 0 is shared
             for (size_t i = 1; i < kt.nodes.size(); i++) {
                 const KdNode& nd = kt.nodes[i];
                 int32_t sb;
@@ -1223,11 +1222,26 @@ struct SceneGPU {
                 if (nd.axis < 0) kd_nodes.push_back(make_int4(-1, 0, nd.a + ob, nd.b));
                 else kd_nodes.push_back(make_int4(nd.axis, sb, nd.a ? nd.a + nb : 0, nd.b ? nd.b + nb : 0));
             }
-            for (int32_t e : kt.objs) kd_objs.push_back(e + o.ent);
-            for (const Box3& b : boxes) {
-                kd_box.push_back(make_float4(b.lo[0], b.lo[1], b.lo[2], 0.f));
-                kd_box.push_back(make_float4(b.hi[0], b.hi[1], b.hi[2], 0.f));
+            // leaf entries as self-contained records: the entity's box (the reference's
+            // GetBoundingBox, as the tree was built on) with its id and kind, then the
+            // triangle's vertices or the sphere / disk record index
+            for (int32_t e : kt.objs) {
+                const Box3& b = boxes[e];
+                const int k = le[e].x, sh = le[e].y;
+                kd_rec.push_back(make_float4(b.lo[0], b.lo[1], b.lo[2], __int_as_float_host(e + o.ent)));
+                kd_rec.push_back(make_float4(b.hi[0], b.hi[1], b.hi[2], __int_as_float_host(k)));
+                if (k == C2_SHAPE_TRIANGLE) {
+                    const float* v = &tp[(size_t)sh * 9];
+                    kd_rec.push_back(make_float4(v[0], v[1], v[2], v[3]));
+                    kd_rec.push_back(make_float4(v[4], v[5], v[6], v[7]));
+                    kd_rec.push_back(make_float4(v[8], 0.f, 0.f, 0.f));
+                } else {
+                    kd_rec.push_back(make_float4(__int_as_float_host(sh + o.sph), 0.f, 0.f, 0.f));
+                    kd_rec.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+                    kd_rec.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+                }
             }
+            kd_refs += (int64_t)kt.objs.size();
             KdMaster m;
             for (int c = 0; c < 3; c++) {
                 m.lo[c] = kt.master.lo[c];
