@@ -1,19 +1,24 @@
 // gfx950 kernels and the device half of the C-ABI (include/crystal2.h):
 // Accelerator (4-wide BVH closest / any hit) and the wavefront PathTracer + MIS.
 //
-// Wavefront pass for Halton sample k over a chunk of pixels (paths in flight):
+// Wavefront chunk: Halton samples k .. k + nsamp - 1 of a range of pixels (paths in
+// flight; the single renderer puts four samples of every pixel in one chunk):
 //   generate  camera ray + sampler state           SamplerIntegrator.cpp:42-52, PerspCamera.cpp:23-29
 //   per bounce b < Depth:
 //     extend   closest hit of every queued path     BVH::Intersect
-//     shade    emission, BSDF, NEE light samples -> shadow slots, BSDF-sampled MIS
-//              ray (one per vertex, see below), continuation + Russian roulette
-//                                                   PathTracingIntegrator.cpp:36-109,175-278
+//     (b >= 1: the queue grouped by shading class, k_class_count / k_class_scatter)
+//     shade    emission, BSDF, BSDF-sampled MIS direction (one per vertex, see below),
+//              continuation + Russian roulette      PathTracingIntegrator.cpp:36-109
+//     nee      light samples -> shadow slots, and whether the MIS ray is needed
+//                                                   PathTracingIntegrator.cpp:175-278
 //     occlude  any hit of every shadow ray          BVH::IntersectTest
-//     mis      closest hit of every MIS ray
+//     mis      closest hit of every MIS ray (side stream, beside occlude)
 //     resolve  L += beta * sum over lights (in the reference's light order) of the
 //              unshadowed light samples and the MIS term, then beta <- beta'
-//   accumulate film += L (NaN/Inf -> 0), one sample per pixel per pass, in sample
-//              order                                FrameBuffer.cpp:20-32
+//   accumulate film += L (NaN/Inf -> 0), one thread per pixel adding its samples in
+//              sample order                         FrameBuffer.cpp:20-32
+// Consecutive chunks alternate between two lanes (path buffers + stream each); the
+// film adds stay in chunk order through an event chain (enqueue_passes).
 //
 // MIS ray: every supported material has a single BxDF, so the BSDF-sampled
 // direction in EsimateDirect (PathTracingIntegrator.cpp:240) is the same for all
