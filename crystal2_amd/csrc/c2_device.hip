@@ -1800,9 +1800,10 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     // default: kSamplesPerChunk samples of the largest pass per launch
     const int64_t dflt = maxPass * kSamplesPerChunk;
     int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : dflt;
-    // keep each lane's path buffers within ~8 GiB
+    // keep the path buffers within ~16 GiB: 8 GiB in each of two lanes, or 16 GiB in a
+    // batch's one lane
     int64_t perPath = (int64_t)P.lmax * (3 * 16 + 1 + 4) + 16 * 14 + 16;
-    int64_t maxChunk = std::max<int64_t>(4096, ((int64_t)8 << 30) / perPath);
+    int64_t maxChunk = std::max<int64_t>(4096, ((int64_t)(per_sample_passes ? 16 : 8) << 30) / perPath);
     chunk = std::max<int64_t>(1, std::min(std::min(chunk, std::max(maxPass, dflt)), maxChunk));
     pt->chunk = pt->chunk_cap = (int32_t)chunk;
     PathBuffers& B = pt->B;
