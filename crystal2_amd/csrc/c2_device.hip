@@ -79,6 +79,10 @@ static_assert(kLanes >= 1 && kLanes <= 4, "1 to 4 lanes");
 #define C2_SAMPLES_PER_CHUNK 4
 #endif
 constexpr int kSamplesPerChunk = C2_SAMPLES_PER_CHUNK;
+#ifndef C2_BATCH_SAMPLES_PER_CHUNK
+#define C2_BATCH_SAMPLES_PER_CHUNK 16
+#endif
+constexpr int kBatchSamplesPerChunk = C2_BATCH_SAMPLES_PER_CHUNK;  // a batch (one lane)
 // Shading classes (k_class_count) and the first bounce whose queue is grouped by them
 constexpr int kClasses = 32;
 #ifndef C2_SORT_FROM
@@ -1969,7 +1973,8 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
             // view's spp, so an equal view count means the same set.
             int ns = 1;
             if (P.cache_bounce < 0 && pt->cap_bounce < 0 && npix > 0) {
-                int64_t lim = std::min<int64_t>({pt->chunk / npix, (int64_t)sample_end - k, (int64_t)kSamplesPerChunk});
+                const int64_t spc = per_sample_passes ? kBatchSamplesPerChunk : kSamplesPerChunk;
+                int64_t lim = std::min<int64_t>({pt->chunk / npix, (int64_t)sample_end - k, spc});
                 if (per_sample_passes)
                     for (int64_t j = 1; j < lim; j++)
                         if (pt->passes[k + j].npv != tab.npv) {
