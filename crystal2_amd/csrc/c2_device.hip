@@ -83,6 +83,10 @@ constexpr int kSamplesPerChunk = C2_SAMPLES_PER_CHUNK;
 #define C2_BATCH_SAMPLES_PER_CHUNK 16
 #endif
 constexpr int kBatchSamplesPerChunk = C2_BATCH_SAMPLES_PER_CHUNK;  // a batch (one lane)
+#ifndef C2_DIRECT_SAMPLES_PER_CHUNK
+#define C2_DIRECT_SAMPLES_PER_CHUNK 8
+#endif
+constexpr int kDirectSamplesPerChunk = C2_DIRECT_SAMPLES_PER_CHUNK;  // direct lighting (short paths)
 // Shading classes (k_class_count) and the first bounce whose queue is grouped by them
 constexpr int kClasses = 32;
 #ifndef C2_SORT_FROM
@@ -1801,8 +1805,10 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     }
     int64_t maxPass = 0;
     for (auto& t : pt->passes) maxPass = std::max(maxPass, t.npix);
-    // default: kSamplesPerChunk samples of the largest pass per launch
-    const int64_t dflt = maxPass * kSamplesPerChunk;
+    // default: the samples per launch of the largest pass (enqueue_passes)
+    const int64_t dflt = maxPass * (per_sample_passes                          ? kBatchSamplesPerChunk
+                                    : opts->integrator == C2_INTEGRATOR_DIRECT ? kDirectSamplesPerChunk
+                                                                               : kSamplesPerChunk);
     int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : dflt;
     // keep the path buffers within ~16 GiB: 8 GiB in each of two lanes, or 16 GiB in a
     // batch's one lane
@@ -1973,7 +1979,9 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
             // view's spp, so an equal view count means the same set.
             int ns = 1;
             if (P.cache_bounce < 0 && pt->cap_bounce < 0 && npix > 0) {
-                const int64_t spc = per_sample_passes ? kBatchSamplesPerChunk : kSamplesPerChunk;
+                const int64_t spc = per_sample_passes                        ? kBatchSamplesPerChunk
+                                    : P.integrator == C2_INTEGRATOR_DIRECT ? kDirectSamplesPerChunk
+                                                                           : kSamplesPerChunk;
                 int64_t lim = std::min<int64_t>({pt->chunk / npix, (int64_t)sample_end - k, spc});
                 if (per_sample_passes)
                     for (int64_t j = 1; j < lim; j++)
