@@ -403,7 +403,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              f"(the path shards by Halton sample range, see DESIGN.md)", file=sys.stderr)
+              f"(several GPUs would each render a Halton sample range and sum the films)", file=sys.stderr)
         sys.exit(2)
 
     import crystal2_amd as c2
