@@ -13,6 +13,9 @@
 //  * Halton dims >= 100 (HaltonSampler.cpp:285-288 draw from a shared
 //    std::mt19937) and Blinn-Phong rejection redraws
 //    (BlinnPhongReflection.cpp:31-53) use a 32-bit hash of (index, stream).
+//    With orc_scene_set_reference_draws(s, 1) and one thread the oracle instead
+//    continues the reference's own two generators in the reference's order
+//    (SeqDraws below), which reproduces a "Threads": 1 reference render.
 //  * Sphere uv (Sphere.cpp:54-55) is not computed: only image textures read it.
 //  * Glass refraction that fails after FresnelDielectric reported no total
 //    internal reflection (rounding) leaves the direction (0,0,0) instead of an
@@ -34,6 +37,24 @@
 namespace {
 
 thread_local std::string g_err;
+
+// The reference's two file-static, default-seeded std::mt19937 generators:
+// HaltonSampler.cpp:116 (consumed first by the permutation-table shuffles,
+// :164-178 via :207-211, then by every SampleValue of a dimension >= 100,
+// :285-288) and BlinnPhongReflection.cpp:31 (the rejection redraws, :45,52).
+// A render with "Threads": 1 runs its 32x32 tiles one after another in
+// submission order (SamplerIntegrator.cpp:30-67, FixedThreadPool's FIFO), so
+// the sequence of draws is fixed; the oracle reproduces it when it renders on
+// one thread with reference draws on. The two pair draws,
+// Point2f(SampleValue(d), SampleValue(d + 1)) (GlobalSampler.cpp:25-26) and
+// glm::vec2(uniformRandomFloat(mt), uniformRandomFloat(mt))
+// (BlinnPhongReflection.cpp:45,52), evaluate their arguments in the build's
+// order: left to right under clang, right to left under GCC and MSVC.
+struct SeqDraws {
+    std::mt19937 halton, phong;
+    bool rtl = false;
+};
+thread_local SeqDraws* g_seq = nullptr;
 
 // ------------------------------------------------------------------ transcendentals
 #ifdef ORACLE_LIBM
@@ -123,6 +144,8 @@ inline vec3 normalize(vec3 a) { return a * (1.f / std::sqrt(dot(a, a))); }  // :
 inline vec3 reflect(vec3 i, vec3 n) { return i - n * dot(n, i) * 2.f; }     // :104-110
 inline float smax(float a, float b) { return (a < b) ? b : a; }             // std::max
 inline float smin(float a, float b) { return (b < a) ? b : a; }             // std::min
+// uniformRandomFloat, src/Core/Utils.h:12-15 (mt() is 32 bits; the product is float)
+inline float uniformRandomFloat(std::mt19937& mt) { return smin(ONE_MINUS_EPS, (float)mt() * 2.3283064365386963e-10f); }
 inline float clampf(float x, float lo, float hi) { return smin(smax(x, lo), hi); }
 inline vec3 fixVector(vec3 v) {  // src/Core/Utils.h:163-166
     return {clampf(v.x, -1.f, 1.f), clampf(v.y, -1.f, 1.f), clampf(v.z, -1.f, 1.f)};
@@ -788,7 +811,8 @@ struct BxDF {
                 return fblendF(wo, *wi);
             }
             case BX_PHONG: {  // BlinnPhongReflection.cpp:32-58 (deterministic redraws)
-                for (uint32_t redraw = 0; redraw < 1024; redraw++) {
+                // the reference loops until a draw succeeds; the hash stream stops after 1024 redraws
+                for (uint32_t redraw = 0; redraw < (g_seq ? (1u << 24) : 1024u); redraw++) {
                     float cosTheta = (float)o_powd(s.x, 1.0 / (k + 2));
                     float phi = TWO_PI * s.y;
                     vec3 N = unitFromCos(cosTheta, phi);
@@ -798,6 +822,13 @@ struct BxDF {
                         ok = wi->y >= 0;
                     }
                     if (!ok) {
+                        if (g_seq) {  // the reference's generator, in the build's argument order
+                            float a, b;
+                            if (g_seq->rtl) { b = uniformRandomFloat(g_seq->phong); a = uniformRandomFloat(g_seq->phong); }
+                            else { a = uniformRandomFloat(g_seq->phong); b = uniformRandomFloat(g_seq->phong); }
+                            s = vec2(a, b);
+                            continue;
+                        }
                         uint32_t stream = 0x40000000u | (redrawDim << 12) | (redraw << 1);
                         s = vec2(hashFloat(redrawIdx, stream), hashFloat(redrawIdx, stream + 1));
                         continue;
@@ -952,7 +983,7 @@ struct Halton {
     }
     float value(int idx, int dim) const {  // :272-291 + RadicalInverse*, :118-162
         if (dim == 0) return (float)((double)rev64((uint64_t)(int64_t)(idx >> exps[0])) * 0x1p-64);
-        if (dim >= 100) return BxDF::hashFloat((uint32_t)idx, (uint32_t)dim);
+        if (dim >= 100) return g_seq ? uniformRandomFloat(g_seq->halton) : BxDF::hashFloat((uint32_t)idx, (uint32_t)dim);
         int base = kPrimes100[dim];
         float invBase = 1.f / base, result = 0, invN = 1.0f;
         if (dim == 1) {
@@ -982,10 +1013,12 @@ struct Sampler {
     const Halton* h;
     int idx = 0, dim = 0;
     float get1D() { return h->value(idx, dim++); }
-    vec2 get2D() {
-        vec2 s(h->value(idx, dim), h->value(idx, dim + 1));
+    vec2 get2D() {  // GlobalSampler::Get2D, GlobalSampler.cpp:19-29
+        float a, b;
+        if (g_seq && g_seq->rtl) { b = h->value(idx, dim + 1); a = h->value(idx, dim); }
+        else { a = h->value(idx, dim); b = h->value(idx, dim + 1); }
         dim += 2;
-        return s;
+        return vec2(a, b);
     }
 };
 
@@ -1044,6 +1077,8 @@ struct orc_scene {
     int accel = ORC_ACCEL_BVH;
     int tie = ORC_TIE_FIRST;
     int integrator = ORC_INTEGRATOR_PATH;
+    bool refDraws = false;  // SeqDraws instead of the hash streams (one thread only)
+    uint64_t refSkip = 0;   // draws of both generators discarded before the render
     // median-split BVH, src/Accelerators/BVH.cpp:13-34,45-54,156-188,269-355
     struct Node {
         Box box;
@@ -1814,6 +1849,13 @@ int orc_scene_create(const c2_scene_desc* d, int accel, int tie, orc_scene** out
 
 void orc_scene_destroy(orc_scene* s) { delete s; }
 
+int orc_scene_set_reference_draws(orc_scene* s, int enable, uint64_t skip) {
+    if (!s) return setErr("null scene");
+    s->refDraws = enable != 0;
+    s->refSkip = skip;
+    return C2_OK;
+}
+
 int orc_scene_set_integrator(orc_scene* s, int integrator) {
     if (!s || (integrator != ORC_INTEGRATOR_PATH && integrator != ORC_INTEGRATOR_DIRECT)) return setErr("bad integrator");
     s->integrator = integrator;
@@ -1823,6 +1865,18 @@ int orc_scene_set_integrator(orc_scene* s, int integrator) {
 static int renderImpl(orc_scene* s, int W, int H, int spp, int depth, int argOrder, int threads, int y0, int y1,
                       float* film, float* samples, double* seconds) {
     if (!s || W <= 0 || H <= 0 || spp <= 0) return setErr("bad arguments");
+    if (s->refDraws && threads != 1) return setErr("reference draws need one thread");
+    SeqDraws seq;
+    if (s->refDraws) {  // the shuffles of GeneratePermutationTables come first (HaltonSampler.cpp:207-211)
+        for (int i = 0; i < 100; i++) {
+            std::vector<int> v(kPrimes100[i]);
+            for (int j = 0; j < kPrimes100[i]; j++) v[j] = j;
+            std::shuffle(v.begin(), v.end(), seq.halton);
+        }
+        seq.halton.discard(s->refSkip);
+        seq.phong.discard(s->refSkip);
+        seq.rtl = argOrder == 0;
+    }
     y0 = std::max(0, y0);
     y1 = std::min(H, y1);
     Halton halton(W, H);
@@ -1836,6 +1890,7 @@ static int renderImpl(orc_scene* s, int W, int H, int spp, int depth, int argOrd
     int ntx = (W + TILE - 1) / TILE, nty = (H + TILE - 1) / TILE;
     std::atomic<int> next(0);
     auto worker = [&]() {
+        g_seq = s->refDraws ? &seq : nullptr;
         while (true) {
             int task = next.fetch_add(1);
             if (task >= ntx * nty) break;

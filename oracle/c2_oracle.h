@@ -39,6 +39,13 @@ void orc_scene_destroy(orc_scene* s);
  * or ORC_INTEGRATOR_DIRECT (DirectLightingIntegrator). */
 enum { ORC_INTEGRATOR_PATH = 0, ORC_INTEGRATOR_DIRECT = 1 };
 int  orc_scene_set_integrator(orc_scene* s, int integrator);
+/* enable != 0: Halton dims >= 100 and Blinn-Phong redraws come from the
+ * reference's own two default-seeded std::mt19937 generators, consumed in the
+ * order of a one-thread reference render (orc_render with threads = 1 only;
+ * other thread counts fail). 0 (the default): the hash streams the GPU uses.
+ * skip > 0 discards that many draws of each generator first: the render then
+ * stands for a reference render from another point of the same streams. */
+int  orc_scene_set_reference_draws(orc_scene* s, int enable, uint64_t skip);
 const char* orc_last_error(void);
 
 /* Render film rows [y_begin, y_end) (film coordinates, y = 0 at the bottom)

@@ -13,7 +13,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBS = {"portable": os.path.join(HERE, "_build", "libc2oracle.so"),
         "libm": os.path.join(HERE, "_build", "libc2oracle_libm.so")}
-REF_BIN = os.path.join(HERE, "_ref", "crystal2_ref")
+REF_BIN = os.path.join(HERE, "_ref", "crystal2_ref")          # clang build: arguments left to right
+REF_BIN_GCC = os.path.join(HERE, "_ref", "crystal2_ref_gcc")  # GCC build: arguments right to left
 
 ACCEL_BVH, ACCEL_BRUTE, ACCEL_KDTREE = 0, 1, 2
 TIE_FIRST, TIE_LOWEST = 0, 1
@@ -38,6 +39,7 @@ def lib(variant: str = "portable"):
     L.orc_scene_create.argtypes = [C.POINTER(_abi.c2_scene_desc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     L.orc_scene_destroy.argtypes = [C.c_void_p]
     L.orc_scene_set_integrator.argtypes = [C.c_void_p, C.c_int]
+    L.orc_scene_set_reference_draws.argtypes = [C.c_void_p, C.c_int, C.c_uint64]
     L.orc_last_error.restype = C.c_char_p
     L.orc_render.argtypes = [C.c_void_p] + [C.c_int] * 8 + [C.POINTER(C.c_float), C.POINTER(C.c_double)]
     L.orc_render_samples.argtypes = [C.c_void_p] + [C.c_int] * 6 + [C.POINTER(C.c_float)]
@@ -53,7 +55,11 @@ def lib(variant: str = "portable"):
 
 class OracleScene:
     def __init__(self, scene, accel: int = ACCEL_BVH, tie: int = TIE_FIRST, variant: str = "portable",
-                 integrator: str = "path"):
+                 integrator: str = "path", reference_draws: bool = False, skip: int = 0):
+        """reference_draws: Halton dims >= 100 and Blinn-Phong redraws from the
+        reference's own generators in one-thread render order (render with
+        threads=1 only); the default is the hash streams the GPU uses. skip:
+        draws of both generators discarded first (another point of the streams)."""
         self._L = lib(variant)
         self._scene = scene  # keeps the product-side arrays alive
         d = scene.desc()
@@ -63,9 +69,11 @@ class OracleScene:
             raise RuntimeError(self._L.orc_last_error().decode())
         if self._L.orc_scene_set_integrator(self._h, INTEGRATORS[integrator]) != 0:
             raise RuntimeError(self._L.orc_last_error().decode())
+        self.reference_draws = bool(reference_draws)
+        self._L.orc_scene_set_reference_draws(self._h, int(self.reference_draws), int(skip))
 
     def render(self, width, height, spp, depth, arg_order="rtl", threads=None, y_range=None):
-        threads = threads or os.cpu_count() or 1
+        threads = 1 if self.reference_draws else (threads or os.cpu_count() or 1)
         y0, y1 = y_range if y_range else (0, height)
         film = np.zeros((height, width, 3), np.float32)
         secs = C.c_double()

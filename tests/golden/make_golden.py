@@ -3,9 +3,10 @@
 Runs oracle/_ref/crystal2_ref (the reference sources from /root/reference,
 compiled by oracle/Makefile) in THIS container. The fixtures are data only:
 scene files (JSON + OBJ written by crystal2_amd.scenes) and the reference's
-outputs on them. The reference binary is built with clang, which evaluates
-SampleDirection(Get1D(), Get2D()) left to right, so every film here is an
-arg_order="ltr" film.
+outputs on them. oracle/_ref/crystal2_ref is built with clang, which evaluates
+SampleDirection(Get1D(), Get2D()) left to right: its films are arg_order="ltr"
+films. The "_rtl" films come from oracle/_ref/crystal2_ref_gcc, the GCC build
+of the same sources, which evaluates the arguments right to left.
 
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py direct   # DirectLighting films only
@@ -15,6 +16,9 @@ arg_order="ltr" film.
     python tests/golden/make_golden.py textured # textures + skybox scene only
     python tests/golden/make_golden.py hdr      # stbi_write_hdr outputs only
     python tests/golden/make_golden.py kdtree   # "Acceleration": "KDTree" films and hits only
+    python tests/golden/make_golden.py rtl      # GCC build (right-to-left arguments) films only
+    python tests/golden/make_golden.py deep     # one-thread films reaching Halton dims >= 100
+    python tests/golden/make_golden.py frame    # digest of the full 1080p sphere-field frame
 """
 import os
 import subprocess
@@ -28,21 +32,85 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, HERE)
 from crystal2_amd import scenes  # noqa: E402
 
-REF = os.path.join(ROOT, "oracle", "_ref", "crystal2_ref")
+REF = os.path.join(ROOT, "oracle", "_ref", "crystal2_ref")          # clang: arguments left to right
+REF_GCC = os.path.join(ROOT, "oracle", "_ref", "crystal2_ref_gcc")  # GCC: arguments right to left
 SCN = os.path.join(HERE, "scenes")
 
 
-def ref(*args, cwd):
-    out = subprocess.run([REF, *args], cwd=cwd, check=True, capture_output=True, text=True)
+def ref(*args, cwd, binary=REF):
+    out = subprocess.run([binary, *args], cwd=cwd, check=True, capture_output=True, text=True)
     return out.stdout
 
 
-def film(name, w, h, config="config.json", tag="", scene="scene.json"):
-    d = os.path.join(SCN, name)
-    ref("render", config, scene, "film.f32", cwd=d)
+def film(name, w, h, config="config.json", tag="", scene="scene.json", binary=REF, save=True):
+    d = os.path.join(SCN, name) if not os.path.isabs(name) else name
+    ref("render", config, scene, "film.f32", cwd=d, binary=binary)
     f = np.fromfile(os.path.join(d, "film.f32"), np.float32).reshape(h, w, 3)
     os.remove(os.path.join(d, "film.f32"))
-    np.save(os.path.join(HERE, f"film_{name}{tag}.npy"), f)
+    if save:
+        np.save(os.path.join(HERE, f"film_{name}{tag}.npy"), f)
+    return f
+
+
+def _need(target):
+    binary = REF if target == "ref" else REF_GCC
+    if not os.path.exists(binary):
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), target], check=True)
+
+
+def rtl_films():
+    """The GCC build's films (SampleDirection(Get1D(), Get2D()) evaluated right to left, the
+    order GCC and MSVC use) of the golden path-tracing scenes, at their existing configs."""
+    _need("refgcc")
+    for name, cfg, scene in (("cornell", "config.json", "scene.json"), ("zoo_nophong", "config.json", "scene.json"),
+                             ("zoo_extra", "config.json", "scene.json"), ("disks", "config.json", "scene.json"),
+                             ("textured", "config.json", "scene.json")):
+        import json
+        c = json.load(open(os.path.join(SCN, name, cfg)))
+        film(name, c["Width"], c["Height"], config=cfg, tag="_rtl", scene=scene, binary=REF_GCC)
+    print("right-to-left films written to", HERE)
+
+
+def deep_films():
+    """Films whose paths reach Halton dims >= 100 and Blinn-Phong redraws, rendered with
+    "Threads": 1 so that the reference's two shared std::mt19937 generators are consumed in
+    a fixed order: the material zoo (with Phong) at depth 14, and a reduced sphere field
+    (40 level-2 spheres, 4 light quads = 8 light triangles, seed 3) at 96x54, 4 spp, depth
+    8. Both builds (argument orders) each."""
+    _need("ref")
+    _need("refgcc")
+    scenes.write_config(os.path.join(SCN, "zoo", "config_deep.json"), 64, 64, 8, 14, threads=1)
+    d = os.path.join(SCN, "spheres40")
+    scenes.spheres_scene(d, width=96, height=54, spp=4, depth=8, n_spheres=40, level=2, seed=3, threads=1)
+    for binary, order in ((REF, "ltr"), (REF_GCC, "rtl")):
+        film("zoo", 64, 64, config="config_deep.json", tag=f"_deep_{order}", binary=binary)
+        film("spheres40", 96, 54, tag=f"_{order}", binary=binary)
+    print("one-thread deep-path films written to", HERE)
+
+
+def full_frame_digest():
+    """The full sphere-field frame (2,048,010 entities, 1920x1080, depth 8) at Halton sample
+    0, rendered by the GCC build with "Threads": 1. Too large to keep (24.9 MB), so the
+    fixture is its MD5 with per-channel sums, and per-row sums for locating a difference."""
+    import hashlib
+    import json
+    import shutil
+    import tempfile
+    _need("refgcc")
+    tmp = tempfile.mkdtemp(prefix="c2frame")
+    try:
+        scenes.spheres_scene(tmp, width=1920, height=1080, spp=1, depth=8, threads=1)
+        f = film(tmp, 1920, 1080, binary=REF_GCC, save=False)
+    finally:
+        shutil.rmtree(tmp)
+    out = {"scene": "sphere field: 400 level-4 spheres, seed 2024", "width": 1920, "height": 1080, "spp": 1,
+           "depth": 8, "arg_order": "rtl", "threads": 1,
+           "md5": hashlib.md5(np.ascontiguousarray(f, np.float32).tobytes()).hexdigest(),
+           "channel_sums": [float(x) for x in f.astype(np.float64).sum((0, 1))]}
+    with open(os.path.join(HERE, "frame_spheres_1080p_s0.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    np.save(os.path.join(HERE, "frame_spheres_1080p_s0_rows.npy"), f.astype(np.float64).sum(1))
+    print("full-frame digest written to", HERE, out["md5"])
 
 
 def direct_lighting():
@@ -357,7 +425,13 @@ def kdtree():
 
 
 if __name__ == "__main__":
-    if sys.argv[1:] == ["direct"]:
+    if sys.argv[1:] == ["rtl"]:
+        rtl_films()
+    elif sys.argv[1:] == ["deep"]:
+        deep_films()
+    elif sys.argv[1:] == ["frame"]:
+        full_frame_digest()
+    elif sys.argv[1:] == ["direct"]:
         direct_lighting()
     elif sys.argv[1:] == ["extra"]:
         extra_materials()
