@@ -142,6 +142,21 @@ __global__ void __launch_bounds__(kBlock) k_intersect_test(DevScene S, const flo
     }
 }
 
+// Results of rays cast into an empty scene: no hit (prim -1, t +inf), no occluder.
+// Kernels rather than hipMemsetAsync, so callers may capture these entry points into
+// a graph.
+__global__ void k_fill_miss(int32_t* __restrict__ prim, float* __restrict__ tout, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        prim[i] = -1;
+        tout[i] = INFINITY;
+    }
+}
+
+__global__ void k_fill_u8(uint8_t* __restrict__ p, int64_t n, uint8_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 // ============================================================================ wavefront state
 
 enum : int32_t {
@@ -1014,7 +1029,10 @@ struct DeviceArena {
         void* p = nullptr;
         size_t b = std::max<size_t>(count * sizeof(T), 16);
         hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) fail(C2_ERR_NOMEM, "hipMalloc(" + std::to_string(b) + " bytes): " + hipGetErrorString(e));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // clear it: a later launch check must not report this failure
+            fail(C2_ERR_NOMEM, "hipMalloc(" + std::to_string(b) + " bytes): " + hipGetErrorString(e));
+        }
         ptrs.push_back(p);
         bytes += (int64_t)b;
         return static_cast<T*>(p);
@@ -1470,10 +1488,10 @@ int c2_accel_get_stats(const c2_accel* a, c2_accel_stats* out) {
 int c2_accel_intersect_device(c2_accel* a, const float* rays, int64_t n, int32_t* prim, float* t, void* stream) {
     return guard([&] {
         if (!a || n < 0 || (n > 0 && (!rays || !prim || !t))) fail(C2_ERR_INVALID, "bad arguments");
-        if (n == 0 || a->scene.ents.empty()) {
-            if (n > 0) {
-                HIP_CHECK(hipMemsetAsync(prim, 0xff, n * sizeof(int32_t), (hipStream_t)stream));
-            }
+        if (n == 0) return;
+        if (a->scene.ents.empty()) {
+            hipLaunchKernelGGL(k_fill_miss, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, prim, t, n);
+            HIP_CHECK(hipGetLastError());
             return;
         }
         if (a->scene.kd)
@@ -1492,7 +1510,8 @@ int c2_accel_intersect_test_device(c2_accel* a, const float* rays, const int32_t
         if (!a || n < 0 || (n > 0 && (!rays || !hit))) fail(C2_ERR_INVALID, "bad arguments");
         if (n == 0) return;
         if (a->scene.ents.empty()) {
-            HIP_CHECK(hipMemsetAsync(hit, 0, n, (hipStream_t)stream));
+            hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream, hit, n, (uint8_t)0);
+            HIP_CHECK(hipGetLastError());
             return;
         }
         if (a->scene.kd)

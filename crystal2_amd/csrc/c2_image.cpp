@@ -759,6 +759,10 @@ ImageRGB decode_jpeg(const uint8_t* d, size_t n) {
                 j.hmax = std::max(j.hmax, c.h);
                 j.vmax = std::max(j.vmax, c.v);
             }
+            // every component's sampling factors must divide the largest: the upsamplers
+            // expand a component row by hmax / h (and vmax / v), which must give img_x bytes
+            for (auto& c : j.comp)
+                if (j.hmax % c.h || j.vmax % c.v) fail(C2_ERR_PARSE, "JPEG: sampling factors do not divide");
             j.mcux = (j.img_x + j.hmax * 8 - 1) / (j.hmax * 8);
             j.mcuy = (j.img_y + j.vmax * 8 - 1) / (j.vmax * 8);
             for (auto& c : j.comp) {

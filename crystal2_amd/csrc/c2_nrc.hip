@@ -350,7 +350,12 @@ struct NrcArena {
     template <class T>
     T* alloc(size_t n) {
         void* p = nullptr;
-        NRC_CHECK(hipMalloc(&p, std::max<size_t>(n * sizeof(T), 16)));
+        const size_t b = std::max<size_t>(n * sizeof(T), 16);
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // clear it: a later launch check must not report this failure
+            fail(C2_ERR_NOMEM, "hipMalloc(" + std::to_string(b) + " bytes): " + hipGetErrorString(e));
+        }
         ptrs.push_back(p);
         return static_cast<T*>(p);
     }

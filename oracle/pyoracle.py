@@ -73,7 +73,7 @@ class OracleScene:
         self._L.orc_scene_set_reference_draws(self._h, int(self.reference_draws), int(skip))
 
     def render(self, width, height, spp, depth, arg_order="rtl", threads=None, y_range=None):
-        threads = 1 if self.reference_draws else (threads or os.cpu_count() or 1)
+        threads = 1 if self.reference_draws else (threads or min(16, os.cpu_count() or 1))
         y0, y1 = y_range if y_range else (0, height)
         film = np.zeros((height, width, 3), np.float32)
         secs = C.c_double()
@@ -84,7 +84,7 @@ class OracleScene:
         return film, secs.value
 
     def render_samples(self, width, height, spp, depth, arg_order="rtl", threads=None):
-        threads = threads or os.cpu_count() or 1
+        threads = threads or min(16, os.cpu_count() or 1)
         out = np.zeros((height, width, spp, 3), np.float32)
         rc = self._L.orc_render_samples(self._h, width, height, spp, depth, 1 if arg_order == "ltr" else 0, threads,
                                         out.ctypes.data_as(C.POINTER(C.c_float)))

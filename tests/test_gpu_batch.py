@@ -98,3 +98,29 @@ def test_batch_direct_lighting(gpu_available):
     for i, r in enumerate(ref):
         assert np.array_equal(b.film(i), r)
     b.close()
+
+
+def test_batch_64_scene_workload(gpu_available):
+    """The batched-scenes workload at its configured size: 64 sphere-field scenes with
+    triangle counts log-uniform in 1k-200k, resolutions in {128, 256, 512}^2 and spp in
+    {4, 16, 64} (198 M samples), depth 8, autotuned, captured into a hipGraph and replayed
+    twice. Every film is bit-identical to the scene rendered alone."""
+    print("case batch 64-scene workload", flush=True)
+    specs = scenes.batch_specs(64, 6400)
+    items = scenes.build_batch(specs)
+    ref = []
+    for sc, w, h, spp in items:
+        pt = c2.WavefrontPathIntegrator(sc, w, h, spp, 8, arg_order="rtl")
+        pt.render()
+        pt.sync()
+        ref.append(pt.film())
+        pt.close()
+    b = c2.BatchRenderer(items, 8, arg_order="rtl")
+    t = b.autotune()
+    assert t["candidates"] >= 3
+    for rep in range(2):
+        b.render(True)
+        b.sync()
+        bad = [i for i, r in enumerate(ref) if not np.array_equal(b.film(i), r)]
+        assert not bad, f"replay {rep}: films {bad} differ"
+    b.close()

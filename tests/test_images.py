@@ -43,6 +43,18 @@ def test_decode_errors(tmp_path):
     with pytest.raises(c2.Crystal2Error) as e:
         c2.load_image(str(p))
     assert e.value.code == -2
+    # sampling factors that do not divide the largest (h = 4, 3, 1): upsampling by
+    # hmax / h would not fill a row; stb rejects the frame, so does the decoder
+    src = bytearray(open(os.path.join(IMAGES, "j22_r0.jpg"), "rb").read())
+    sof = src.index(bytes([0xFF, 0xC0]))
+    assert src[sof + 9] == 3  # three components follow: id, sampling, table
+    for k, hv in enumerate((0x41, 0x31, 0x11)):
+        src[sof + 11 + 3 * k] = hv
+    p = tmp_path / "sampling.jpg"
+    p.write_bytes(bytes(src))
+    with pytest.raises(c2.Crystal2Error) as e:
+        c2.load_image(str(p))
+    assert e.value.code == -2
     p = tmp_path / "x.gif"
     p.write_bytes(b"GIF89a....")
     with pytest.raises(c2.Crystal2Error) as e:
