@@ -150,6 +150,7 @@ SIGNATURES = [
     ("c2_accel_intersect", C.c_int, [P, _f32p, C.c_int64, _i32p, _f32p]),
     ("c2_accel_intersect_test", C.c_int, [P, _f32p, _i32p, C.c_int64, C.POINTER(C.c_uint8)]),
     ("c2_accel_intersect_device", C.c_int, [P, P, C.c_int64, P, P, P]),
+    ("c2_accel_intersect_noalloc", C.c_int, [P, _f32p, C.c_int64, _i32p, _f32p]),
     ("c2_accel_intersect_test_device", C.c_int, [P, P, P, C.c_int64, P, P]),
     ("c2_accel_count_visits", C.c_int, [P, _f32p, C.c_int64, C.c_int32, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]),

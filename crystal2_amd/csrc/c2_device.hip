@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1555,6 +1556,67 @@ int c2_accel_intersect_test(c2_accel* a, const float* rays, const int32_t* ignor
         if (rc != C2_OK) fail(rc, c2_last_error());
         HIP_CHECK(hipDeviceSynchronize());
         HIP_CHECK(hipMemcpy(hit, dh, n, hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------- staging without allocation
+// c2_accel_intersect_noalloc stages rays and results in this file-scope device array
+// instead of allocating device buffers per call: C2_NOALLOC_MAX_RAYS rays of 32 B in,
+// then 4 B ids and 4 B t out (40 MiB). Each GPU holds its own copy until the process
+// ends. In host code `scratch` only names the symbol; its device address, per device,
+// comes from hipGetSymbolAddress on the first call made with that device current.
+namespace {
+constexpr int64_t kNoallocRays = C2_NOALLOC_MAX_RAYS;
+constexpr size_t kScratchBytes = (size_t)kNoallocRays * (8 * sizeof(float) + sizeof(int32_t) + sizeof(float));
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+unsigned char* scratch_for_current_device() {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) fail(C2_ERR_UNSUPPORTED, "device index beyond the staging table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(C2_ERR_DEVICE, "hipGetSymbolAddress returned no address for the staging array");
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int c2_accel_intersect_noalloc(c2_accel* a, const float* rays, int64_t n, int32_t* prim, float* t) {
+    return guard([&] {
+        if (!a || n < 0 || (n > 0 && (!rays || !prim || !t))) fail(C2_ERR_INVALID, "bad arguments");
+        if (n > kNoallocRays)
+            fail(C2_ERR_INVALID, "c2_accel_intersect_noalloc: " + std::to_string(n) + " rays exceed the staging maximum of " +
+                                     std::to_string(kNoallocRays));
+        if (n == 0) return;
+        if (a->scene.ents.empty()) {  // no device work: every ray misses
+            for (int64_t i = 0; i < n; i++) {
+                prim[i] = -1;
+                t[i] = INFINITY;
+            }
+            return;
+        }
+        unsigned char* base = scratch_for_current_device();
+        float* dr = reinterpret_cast<float*>(base);
+        int32_t* dp = reinterpret_cast<int32_t*>(base + (size_t)n * 8 * sizeof(float));
+        float* dt = reinterpret_cast<float*>(base + (size_t)n * (8 * sizeof(float) + sizeof(int32_t)));
+        HIP_CHECK(hipMemcpy(dr, rays, (size_t)n * 8 * sizeof(float), hipMemcpyHostToDevice));
+        int rc = c2_accel_intersect_device(a, dr, n, dp, dt, nullptr);
+        if (rc != C2_OK) fail(rc, c2_last_error());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(prim, dp, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(t, dt, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
     });
 }
 

@@ -234,6 +234,17 @@ int  c2_accel_intersect_device(c2_accel* accel, const float* rays, int64_t n,
                                int32_t* prim, float* t, void* stream);
 int  c2_accel_intersect_test_device(c2_accel* accel, const float* rays, const int32_t* ignore,
                                     int64_t n, uint8_t* hit, void* stream);
+/* c2_accel_intersect without any device allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (serving loops that cast host
+ * rays repeatedly, code that forbids hipMalloc). Same arguments and results as
+ * c2_accel_intersect; the rays and results are staged in one fixed device array of
+ * the library (40 MiB on each GPU it runs on, held until the process ends), so
+ * n <= C2_NOALLOC_MAX_RAYS (larger n: C2_ERR_INVALID). Synchronous, on the device
+ * current when called (the accelerator's). Calls that overlap on one GPU (several
+ * threads) share that array and race: serialise them. */
+#define C2_NOALLOC_MAX_RAYS (1 << 20)
+int  c2_accel_intersect_noalloc(c2_accel* accel, const float* rays, int64_t n,
+                                int32_t* prim, float* t);
 /* Algorithmic traversal counts on the host binary SAH BVH (32-byte nodes,
  * leaf <= 4), the figure behind "algorithmic bytes per ray":
  *   bytes/ray = 24 + 8 + nodes*32 + tris*36. any_hit selects IntersectTest. */
