@@ -84,7 +84,7 @@ def cpu_baseline(scene, width, height, depth, seconds_budget=20.0, integrator="p
     """Oracle (C++ port of the reference path) on a band of rows at sample 0."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pyoracle
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads, avail, _ = host_cores()
     t0 = time.time()
     orc = pyoracle.OracleScene(scene, variant="portable", integrator=integrator)
     log(f"cpu baseline: oracle scene built in {time.time() - t0:.1f}s, {threads} threads")
@@ -96,7 +96,7 @@ def cpu_baseline(scene, width, height, depth, seconds_budget=20.0, integrator="p
     y0 = max(0, mid - rows // 2)
     _, s = orc.render(width, height, 1, depth, threads=threads, y_range=(y0, y0 + rows))
     samples = rows * width
-    return {"value": samples / s / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+    return {"value": samples / s / 1e6, "unit": "Msamples/s", "cores": threads, "host_cpus": avail, "kind": "port",
             "sample": f"rows {y0}..{y0 + rows} of {width}x{height}, Halton sample 0 ({samples} samples, {s:.1f}s)"}
 
 
@@ -125,7 +125,7 @@ def cpu_reference_baselines(width, height, depth, threads):
                                  capture_output=True, text=True, timeout=600).stdout
             rec = json.loads(res.strip().splitlines()[-1])
             out[key] = {"value": rec["samples"] / rec["render_s"] / 1e6, "unit": "Msamples/s",
-                        "cores": rec["threads"], "kind": "reference",
+                        "cores": rec["threads"], "host_cpus": len(os.sched_getaffinity(0)), "kind": "reference",
                         "sample": f"Halton sample 0 of the full {width}x{height} frame, depth {depth} "
                                   f"({rec['samples']} samples, {rec['render_s']:.1f}s in Integrator::Render)"}
     finally:
@@ -199,15 +199,187 @@ def extras_bench(W, H, depth, steps=16, threads=16, cpu=True):
 
 
 def cpu_model():
-    """The host CPU's model name (first /proc/cpuinfo entry), or None."""
+    """The host CPU as /proc/cpuinfo names it: model name with family / model / stepping
+    (some hosts report only "AMD EPYC"), and the logical CPU count."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                    info[k] = v.strip()
+                if not line.strip() and info:
+                    break
     except OSError:
-        pass
-    return None
+        return None
+    name = info.get("model name")
+    if name is None:
+        return None
+    return (f"{name} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, "
+            f"stepping {info.get('stepping', '?')}; {os.cpu_count()} logical CPUs)")
+
+
+def host_cores():
+    """Threads used for CPU baselines (at most 16: the reference's SamplerIntegrator asserts
+    threads <= 16, SamplerIntegrator.cpp:13) and what the host offers."""
+    avail = len(os.sched_getaffinity(0))
+    return max(1, min(16, avail)), avail, os.cpu_count()
+
+
+def cornell_bench(reps=5):
+    """The Cornell-box workload: 5 wall quads, a 0.5x0.5 light quad (Lemit 15), diffuse and
+    mirror spheres r = 0.3, 256x256, 16 spp, depth 5, FOV 40 (1,048,576 samples). GPU: the
+    full 16-spp render timed `reps` times after a warm-up (median). The reference renderer
+    (GCC build: the right-to-left argument order the GPU renders) renders the same scene
+    files on the host cores; its film is the parity check (relMSE, north star <= 1e-4)."""
+    import shutil
+    import subprocess
+    import tempfile
+    import crystal2_amd as c2
+    from crystal2_amd import scenes
+    W = H = 256
+    spp, depth = 16, 5
+    threads, avail, ncpu = host_cores()
+    d = tempfile.mkdtemp(prefix="c2cornell_")
+    try:
+        scenes.cornell(d, W, H, spp, depth, threads=threads)
+        sc = c2.Scene.load(os.path.join(d, "scene.json"), d)
+        pt = c2.WavefrontPathIntegrator(sc, W, H, spp, depth, arg_order="rtl")
+        pt.render()
+        pt.sync()
+        times = []
+        for _ in range(reps):
+            pt.reset()
+            a = time.perf_counter()
+            pt.render()
+            pt.sync()
+            times.append(time.perf_counter() - a)
+        film = pt.film()
+        pt.close()
+        el = float(np.median(times))
+        out = {"workload": "Cornell box, 256x256, 16 spp, depth 5", "samples": W * H * spp,
+               "msamples_s": round(W * H * spp / el / 1e6, 2), "render_ms": round(el * 1e3, 3)}
+        ref = os.path.join(HERE, "oracle", "_ref", "crystal2_ref_gcc")
+        if os.path.exists(ref):
+            res = subprocess.run([ref, "render", "config.json", "scene.json", "film.f32"], cwd=d, check=True,
+                                 capture_output=True, text=True, timeout=600).stdout
+            r = json.loads(res.strip().splitlines()[-1])
+            rf = np.fromfile(os.path.join(d, "film.f32"), np.float32).reshape(H, W, 3)
+            v = r["samples"] / r["render_s"] / 1e6
+            out["cpu_reference"] = {"value": round(v, 3), "unit": "Msamples/s", "cores": r["threads"],
+                                    "host_cpus": avail, "kind": "reference",
+                                    "sample": f"the full render ({r['samples']} samples, {r['render_s']:.3f}s)"}
+            out["vs_cpu_reference"] = round(out["msamples_s"] / v, 1)
+            out["relmse_vs_reference"] = float(_relmse(film, rf))
+            out["identical_pixels_vs_reference"] = round(float((film == rf).all(-1).mean()), 5)
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def raycast_bench(reps=5, cpu_rays=1 << 21):
+    """The ray-cast workload: a displaced icosphere of 1,310,720 triangles (level 8,
+    radius 1 + 0.1 value noise, seed 7) and 2^24 rays (origins uniform in 1.5x its box,
+    directions uniform on S^2, tmax = inf), resident in HBM. Closest hit and any hit through
+    c2_accel_intersect[_test]_device, each timed `reps` times with HIP events on the launch
+    stream. Achieved GB/s uses the algorithmic bytes per ray (24 + 8 + 32 N_node + 36 N_tri,
+    counted on the binary SAH tree over 65,536 of the rays). The reference's own BVH casts
+    the first `cpu_rays` rays on the host cores (its raycast command); its ids are the
+    agreement check (north star: >= 99.99% identical, t within 1e-5 relative)."""
+    import shutil
+    import subprocess
+    import tempfile
+    import torch
+    import crystal2_amd as c2
+    from crystal2_amd import scenes
+    t0 = time.time()
+    pos, nrm, faces = scenes.displaced_icosphere(8, seed=7)
+    sc = c2.Scene.create()
+    m = sc.add_material(0, [0.5, 0.5, 0.5])
+    sc.add_mesh(pos, nrm, faces, m)
+    n = 1 << 24
+    rays = scenes.icosphere_rays(n, pos, seed=11)
+    bvh = c2.GpuBVH(sc)
+    log(f"raycast: {sc.num_entities} triangles, {n} rays, BVH in {time.time() - t0:.1f}s")
+    sub = rays[np.random.default_rng(0).choice(n, 1 << 16, replace=False)]
+    cn, ct = bvh.count_visits(sub, any_hit=False)
+    an, at = bvh.count_visits(sub, any_hit=True)
+    dr = torch.from_numpy(rays).cuda()
+    dp = torch.empty(n, dtype=torch.int32, device="cuda")
+    dt = torch.empty(n, dtype=torch.float32, device="cuda")
+    dh = torch.empty(n, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        return float(np.median(ms))
+
+    cms = timed(lambda: bvh.intersect_device(dr.data_ptr(), n, dp.data_ptr(), dt.data_ptr(), sp))
+    ams = timed(lambda: bvh.intersect_test_device(dr.data_ptr(), 0, n, dh.data_ptr(), sp))
+    ids, tt, hits = dp.cpu().numpy(), dt.cpu().numpy(), dh.cpu().numpy().astype(bool)
+    out = {"workload": "ray cast: displaced icosphere 1,310,720 tris, 2^24 rays", "rays": n,
+           "closest": {"mrays_s": round(n / cms * 1e-3, 1), "ms": round(cms, 3), "nodes": round(cn, 2),
+                       "tris": round(ct, 2), "bytes_per_ray": round(per_ray_bytes(cn, ct), 1),
+                       "achieved_gbs": round(n * per_ray_bytes(cn, ct) / (cms * 1e-3) / 1e9, 1),
+                       "hit_fraction": round(float((ids >= 0).mean()), 4)},
+           "any": {"mrays_s": round(n / ams * 1e-3, 1), "ms": round(ams, 3), "nodes": round(an, 2),
+                   "tris": round(at, 2), "bytes_per_ray": round(per_ray_bytes(an, at), 1),
+                   "achieved_gbs": round(n * per_ray_bytes(an, at) / (ams * 1e-3) / 1e9, 1)}}
+    for k in ("closest", "any"):
+        out[k]["frac_hbm"] = round(out[k]["achieved_gbs"] / HBM_PEAK_GBS, 4)
+    del dr, dp, dt, dh
+    ref = os.path.join(HERE, "oracle", "_ref", "crystal2_ref")
+    if cpu_rays and os.path.exists(ref):
+        threads, avail, _ = host_cores()
+        d = tempfile.mkdtemp(prefix="c2raycast_")
+        try:
+            scenes.write_obj(os.path.join(d, "mesh.obj"), pos, nrm, faces)
+            with open(os.path.join(d, "scene.json"), "w") as fh:
+                json.dump({"Camera": {"Type": "Normal", "Position": [0, 0, 4], "LookAt": [0, 0, 0], "Up": [0, 1, 0],
+                                      "FOV": 40},
+                           "Scene": {"Textures": {}, "Objects": [{"Type": "Geometry", "Shape": {
+                               "Type": "TriangleMesh", "ObjFile": "mesh.obj", "Translation": [0, 0, 0],
+                               "Scale": [1, 1, 1], "Rotation": [0, 0, 0]},
+                               "Material": {"Type": "Diffuse", "Color": [0.5, 0.5, 0.5]}}]}}, fh)
+            scenes.write_config(os.path.join(d, "config.json"), 64, 64, 1, 1, threads=threads)
+            m_ = min(cpu_rays, n)
+            with open(os.path.join(d, "rays.bin"), "wb") as fh:
+                np.array([m_], np.int64).tofile(fh)
+                rays[:m_].tofile(fh)
+            res = subprocess.run([ref, "raycast", "config.json", "scene.json", "rays.bin", str(threads), "out.bin"],
+                                 cwd=d, check=True, capture_output=True, text=True, timeout=900).stdout
+            r = json.loads(res.strip().splitlines()[-1])
+            raw = open(os.path.join(d, "out.bin"), "rb").read()
+            rid = np.frombuffer(raw, np.int32, m_, 0)
+            rt = np.frombuffer(raw, np.float32, m_, 4 * m_)
+            ra = np.frombuffer(raw, np.uint8, m_, 8 * m_).astype(bool)
+            agree = ids[:m_] == rid
+            both = agree & (rid >= 0)
+            rel = np.abs(tt[:m_][both].astype(np.float64) - rt[both]) / np.maximum(rt[both], 1e-30)
+            out["cpu_reference"] = {
+                "closest_mrays_s": round(m_ / r["closest_s"] / 1e6, 3), "any_mrays_s": round(m_ / r["any_s"] / 1e6, 3),
+                "cores": r["threads"], "host_cpus": avail, "kind": "reference",
+                "sample": f"the first {m_} rays on the reference's median-split BVH"}
+            out["vs_cpu_reference"] = {"closest": round(out["closest"]["mrays_s"] / (m_ / r["closest_s"] / 1e6), 1),
+                                       "any": round(out["any"]["mrays_s"] / (m_ / r["any_s"] / 1e6), 1)}
+            out["agreement_vs_reference"] = {"ids": round(float(agree.mean()), 6),
+                                             "max_rel_dt": float(rel.max()) if rel.size else 0.0,
+                                             "any_hit": round(float((hits[:m_] == ra).mean()), 6)}
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    bvh.close()
+    return out
 
 
 def _crop(film, W, H, win):
@@ -392,6 +564,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched-render comparison")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-direct", action="store_true", help="skip the DirectLighting integrator line")
+    ap.add_argument("--no-cornell", action="store_true", help="skip the Cornell-box (256x256, 16 spp) line")
+    ap.add_argument("--no-raycast", action="store_true", help="skip the ray-cast (2^24 rays) line")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the KD-tree / textures / disk / material-model throughput lines")
     ap.add_argument("--equal-seconds", type=float, default=0.0, help="equal-time budget (0: the timed run's)")
@@ -406,6 +580,12 @@ def main():
               f"(several GPUs would each render a Halton sample range and sum the films)", file=sys.stderr)
         sys.exit(2)
 
+    # torch holds the ray-cast line's device buffers; it must find the GPU before the
+    # library's HIP runtime claims it in this process
+    import torch
+    if not torch.cuda.is_available():
+        log("torch sees no GPU: the ray-cast line will be skipped")
+        args.no_raycast = True
     import crystal2_amd as c2
     W, H, K = args.width, args.height, args.steps
     t0 = time.time()
@@ -416,14 +596,29 @@ def main():
                                     arg_order="rtl", film="f32")
     log(f"renderer + BVH ready in {time.time() - t0:.1f}s")
 
-    # algorithmic bytes per ray from the rays the renderer actually traces (sample 0)
+    # algorithmic bytes per ray from the rays the renderer actually traces: every bounce of
+    # sample 0, each bounce's queue captured and subsampled uniformly, the node and triangle
+    # counts averaged with the bounce's ray count as its weight
     bvh_for_counts = c2.GpuBVH(scene)
     counts = {}
-    for kind, bounce in (("extension", 1), ("shadow", 0)):
-        rays, _ = pt.capture_rays(0, bounce, "shadow" if kind == "shadow" else "extension", n=1 << 14)
-        nodes, tris = bvh_for_counts.count_visits(rays, any_hit=(kind == "shadow"))
+    rng = np.random.default_rng(1)
+    for kind in ("extension", "shadow"):
+        tot = wn = wt = 0.0
+        per_bounce = []
+        for bounce in range(args.depth):
+            rays, _ = pt.capture_rays(0, bounce, "shadow" if kind == "shadow" else "extension", n=1 << 23)
+            if len(rays) == 0:
+                continue
+            sub = rays[rng.choice(len(rays), min(len(rays), 8192), replace=False)]
+            nodes, tris = bvh_for_counts.count_visits(sub, any_hit=(kind == "shadow"))
+            tot += len(rays)
+            wn += nodes * len(rays)
+            wt += tris * len(rays)
+            per_bounce.append((bounce, len(rays), round(nodes, 1)))
+        nodes, tris = wn / max(tot, 1), wt / max(tot, 1)
         counts[kind] = (nodes, tris, per_ray_bytes(nodes, tris))
-        log(f"{kind} rays: {nodes:.1f} node visits, {tris:.1f} tri tests -> {counts[kind][2]:.0f} B/ray")
+        log(f"{kind} rays (all bounces of sample 0, (bounce, rays, nodes)): {per_bounce} -> {nodes:.1f} node "
+            f"visits, {tris:.1f} tri tests -> {counts[kind][2]:.0f} B/ray")
     bvh_for_counts.close()
     pt.reset()
 
@@ -481,13 +676,29 @@ def main():
     achieved = bytes_per_launch / avg_s / 1e9
     kname = {"extend": "k_extend", "occlude": "k_occlude"}[dom]
     traffic = load_traffic(kname)
+    # The path has no dense contraction: HBM is the roofline that applies. What the kernel
+    # actually moves through HBM (the committed FETCH_SIZE + WRITE_SIZE summary) over its
+    # launch time tells whether it is at that roof or limited below it by memory latency
+    # and address issue (the upper tree levels stay in L2 / MALL).
+    measured = traffic / avg_s / 1e9 if traffic else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
-                "bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(avg_s * 1e3, 4)}
+                "bytes_per_launch": round(bytes_per_launch), "avg_launch_ms": round(avg_s * 1e3, 4),
+                "measured_hbm_gbs": round(measured, 1) if measured else None,
+                "measured_hbm_frac": round(measured / HBM_PEAK_GBS, 4) if measured else None,
+                "limiter": (None if measured is None else "hbm bandwidth" if measured >= 0.6 * HBM_PEAK_GBS
+                            else "memory latency and address issue (HBM traffic below 60% of peak)")}
 
     pt.close()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+    cornell = ray_cast = None
+    if not args.no_cornell:
+        cornell = cornell_bench()
+        log(f"cornell: {cornell}")
+    if not args.no_raycast:
+        ray_cast = raycast_bench(cpu_rays=0 if args.no_cpu_baseline else 1 << 21)
+        log(f"raycast: {ray_cast}")
     eq = None
     if not args.no_cache:
         eq = equal_time(scene, W, H, args.depth, seconds=args.equal_seconds or elapsed, judge_spp=args.judge_spp)
@@ -499,7 +710,7 @@ def main():
     refs = None
     if not args.no_cpu_baseline:
         try:
-            refs = cpu_reference_baselines(W, H, args.depth, max(1, min(16, len(os.sched_getaffinity(0)))))
+            refs = cpu_reference_baselines(W, H, args.depth, host_cores()[0])
         except Exception as e:
             log(f"reference renderer unavailable: {e}")
 
@@ -514,7 +725,7 @@ def main():
     extras = None
     if not args.no_extras:
         try:
-            extras = extras_bench(W, H, args.depth, threads=max(1, min(16, len(os.sched_getaffinity(0)))),
+            extras = extras_bench(W, H, args.depth, threads=host_cores()[0],
                                   cpu=not args.no_cpu_baseline)
         except Exception as e:
             log(f"extras unavailable: {e}")
@@ -562,6 +773,8 @@ def main():
         "cpu_port": cpu_port,
         "relmse_equal_time": eq,
         "batch": bat,
+        "cornell": cornell,
+        "raycast": ray_cast,
         "direct_lighting": dl,
         "extras": extras,
     }

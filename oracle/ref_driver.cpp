@@ -18,6 +18,13 @@
 //                 (src/Core/Scene.cpp:50-53), n float32 t, n float32 hit
 //                 position xyz x3 ... see write order below, and n uint8 from
 //                 Scene::IntersectTest(ray, tmin, tmax, nullptr).
+//   crystal2_ref raycast <config.json> <scene.json> <rays.bin> <threads> <out.bin>
+//       The ray-cast timing: Scene::Intersect over all rays, then
+//       Scene::IntersectTest(ray, tmin, tmax, nullptr), each split into equal
+//       contiguous ranges over <threads> std::threads (the reference calls both
+//       concurrently from its tile pool) and timed with steady_clock.
+//       out.bin: n int32 entity index (-1 = miss), n float32 t, n uint8 any hit.
+//       Prints {"closest_s":..,"any_s":..,"rays":..,"threads":..,"hw_threads":..}.
 //   crystal2_ref halton <W> <H> <spp> <queries.bin> <ndims> <out.bin>
 //       queries.bin: int64 n, then n x 3 int32 (x, y, k). For each query:
 //       StartPixel((x,y)); idx = GetVectorIndexForSample(k)
@@ -136,6 +143,61 @@ int cmdHits(int argc, char** argv) {
     return 0;
 }
 
+int cmdRaycast(int argc, char** argv) {
+    if (argc < 7) return 2;
+    Loaded L = load(argv[2], argv[3]);
+    auto objects = L.scene->GetObjects();
+    std::unordered_map<const crystal::Entity*, int> index;
+    for (size_t i = 0; i < objects.size(); i++) index[objects[i]] = (int)i;
+    FILE* f = std::fopen(argv[4], "rb");
+    if (!f) return 3;
+    long long n = 0;
+    if (std::fread(&n, sizeof(n), 1, f) != 1) return 3;
+    std::vector<float> rays((size_t)n * 8);
+    if (std::fread(rays.data(), sizeof(float), rays.size(), f) != rays.size()) return 3;
+    std::fclose(f);
+    const int nt = std::max(1, std::atoi(argv[5]));
+    std::vector<int> id((size_t)n, -1);
+    std::vector<float> t((size_t)n, 0.f);
+    std::vector<unsigned char> any((size_t)n, 0);
+    auto run = [&](bool anyHit) {
+        std::vector<std::thread> pool;
+        for (int w = 0; w < nt; w++)
+            pool.emplace_back([&, w] {
+                long long a = n * w / nt, b = n * (w + 1) / nt;
+                for (long long i = a; i < b; i++) {
+                    const float* r = &rays[(size_t)i * 8];
+                    Ray3f ray(glm::vec3(r[0], r[1], r[2]), glm::vec3(r[3], r[4], r[5]));
+                    if (anyHit) {
+                        any[i] = L.scene->IntersectTest(ray, r[6], r[7], nullptr) ? 1 : 0;
+                    } else {
+                        SurfaceInteraction isec;
+                        if (L.scene->Intersect(ray, &isec) && isec.GetHitEntity()) {
+                            id[i] = index.at(isec.GetHitEntity());
+                            t[i] = isec.GetDistance();
+                        }
+                    }
+                }
+            });
+        for (auto& th : pool) th.join();
+    };
+    auto t0 = std::chrono::steady_clock::now();
+    run(false);
+    auto t1 = std::chrono::steady_clock::now();
+    run(true);
+    auto t2 = std::chrono::steady_clock::now();
+    f = std::fopen(argv[6], "wb");
+    if (!f) return 3;
+    std::fwrite(id.data(), sizeof(int), id.size(), f);
+    std::fwrite(t.data(), sizeof(float), t.size(), f);
+    std::fwrite(any.data(), 1, any.size(), f);
+    std::fclose(f);
+    std::printf("{\"closest_s\": %.6f, \"any_s\": %.6f, \"rays\": %lld, \"threads\": %d, \"hw_threads\": %u}\n",
+                std::chrono::duration<double>(t1 - t0).count(), std::chrono::duration<double>(t2 - t1).count(), n, nt,
+                std::thread::hardware_concurrency());
+    return 0;
+}
+
 int cmdHalton(int argc, char** argv) {
     if (argc < 8) return 2;
     int W = std::atoi(argv[2]), H = std::atoi(argv[3]), spp = std::atoi(argv[4]);
@@ -195,13 +257,14 @@ int cmdDecode(int argc, char** argv) {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: crystal2_ref render|hits|halton|decode|writehdr ...\n");
+        std::fprintf(stderr, "usage: crystal2_ref render|hits|raycast|halton|decode|writehdr ...\n");
         return 2;
     }
     std::string cmd = argv[1];
     int rc = 2;
     if (cmd == "render") rc = cmdRender(argc, argv);
     else if (cmd == "hits") rc = cmdHits(argc, argv);
+    else if (cmd == "raycast") rc = cmdRaycast(argc, argv);
     else if (cmd == "halton") rc = cmdHalton(argc, argv);
     else if (cmd == "decode") rc = cmdDecode(argc, argv);
     else if (cmd == "writehdr") rc = cmdWriteHdr(argc, argv);
