@@ -392,11 +392,15 @@ def _relmse(x, r):
     return float(np.mean((x.astype(np.float64) - r) ** 2 / (r.astype(np.float64) ** 2 + 1e-2)))
 
 
-def equal_time(scene, W, H, depth, seconds, judge_spp=4096, crop=256, cache_bounce=2, train_ratio=64,
-               warmup_passes=8):
+def equal_time(scene, W, H, depth, seconds=0.0, judge_spp=4096, crop=256, cache_bounce=2, train_ratio=48,
+               warmup_passes=8, budget_passes=64):
     """relMSE at equal wall time: plain PT vs radiance-cache PT, both judged against a
     plain render of `judge_spp` samples with an FP64 film on a centre crop. The judge uses
-    Halton samples [judge_offset, judge_offset + judge_spp), the runs samples from 0."""
+    Halton samples [judge_offset, judge_offset + judge_spp), the runs samples from 0.
+    The budget is fixed by the workload, not by the headline's step count: the wall time of
+    `budget_passes` plain passes rendered in one call (or `seconds` when given). The cache
+    run's `warmup_passes` training passes (samples outside both runs) count in its time.
+    train_ratio 48 trains about 2^16 path vertices per pass at 1920x1080."""
     import crystal2_amd as c2
     win = ((W - crop) // 2, (H - crop) // 2, crop, crop)
     judge_offset = 8192
@@ -428,8 +432,18 @@ def equal_time(scene, W, H, depth, seconds, judge_spp=4096, crop=256, cache_boun
                 curve.append((round(used, 4), k - start, _relmse(_crop(pt.film(), W, H, win), ref)))
         return curve
 
-    cps = {1, 2, 4, 8, 16, 32, 64, 128, 256, 512}
+    cps = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 40, 48, 56, 64, 80, 96, 112, 128, 160, 192, 224, 256, 320, 384, 448,
+           512}
     plain = c2.WavefrontPathIntegrator(scene, W, H, 4096, depth, film="f32")
+    if seconds <= 0:  # the fixed budget: budget_passes plain passes in one call (after a warm-up)
+        plain.render(0, 4)
+        plain.sync()
+        plain.reset()
+        a = time.perf_counter()
+        plain.render(0, budget_passes)
+        plain.sync()
+        seconds = time.perf_counter() - a
+        plain.reset()
     plain_curve = run(plain, seconds, cps)
     plain.close()
     lo, hi = scene_bounds(scene)
@@ -456,6 +470,7 @@ def equal_time(scene, W, H, depth, seconds, judge_spp=4096, crop=256, cache_boun
     train_tflops = st["train_rows"] * 3 * flop_q / max(ct["train_ms"], 1e-9) / 1e9
     return {
         "seconds": seconds,
+        "budget": (f"the wall time of {budget_passes} plain passes in one call" if budget_passes else "given"),
         "judge": {"spp": judge_spp, "film": "f64", "window": list(win), "samples": [judge_offset,
                                                                                   judge_offset + judge_spp]},
         "plain": {"passes": plain_curve[-1][1], "relmse": plain_curve[-1][2], "curve": plain_curve},
@@ -568,7 +583,8 @@ def main():
     ap.add_argument("--no-raycast", action="store_true", help="skip the ray-cast (2^24 rays) line")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the KD-tree / textures / disk / material-model throughput lines")
-    ap.add_argument("--equal-seconds", type=float, default=0.0, help="equal-time budget (0: the timed run's)")
+    ap.add_argument("--equal-seconds", type=float, default=0.0,
+                    help="equal-time budget in seconds (0: the wall time of 64 plain passes)")
     ap.add_argument("--judge-spp", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -701,7 +717,7 @@ def main():
         log(f"raycast: {ray_cast}")
     eq = None
     if not args.no_cache:
-        eq = equal_time(scene, W, H, args.depth, seconds=args.equal_seconds or elapsed, judge_spp=args.judge_spp)
+        eq = equal_time(scene, W, H, args.depth, seconds=args.equal_seconds, judge_spp=args.judge_spp)
         log(f"equal time {eq['seconds']:.2f}s: plain relMSE {eq['plain']['relmse']:.4e} ({eq['plain']['passes']} "
             f"passes), cache relMSE {eq['cache']['relmse']:.4e} ({eq['cache']['passes']} passes); "
             f"cache reaches plain relMSE at {eq['cache_time_to_plain_relmse_frac']} of the time; "

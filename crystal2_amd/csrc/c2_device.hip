@@ -88,6 +88,13 @@ constexpr int kBatchSamplesPerChunk = C2_BATCH_SAMPLES_PER_CHUNK;  // a batch (o
 #define C2_DIRECT_SAMPLES_PER_CHUNK 8
 #endif
 constexpr int kDirectSamplesPerChunk = C2_DIRECT_SAMPLES_PER_CHUNK;  // direct lighting (short paths)
+// With the radiance cache: samples per launch, each with its own training pixels; the
+// launch's training rows are split into one Adam step per sample, taken in order
+// after the launch (the next launch's queries see all of them).
+#ifndef C2_CACHE_SAMPLES_PER_CHUNK
+#define C2_CACHE_SAMPLES_PER_CHUNK 4
+#endif
+constexpr int kCacheSamplesPerChunk = C2_CACHE_SAMPLES_PER_CHUNK;
 // Shading classes (k_class_count) and the first bounce whose queue is grouped by them
 constexpr int kClasses = 32;
 #ifndef C2_SORT_FROM
@@ -242,8 +249,8 @@ struct PassParams {
     const int2* pv;          // views of this pass: (view, first linear index), ascending
     int32_t npv;
     int32_t cache_bounce;            // radiance cache: -1 off
-    int32_t train_ratio, train_phase;
-    int32_t train_cap;               // training rows per step
+    int32_t train_ratio;
+    int32_t train_cap;               // training rows per Adam step (the launch collects nsamp x this)
     int32_t train_paths;             // training path record capacity
 };
 
@@ -374,7 +381,11 @@ __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, 
         const int idxb = lin - band * V.win_w * 8;
         int x = V.win_x0 + idxb / hb, y = V.win_y0 + band * 8 + idxb % hb;
         int pixel = y * V.width + x;
-        bool train = live && P.cache_bounce >= 0 && (pixel % P.train_ratio) == P.train_phase;
+        // training pixels: a phase per sample, so the samples of one launch train different pixels
+        const int phase = P.cache_bounce >= 0
+                              ? (int)(((uint32_t)(P.sample + js) * 2654435761u) % (uint32_t)P.train_ratio)
+                              : 0;
+        bool train = live && P.cache_bounce >= 0 && (pixel % P.train_ratio) == phase;
         int tp = P.cache_bounce >= 0 ? block_append(&B.counters[5], train, sm) : -1;  // uniform condition
         if (tp >= P.train_paths) train = false;  // beyond the record capacity: an ordinary path
         if (!live) continue;
@@ -742,7 +753,7 @@ __global__ void __launch_bounds__(kBlock) k_cache_collect(PathBuffers B, PassPar
             float t2 = be.z > 0.f ? (Lf.z - lp.z) / be.z : 0.f;
             if (!(isfinite(t0) && isfinite(t1) && isfinite(t2))) continue;
             int slot = atomicAdd(&B.counters[7], 1);
-            if (slot >= P.train_cap) continue;
+            if (slot >= P.train_cap * P.nsamp) continue;
             for (int k = 0; k < 4; k++) B.bfeat[(int64_t)slot * 4 + k] = rec[k];
             B.btarget[slot] = make_float4(fmaxf(t0, 0.f), fmaxf(t1, 0.f), fmaxf(t2, 0.f), 0.f);
         }
@@ -1705,8 +1716,8 @@ struct c2_pt {
 namespace c2 {
 void nrc_launch_infer(c2_nrc* c, const int32_t* list, const int32_t* countp, int32_t maxn, const float4* feats,
                       float4* L, const float4* beta, hipStream_t s);
-void nrc_launch_train(c2_nrc* c, const float4* feats, const float4* targets, const int32_t* countp, int32_t maxn,
-                      hipStream_t s, bool adam);
+void nrc_launch_train(c2_nrc* c, const float4* feats, const float4* targets, const int32_t* countp, int32_t js,
+                      int32_t ns, int32_t maxn, hipStream_t s, bool adam);
 }  // namespace c2
 
 extern "C" {
@@ -1803,7 +1814,6 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     P.cache_bounce = -1;
     P.nsamp = 1;
     P.train_ratio = 1;
-    P.train_phase = 0;
     P.train_cap = 0;
     P.train_paths = 0;
     std::vector<int32_t> elig;
@@ -2059,8 +2069,9 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
             // k .. k + ns - 1 must have the same views: the sets shrink as k passes each
             // view's spp, so an equal view count means the same set.
             int ns = 1;
-            if (P.cache_bounce < 0 && pt->cap_bounce < 0 && npix > 0) {
+            if (pt->cap_bounce < 0 && npix > 0) {
                 const int64_t spc = per_sample_passes                        ? kBatchSamplesPerChunk
+                                    : P.cache_bounce >= 0                   ? kCacheSamplesPerChunk
                                     : P.integrator == C2_INTEGRATOR_DIRECT ? kDirectSamplesPerChunk
                                                                            : kSamplesPerChunk;
                 int64_t lim = std::min<int64_t>({pt->chunk / npix, (int64_t)sample_end - k, spc});
@@ -2077,7 +2088,6 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
             P.npv = tab.npv;
             P.sample = k;
             P.nsamp = ns;
-            if (P.cache_bounce >= 0) P.train_phase = (int32_t)(((uint32_t)k * 2654435761u) % (uint32_t)P.train_ratio);
             for (int64_t beg = 0; beg < npix; beg += step) {
                 P.pix_begin = (int32_t)beg;
                 P.pix_count = (int32_t)std::min<int64_t>(step, npix - beg);
@@ -2141,7 +2151,7 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                             HIP_CHECK(hipEventCreate(&e0));
                             HIP_CHECK(hipEventCreate(&e1));
                             HIP_CHECK(hipEventRecord(e0, s));
-                            nrc_launch_infer(pt->nrc, LB.nrc_list, LB.counters + 4, P.pix_count, LB.nrc_feat,
+                            nrc_launch_infer(pt->nrc, LB.nrc_list, LB.counters + 4, (int32_t)npath, LB.nrc_feat,
                                              LB.L, LB.beta, s);
                             HIP_CHECK(hipEventRecord(e1, s));
                             pt->cache_events.push_back({0, {e0, e1}});
@@ -2193,7 +2203,10 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                     HIP_CHECK(hipEventCreate(&e1));
                     HIP_CHECK(hipEventRecord(e0, s));
                     hipLaunchKernelGGL(k_cache_collect, dim3(grid_for(P.train_paths)), dim3(kBlock), 0, s, LB, P);
-                    nrc_launch_train(pt->nrc, LB.bfeat, LB.btarget, LB.counters + 7, P.train_cap, s, true);
+                    // one Adam step per sample of the launch, each on an equal slice of the rows
+                    for (int js = 0; js < ns; js++)
+                        nrc_launch_train(pt->nrc, LB.bfeat, LB.btarget, LB.counters + 7, js, ns, P.train_cap, s,
+                                         true);
                     HIP_CHECK(hipEventRecord(e1, s));
                     pt->cache_events.push_back({1, {e0, e1}});
                 }
@@ -2358,8 +2371,8 @@ int c2_pt_set_cache(c2_pt* pt, c2_nrc* nrc, int32_t cache_bounce, int32_t train_
         B.tlist = A.alloc<int32_t>(tpaths);
         B.trec = A.alloc<float4>((size_t)tpaths * pt->pp.max_depth * 6);
         B.trec_valid = A.alloc<uint8_t>((size_t)tpaths * pt->pp.max_depth);
-        B.bfeat = A.alloc<float4>((size_t)cap * 4);
-        B.btarget = A.alloc<float4>(cap);
+        B.bfeat = A.alloc<float4>((size_t)cap * kCacheSamplesPerChunk * 4);
+        B.btarget = A.alloc<float4>((size_t)cap * kCacheSamplesPerChunk);
         (void)st;
         pt->nrc = nrc;
         pt->pp.cache_bounce = cache_bounce;

@@ -32,6 +32,20 @@ __device__ __forceinline__ float4_t mfma16(half4_t a, half4_t b, float4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
+// Training slice js of ns over the *countp rows collected on the device: equal slices of
+// ceil(count / ns) rows (at most nfixed), the last one shorter. Without a device count
+// (countp == nullptr) the batch is rows [0, nfixed). Returns the row count, sets *off.
+__device__ __forceinline__ int slice_rows(const int32_t* countp, int32_t js, int32_t ns, int32_t nfixed, int* off) {
+    if (!countp) {
+        *off = 0;
+        return nfixed;
+    }
+    const int count = *countp;
+    const int per = min((count + ns - 1) / ns, nfixed);
+    *off = js * per;
+    return min(max(count - *off, 0), per);
+}
+
 // Input encoding: feature f in [0, 64) of raw query r[16].
 __device__ __forceinline__ float nrc_feature(const float* r, const NrcDev& N, int f) {
     if (f < 48) {
@@ -159,13 +173,16 @@ __global__ void __launch_bounds__(kNrcBlock) k_nrc_infer(NrcDev N, const int32_t
 // relative-L2 loss gradient dY = 2 (y - t) / (y^2 + 0.01) / 3 (clamped) as fp16 in D5.
 __global__ void __launch_bounds__(kNrcBlock) k_nrc_fwd_train(NrcDev N, const float4* __restrict__ feats,
                                                              const float4* __restrict__ targets,
-                                                             const int32_t* __restrict__ countp, int32_t nfixed,
-                                                             _Float16* H, _Float16* D5, int64_t rowStride,
-                                                             float* lossAcc) {
+                                                             const int32_t* __restrict__ countp, int32_t js,
+                                                             int32_t ns, int32_t nfixed, _Float16* H, _Float16* D5,
+                                                             int64_t rowStride, float* lossAcc) {
     __shared__ __align__(16) _Float16 sW[kNrcLayers * kNrcW * kNrcW];
     __shared__ float sB[kNrcLayers * kNrcW];
     stage_weights(N.wT, N.b, sW, sB);
-    const int n = countp ? min(*countp, nfixed) : nfixed;
+    int off;
+    const int n = slice_rows(countp, js, ns, nfixed, &off);
+    feats += (int64_t)off * 4;
+    targets += off;
     const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 4;
     float lsum = 0.f;
     for (int base = (blockIdx.x * 4 + wv) * 16; base < n; base += gridDim.x * 4 * 16) {
@@ -197,20 +214,21 @@ __global__ void __launch_bounds__(kNrcBlock) k_nrc_fwd_train(NrcDev N, const flo
         }
     }
     if (lossAcc) {
-        for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+        for (int sh = 32; sh > 0; sh >>= 1) lsum += __shfl_xor(lsum, sh);
         if (l == 0) atomicAdd(lossAcc, lsum);
     }
 }
 
 // Backward through the layers: D_k = (W_k . D_{k+1}) * (H_k > 0) for k = 4..1, with
 // D5 = dY (16 padded columns). D has 5 slabs (D1..D5), H 5 slabs (H0..H4).
-__global__ void __launch_bounds__(kNrcBlock) k_nrc_bwd(NrcDev N, const int32_t* __restrict__ countp, int32_t nfixed,
-                                                       const _Float16* __restrict__ H, _Float16* D,
-                                                       int64_t rowStride) {
+__global__ void __launch_bounds__(kNrcBlock) k_nrc_bwd(NrcDev N, const int32_t* __restrict__ countp, int32_t js,
+                                                       int32_t ns, int32_t nfixed, const _Float16* __restrict__ H,
+                                                       _Float16* D, int64_t rowStride) {
     __shared__ __align__(16) _Float16 sW[kNrcLayers * kNrcW * kNrcW];
     __shared__ float sB[kNrcLayers * kNrcW];
     stage_weights(N.w, N.b, sW, sB);
-    const int n = countp ? min(*countp, nfixed) : nfixed;
+    int off;
+    const int n = slice_rows(countp, js, ns, nfixed, &off);
     const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 4, r16 = l & 15;
     for (int base = (blockIdx.x * 4 + wv) * 16; base < n; base += gridDim.x * 4 * 16) {
         int q = base + r16;
@@ -245,14 +263,15 @@ __global__ void __launch_bounds__(kNrcBlock) k_nrc_bwd(NrcDev N, const int32_t* 
 
 // Split-K weight gradients: partial[layer][wave] = sum over the wavefront's rows of
 // H_layer^T D_{layer+1} (64 x 64, output layer 64 x 16) and column sums of D_{layer+1}.
-__global__ void __launch_bounds__(kNrcBlock) k_nrc_wgrad(const int32_t* __restrict__ countp, int32_t nfixed,
-                                                         const _Float16* __restrict__ H,
+__global__ void __launch_bounds__(kNrcBlock) k_nrc_wgrad(const int32_t* __restrict__ countp, int32_t js,
+                                                         int32_t ns, int32_t nfixed, const _Float16* __restrict__ H,
                                                          const _Float16* __restrict__ D, int64_t rowStride,
                                                          float* partial, int32_t maxWaves) {
     __shared__ __align__(16) _Float16 sH[4][16 * kNrcW];
     __shared__ __align__(16) _Float16 sD[4][16 * kNrcW];
     const int layer = blockIdx.y;
-    const int n = countp ? min(*countp, nfixed) : nfixed;
+    int off;
+    const int n = slice_rows(countp, js, ns, nfixed, &off);
     const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 4, r16 = l & 15;
     const int wave = blockIdx.x * 4 + wv;
     const int mo_tiles = layer == 4 ? 1 : 4;
@@ -300,9 +319,11 @@ __global__ void __launch_bounds__(kNrcBlock) k_nrc_wgrad(const int32_t* __restri
 }
 
 // Sum the per-wavefront partials (mean over n rows) into grad [layer][in][out] + [layer][64].
-__global__ void k_nrc_reduce(const int32_t* __restrict__ countp, int32_t nfixed, const float* __restrict__ partial,
-                             int32_t maxWaves, float* grad, unsigned long long* tot) {
-    const int n = countp ? min(*countp, nfixed) : nfixed;
+__global__ void k_nrc_reduce(const int32_t* __restrict__ countp, int32_t js, int32_t ns, int32_t nfixed,
+                             const float* __restrict__ partial, int32_t maxWaves, float* grad,
+                             unsigned long long* tot) {
+    int off;
+    const int n = slice_rows(countp, js, ns, nfixed, &off);
     if (tot && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(tot, (unsigned long long)n);
     const int used = min(maxWaves, (n + kRowsPerWave - 1) / kRowsPerWave);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -318,10 +339,13 @@ __global__ void k_nrc_reduce(const int32_t* __restrict__ countp, int32_t nfixed,
 }
 
 // Adam on the FP32 master parameters; refresh the fp16 copies in both layouts.
-__global__ void k_nrc_adam(float* master, float* m1, float* m2, const float* __restrict__ grad, _Float16* w,
+__global__ void k_nrc_adam(const int32_t* __restrict__ countp, int32_t js, int32_t ns, int32_t nfixed,
+                           float* master, float* m1, float* m2, const float* __restrict__ grad, _Float16* w,
                            _Float16* wT, float lr, float beta1, float beta2, float eps, float c1, float c2) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= kNrcParams) return;
+    int off;
+    if (slice_rows(countp, js, ns, nfixed, &off) == 0) return;  // an empty slice: no step
     bool isW = i < kNrcLayers * kNrcW * kNrcW;
     int layer = isW ? i / (kNrcW * kNrcW) : (i - kNrcLayers * kNrcW * kNrcW) / kNrcW;
     int out = isW ? i % kNrcW : (i - kNrcLayers * kNrcW * kNrcW) % kNrcW;
@@ -419,26 +443,30 @@ void nrc_launch_infer(c2_nrc* c, const int32_t* list, const int32_t* countp, int
                        L, beta, (float4*)nullptr, c->totals);
 }
 
-void nrc_launch_train(c2_nrc* c, const float4* feats, const float4* targets, const int32_t* countp, int32_t maxn,
-                      hipStream_t s, bool adam) {
+// One training step on slice js of ns of the *countp rows collected on the device (at most
+// maxn rows per slice); a host-sized batch passes countp = nullptr, js = 0, ns = 1.
+void nrc_launch_train(c2_nrc* c, const float4* feats, const float4* targets, const int32_t* countp, int32_t js,
+                      int32_t ns, int32_t maxn, hipStream_t s, bool adam) {
     if (maxn > c->cap) fail(C2_ERR_INVALID, "radiance cache: training batch exceeds capacity");
     int64_t rs = (int64_t)c->cap * kNrcW;
     hipLaunchKernelGGL(k_nrc_fwd_train, dim3(nrc_grid(maxn)), dim3(kNrcBlock), 0, s, c->dev, feats, targets, countp,
-                       maxn, c->H, c->D + 4 * rs, rs, c->loss);
-    hipLaunchKernelGGL(k_nrc_bwd, dim3(nrc_grid(maxn)), dim3(kNrcBlock), 0, s, c->dev, countp, maxn, c->H, c->D, rs);
+                       js, ns, maxn, c->H, c->D + 4 * rs, rs, c->loss);
+    hipLaunchKernelGGL(k_nrc_bwd, dim3(nrc_grid(maxn)), dim3(kNrcBlock), 0, s, c->dev, countp, js, ns, maxn, c->H,
+                       c->D, rs);
     int waves = (maxn + kRowsPerWave - 1) / kRowsPerWave;
-    hipLaunchKernelGGL(k_nrc_wgrad, dim3((waves + 3) / 4, kNrcLayers), dim3(kNrcBlock), 0, s, countp, maxn, c->H,
-                       c->D, rs, c->partial, c->maxWaves);
+    hipLaunchKernelGGL(k_nrc_wgrad, dim3((waves + 3) / 4, kNrcLayers), dim3(kNrcBlock), 0, s, countp, js, ns, maxn,
+                       c->H, c->D, rs, c->partial, c->maxWaves);
     int tot = kNrcLayers * kPartial;
-    hipLaunchKernelGGL(k_nrc_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, countp, maxn, c->partial, c->maxWaves,
-                       c->grad, c->totals + 1);
+    hipLaunchKernelGGL(k_nrc_reduce, dim3((tot + 255) / 256), dim3(256), 0, s, countp, js, ns, maxn, c->partial,
+                       c->maxWaves, c->grad, c->totals + 1);
     if (adam) {
         c->step++;
         const c2_nrc_opts& o = c->opts;
         float c1 = 1.f - (float)std::pow((double)o.beta1, (double)c->step);
         float c2v = 1.f - (float)std::pow((double)o.beta2, (double)c->step);
-        hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, c->master, c->m1, c->m2,
-                           c->grad, c->w, c->wT, o.learning_rate, o.beta1, o.beta2, o.epsilon, c1, c2v);
+        hipLaunchKernelGGL(k_nrc_adam, dim3((kNrcParams + 255) / 256), dim3(256), 0, s, countp, js, ns, maxn,
+                           c->master, c->m1, c->m2, c->grad, c->w, c->wT, o.learning_rate, o.beta1, o.beta2,
+                           o.epsilon, c1, c2v);
     }
     NRC_CHECK(hipGetLastError());
 }
@@ -549,7 +577,7 @@ int c2_nrc_train_step(c2_nrc* c, const float* feats, const float* targets, int64
         for (int64_t i = 0; i < n; i++) t4[i] = make_float4(targets[i * 3], targets[i * 3 + 1], targets[i * 3 + 2], 0);
         NRC_CHECK(hipMemcpy(dt, t4.data(), n * sizeof(float4), hipMemcpyHostToDevice));
         NRC_CHECK(hipMemset(c->loss, 0, sizeof(float)));
-        nrc_launch_train(c, df, dt, nullptr, (int32_t)n, nullptr, apply != 0);
+        nrc_launch_train(c, df, dt, nullptr, 0, 1, (int32_t)n, nullptr, apply != 0);
         NRC_CHECK(hipDeviceSynchronize());
         if (grad_out) NRC_CHECK(hipMemcpy(grad_out, c->grad, kNrcParams * sizeof(float), hipMemcpyDeviceToHost));
         if (loss_out) {
