@@ -456,17 +456,21 @@ def equal_time(scene, W, H, depth, seconds=0.0, judge_spp=4096, crop=256, cache_
     warm = time.perf_counter() - a
     pt.reset()
     pt.cache_times()  # clear
+    st0 = cache.stats()  # the warm-up's queries and rows are not in the timed kernels
     cache_curve = run(pt, seconds - warm, cps)
     cache_curve = [(round(t + warm, 4), n, r) for (t, n, r) in cache_curve]
     ct = pt.cache_times()
     st = cache.stats()
+    st = dict(st, queries=st["queries"] - st0["queries"], train_rows=st["train_rows"] - st0["train_rows"],
+              steps_timed=st["steps"] - st0["steps"])
     pt.close()
     target = plain_curve[-1][2]
     reach = next((t for (t, n, r) in cache_curve if r <= target), None)
     flop_q = st["flop_forward_per_query"]
     npass = cache_curve[-1][1]
     infer_tflops = st["queries"] * flop_q / max(ct["infer_ms"], 1e-9) / 1e9
-    # training: forward + backward (dX) + weight gradient = 3x the forward FLOP per row
+    # training: forward + backward (dX) + weight gradient = 3x the forward FLOP per row, over
+    # the training kernels' time (gathering the rows, which runs no MFMA work, is apart)
     train_tflops = st["train_rows"] * 3 * flop_q / max(ct["train_ms"], 1e-9) / 1e9
     return {
         "seconds": seconds,
@@ -482,7 +486,9 @@ def equal_time(scene, W, H, depth, seconds=0.0, judge_spp=4096, crop=256, cache_
                  "infer_util_vs_fp16_peak": round(infer_tflops / 2500.0, 5),
                  "train_util_vs_fp16_peak": round(train_tflops / 2500.0, 5),
                  "infer_ms_per_pass": round(ct["infer_ms"] / max(1, npass), 4),
-                 "train_ms_per_pass": round(ct["train_ms"] / max(1, npass), 4)},
+                 "train_ms_per_pass": round(ct["train_ms"] / max(1, npass), 4),
+                 "collect_ms_per_pass": round(ct["collect_ms"] / max(1, npass), 4),
+                 "train_rows_per_pass": round(st["train_rows"] / max(1, npass))},
     }
 
 

@@ -339,7 +339,7 @@ class WavefrontPathIntegrator:
     def cache_times(self) -> dict:
         t = _abi.c2_cache_times()
         check(self._lib.c2_pt_get_cache_times(self._h, C.byref(t)))
-        return {"infer_ms": t.infer_ms, "train_ms": t.train_ms}
+        return {"infer_ms": t.infer_ms, "train_ms": t.train_ms, "collect_ms": t.collect_ms}
 
     def set_profiling(self, enable: bool = True, kernels=None):
         """Time kernels with HIP events: all classes, or only those named in ``kernels``

@@ -85,7 +85,7 @@ class c2_nrc_stats(C.Structure):
 
 class c2_cache_times(C.Structure):
     _fields_ = [("infer_ms", C.c_double), ("train_ms", C.c_double), ("queries", C.c_int64),
-                ("train_rows", C.c_int64)]
+                ("train_rows", C.c_int64), ("collect_ms", C.c_double)]
 
 
 NRC_PARAMS = 5 * 64 * 64 + 5 * 64

@@ -272,7 +272,9 @@ __device__ __forceinline__ void cache_features(const DevScene& S, const Inter& i
     out[0] = make_float4(it.p.x, it.p.y, it.p.z, -d.x);
     out[1] = make_float4(-d.y, -d.z, it.n.x, it.n.y);
     out[2] = make_float4(it.n.z, rough, alb.x, alb.y);
-    out[3] = make_float4(alb.z, 0.f, 0.f, 0.f);
+    // azimuths of the outgoing direction and the normal, for the cache's input encoding
+    out[3] = make_float4(alb.z, atan2f(-d.z, -d.x) * 0.15915494f + 0.5f, atan2f(it.n.z, it.n.x) * 0.15915494f + 0.5f,
+                         0.f);
 }
 
 __device__ __forceinline__ int32_t halton_index_dev(const View& P, int x, int y, int k) {
@@ -737,26 +739,39 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
     }
 }
 
-// Training rows of this pass: target = (L_final - L after the vertex's emission) / beta.
+// Training rows of this launch: target = (L_final - L after the vertex's emission) / beta.
+// One thread per (training path, bounce) record, so every load is independent and the
+// records are read in their memory order; one append per wavefront (ballot + one atomic),
+// not one atomic per row: every row of a launch appends to the same counter.
 __global__ void __launch_bounds__(kBlock) k_cache_collect(PathBuffers B, PassParams P) {
-    int nt = min(B.counters[5], P.train_paths);
-    for (int tp = blockIdx.x * blockDim.x + threadIdx.x; tp < nt; tp += gridDim.x * blockDim.x) {
-        int p = B.tlist[tp];
-        float4 Lf = B.L[p];
-        for (int b = 0; b < P.max_depth; b++) {
-            int64_t r = (int64_t)tp * P.max_depth + b;
-            if (!B.trec_valid[r]) continue;
-            const float4* rec = B.trec + r * 6;
-            float4 be = rec[4], lp = rec[5];
-            float t0 = be.x > 0.f ? (Lf.x - lp.x) / be.x : 0.f;
-            float t1 = be.y > 0.f ? (Lf.y - lp.y) / be.y : 0.f;
-            float t2 = be.z > 0.f ? (Lf.z - lp.z) / be.z : 0.f;
-            if (!(isfinite(t0) && isfinite(t1) && isfinite(t2))) continue;
-            int slot = atomicAdd(&B.counters[7], 1);
-            if (slot >= P.train_cap * P.nsamp) continue;
-            for (int k = 0; k < 4; k++) B.bfeat[(int64_t)slot * 4 + k] = rec[k];
-            B.btarget[slot] = make_float4(fmaxf(t0, 0.f), fmaxf(t1, 0.f), fmaxf(t2, 0.f), 0.f);
+    const int64_t nr = (int64_t)min(B.counters[5], P.train_paths) * P.max_depth;
+    const int cap = P.train_cap * P.nsamp;
+    const int lane = __lane_id();
+    const uint64_t lower = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x + (int)threadIdx.x - lane; r0 < nr; r0 += stride) {  // wave-uniform
+        const int64_t r = r0 + lane;
+        bool ok = r < nr && B.trec_valid[r];
+        const float4* rec = B.trec + r * 6;
+        float4 tgt = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ok) {
+            const float4 Lf = B.L[B.tlist[r / P.max_depth]];
+            const float4 be = rec[4], lp = rec[5];
+            const float t0v = be.x > 0.f ? (Lf.x - lp.x) / be.x : 0.f;
+            const float t1v = be.y > 0.f ? (Lf.y - lp.y) / be.y : 0.f;
+            const float t2v = be.z > 0.f ? (Lf.z - lp.z) / be.z : 0.f;
+            ok = isfinite(t0v) && isfinite(t1v) && isfinite(t2v);
+            tgt = make_float4(fmaxf(t0v, 0.f), fmaxf(t1v, 0.f), fmaxf(t2v, 0.f), 0.f);
         }
+        const uint64_t m = __ballot(ok);
+        if (!m) continue;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&B.counters[7], __popcll(m));
+        base = __shfl(base, 0);
+        const int slot = base + __popcll(m & lower);
+        if (!ok || slot >= cap) continue;
+        for (int k = 0; k < 4; k++) B.bfeat[(int64_t)slot * 4 + k] = rec[k];
+        B.btarget[slot] = tgt;
     }
 }
 
@@ -1691,7 +1706,7 @@ struct c2_pt {
     // radiance cache
     c2_nrc* nrc = nullptr;
     DeviceArena cache_arena;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> cache_events;  // 0 infer, 1 train
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> cache_events;  // 0 infer, 1 train, 2 collect
     int64_t cache_queries = 0, cache_rows = 0;
 
     ~c2_pt() {
@@ -2198,17 +2213,22 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                     }
                 }
                 if (P.cache_bounce >= 0) {
-                    hipEvent_t e0, e1;
-                    HIP_CHECK(hipEventCreate(&e0));
-                    HIP_CHECK(hipEventCreate(&e1));
+                    // each record owns its two events (c2_pt_get_cache_times and the destructor
+                    // destroy both of every record)
+                    hipEvent_t e0, e1, e2, e3;
+                    for (hipEvent_t* e : {&e0, &e1, &e2, &e3}) HIP_CHECK(hipEventCreate(e));
                     HIP_CHECK(hipEventRecord(e0, s));
-                    hipLaunchKernelGGL(k_cache_collect, dim3(grid_for(P.train_paths)), dim3(kBlock), 0, s, LB, P);
+                    const int64_t pairs = (int64_t)P.train_paths * P.max_depth;  // (training path, bounce)
+                    hipLaunchKernelGGL(k_cache_collect, dim3(grid_for(pairs)), dim3(kBlock), 0, s, LB, P);
+                    HIP_CHECK(hipEventRecord(e1, s));
+                    HIP_CHECK(hipEventRecord(e2, s));
                     // one Adam step per sample of the launch, each on an equal slice of the rows
                     for (int js = 0; js < ns; js++)
                         nrc_launch_train(pt->nrc, LB.bfeat, LB.btarget, LB.counters + 7, js, ns, P.train_cap, s,
                                          true);
-                    HIP_CHECK(hipEventRecord(e1, s));
-                    pt->cache_events.push_back({1, {e0, e1}});
+                    HIP_CHECK(hipEventRecord(e3, s));
+                    pt->cache_events.push_back({2, {e0, e1}});
+                    pt->cache_events.push_back({1, {e2, e3}});
                 }
                 // films accumulate in chunk order: after the previous chunk's (other lane)
                 if (L > 1 && nchunk > 0) HIP_CHECK(hipStreamWaitEvent(s, pt->ev_acc[(lane + L - 1) % L], 0));
@@ -2387,8 +2407,8 @@ int c2_pt_get_cache_times(c2_pt* pt, c2_cache_times* out) {
         if (!pt || !out) fail(C2_ERR_INVALID, "null argument");
         HIP_CHECK(hipDeviceSynchronize());
         c2_cache_times r{};
-        int passes[2] = {0, 0};
-        double ms[2] = {0, 0};
+        int passes[3] = {0, 0, 0};
+        double ms[3] = {0, 0, 0};
         for (auto& e : pt->cache_events) {
             float t = 0;
             HIP_CHECK(hipEventElapsedTime(&t, e.second.first, e.second.second));
@@ -2400,6 +2420,7 @@ int c2_pt_get_cache_times(c2_pt* pt, c2_cache_times* out) {
         pt->cache_events.clear();
         r.infer_ms = passes[0] ? ms[0] : 0;
         r.train_ms = passes[1] ? ms[1] : 0;
+        r.collect_ms = passes[2] ? ms[2] : 0;
         *out = r;
     });
 }

@@ -1,19 +1,22 @@
 // Radiance cache network shared definitions (device + host).
 //
 // MLP 64 -> 64 -> 64 -> 64 -> 64 -> 3 (4 hidden ReLU layers, linear output),
-// FP16 operands on v_mfma_f32_16x16x16f16 with FP32 accumulation, FP32 master
+// FP16 operands on v_mfma_f32_16x16x32_f16 with FP32 accumulation, FP32 master
 // weights and Adam. A query is 16 raw floats: position, outgoing direction,
 // facing normal, roughness, albedo; the 64-dim input encoding is built inside
 // the kernels (per coordinate 8 frequencies x {sin, cos}, spherical direction and
 // normal, 4-bin one-blob roughness, albedo, bias).
 //
 // Feature-major formulation per wavefront: H_out^T (64 x 16 queries) =
-// W^T (64 x 64) * H_in^T (64 x 16). With the MFMA fragment layouts
-//   A: lane l holds A[l % 16][4 (l / 16) + j]
-//   B: lane l holds B[4 (l / 16) + j][l % 16]
-//   D: lane l holds D[4 (l / 16) + j][l % 16]
-// the D fragment of output tile m is exactly the B fragment of input tile m of
-// the next layer, so layers chain in registers without transposes.
+// W^T (64 x 64) * H_in^T (64 x 16). With the MFMA fragment layouts (lane l = 16 g + c)
+//   A: lane l holds A[c][8 g + j], j < 8
+//   B: lane l holds B[8 g + j][c]
+//   D: lane l holds D[4 g + i][c], i < 4
+// and the K order of a 32-feature k-tile chosen so that slot j of lane group g is
+// feature 32 t + 16 (j / 4) + 4 g + j % 4, the D fragments of output tiles 2t and
+// 2t + 1 of a layer, concatenated per lane, are exactly the next layer's B fragment
+// of k-tile t: layers chain in registers without transposes (the weights are staged
+// with their columns in that order).
 #pragma once
 #include <cstdint>
 

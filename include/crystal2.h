@@ -334,8 +334,10 @@ int  c2_pt_capture_rays(c2_pt* pt, int32_t sample, int32_t bounce, int32_t kind,
 /* ---------------------------------------------------------------- Radiance cache (new, no reference counterpart)
  * An online-trained MLP (64-dim input encoding, 4 hidden layers x 64 ReLU, 3
  * outputs) with FP16 MFMA forward / backward and FP32 Adam. A query is 16 floats:
- * position, outgoing direction, facing normal, roughness, albedo, 3 zeros; the
- * prediction is the radiance scattered from that point along the direction.
+ * position, outgoing direction, facing normal, roughness, albedo, then 3 slots the
+ * library fills itself (the azimuths of the direction and the normal; values passed
+ * there are ignored); the prediction is the radiance scattered from that point along
+ * the direction.
  * Parameters: [5 layers][64 in][64 out] weights then [5][64] biases (the output
  * layer uses 3 of its 64 columns). */
 typedef struct c2_nrc c2_nrc;
@@ -373,13 +375,15 @@ int  c2_nrc_get_stats(const c2_nrc* nrc, c2_nrc_stats* out);
 /* Cache mode for a renderer: camera paths end at bounce `cache_bounce` with the
  * cache's prediction; one pixel in `train_ratio` per pass traces a full training
  * path whose vertices become training rows (target = radiance after the vertex /
- * throughput), and one Adam step runs per pass. nrc == NULL disables it. */
+ * throughput), and one Adam step runs per sample pass. nrc == NULL disables it. */
 int  c2_pt_set_cache(c2_pt* pt, c2_nrc* nrc, int32_t cache_bounce, int32_t train_ratio);
-/* Mean time per pass of the cache's kernels (inference, training step), in ms,
- * and the rows / queries they processed (for MFMA utilisation). */
+/* Time of the cache's kernels since the last call, in ms (inference; training steps;
+ * gathering the training rows, which runs no MFMA work), and the rows / queries they
+ * processed (for MFMA utilisation). */
 typedef struct c2_cache_times {
     double infer_ms, train_ms;
     int64_t queries, train_rows;
+    double collect_ms;
 } c2_cache_times;
 int  c2_pt_get_cache_times(c2_pt* pt, c2_cache_times* out);
 
