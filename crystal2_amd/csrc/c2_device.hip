@@ -40,6 +40,7 @@
 
 #include "c2_bvh.h"
 #include "c2_kdtree.h"
+#include "c2_sched.h"
 #include "c2_device.h"
 #include "c2_host.h"
 #include "c2_shade.h"
@@ -55,7 +56,10 @@ namespace c2 {
 
 constexpr int kBlock = 256;
 static_assert(kBlock == kTravBlock, "traversal kernels index their LDS stack by threadIdx.x");
-constexpr int kMisGrid = 1024;  // blocks for the MIS-ray kernel (grid-stride; few rays)
+// blocks for the MIS-ray kernel (grid-stride; about 0.005 rays per sample, a few ten thousand
+// per launch): a small grid is dispatched and drained early instead of holding block slots
+// beside the shadow-ray kernel for its whole duration
+constexpr int kMisGrid = 64;
 // Direct lighting: wavefront steps per path (transparent surfaces crossed + 1).
 constexpr int kDlChainLimit = 8;
 // Default grid cap of the per-bounce queue kernels. Their live counts are only known
@@ -2070,44 +2074,46 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                        cap == hipStreamCaptureStatusNone)
                           ? pt->lanes
                           : 1;
+        // the chunks and lanes of this call (host-only plan, c2_sched.h)
+        std::vector<SchedChunk> plan;
+        {
+            std::vector<SchedPass> tabs(pt->passes.size());
+            for (size_t t = 0; t < tabs.size(); t++)
+                tabs[t] = SchedPass{pt->passes[t].off, pt->passes[t].npv, pt->passes[t].npix};
+            SchedInput in{};
+            in.passes = tabs.data();
+            in.npasses = (int32_t)tabs.size();
+            in.per_sample_passes = per_sample_passes;
+            in.sample_begin = sample_begin;
+            in.sample_count = sample_count;
+            in.chunk = pt->chunk;
+            // samples per chunk: several whole passes when the buffers hold them (not with ray
+            // capture, which reads one pass)
+            in.samples_per_chunk = pt->cap_bounce >= 0                      ? 1
+                                   : per_sample_passes                      ? kBatchSamplesPerChunk
+                                   : P.cache_bounce >= 0                   ? kCacheSamplesPerChunk
+                                   : P.integrator == C2_INTEGRATOR_DIRECT ? kDirectSamplesPerChunk
+                                                                           : kSamplesPerChunk;
+            in.lanes = L;
+            if (plan_passes(in, plan) != 0) fail(C2_ERR_INVALID, "render: sample range outside the pass tables");
+        }
         if (L > 1) {
             HIP_CHECK(hipEventRecord(pt->ev_start, s0));
             for (int l = 0; l < L - 1; l++) HIP_CHECK(hipStreamWaitEvent(pt->lstream[l], pt->ev_start, 0));
         }
-        int64_t nchunk = 0;
-        const int32_t sample_end = sample_begin + sample_count;
-        for (int k = sample_begin; k < sample_end;) {
-            const c2_pt::PassTab& tab = pt->passes[per_sample_passes ? k : 0];
-            const int64_t npix = tab.npix;
-            // samples per chunk: several whole passes when the buffers hold them (without
-            // the radiance cache or ray capture), else one pass in chunks. A batch's passes
-            // k .. k + ns - 1 must have the same views: the sets shrink as k passes each
-            // view's spp, so an equal view count means the same set.
-            int ns = 1;
-            if (pt->cap_bounce < 0 && npix > 0) {
-                const int64_t spc = per_sample_passes                        ? kBatchSamplesPerChunk
-                                    : P.cache_bounce >= 0                   ? kCacheSamplesPerChunk
-                                    : P.integrator == C2_INTEGRATOR_DIRECT ? kDirectSamplesPerChunk
-                                                                           : kSamplesPerChunk;
-                int64_t lim = std::min<int64_t>({pt->chunk / npix, (int64_t)sample_end - k, spc});
-                if (per_sample_passes)
-                    for (int64_t j = 1; j < lim; j++)
-                        if (pt->passes[k + j].npv != tab.npv) {
-                            lim = j;
-                            break;
-                        }
-                ns = (int)std::max<int64_t>(1, lim);
-            }
-            const int64_t step = ns > 1 ? npix : pt->chunk;
+        for (const SchedChunk& ch : plan) {
+            const c2_pt::PassTab& tab = pt->passes[ch.tab];
+            const int ns = ch.nsamp;
             P.pv = pt->pv + tab.off;
             P.npv = tab.npv;
-            P.sample = k;
+            P.sample = ch.sample;
             P.nsamp = ns;
-            for (int64_t beg = 0; beg < npix; beg += step) {
-                P.pix_begin = (int32_t)beg;
-                P.pix_count = (int32_t)std::min<int64_t>(step, npix - beg);
+            {
+                P.pix_begin = ch.pix_begin;
+                P.pix_count = ch.pix_count;
                 const int64_t npath = (int64_t)ns * P.pix_count;
-                const int lane = (int)(nchunk % L);
+                const int lane = ch.lane;
+                if (lane < 0 || lane >= L) fail(C2_ERR_INVALID, "render: lane outside the renderer's lanes");
                 const hipStream_t s = lane ? pt->lstream[lane - 1] : s0;
                 const hipStream_t side = !kMisSide ? s : lane ? pt->lside[lane - 1] : pt->side;
                 const hipEvent_t evf = lane ? pt->lfork[lane - 1] : pt->ev_fork;
@@ -2231,7 +2237,7 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                     pt->cache_events.push_back({1, {e2, e3}});
                 }
                 // films accumulate in chunk order: after the previous chunk's (other lane)
-                if (L > 1 && nchunk > 0) HIP_CHECK(hipStreamWaitEvent(s, pt->ev_acc[(lane + L - 1) % L], 0));
+                if (ch.wait_acc >= 0) HIP_CHECK(hipStreamWaitEvent(s, pt->ev_acc[ch.wait_acc], 0));
                 {
                     KTimer kt(pt, C2_K_ACCUMULATE, s);
                     hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(kBlock), 0, s, LB, P, pt->opts.film_mode,
@@ -2240,10 +2246,8 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                 if (L > 1) HIP_CHECK(hipEventRecord(pt->ev_acc[lane], s));
                 pt->launches++;
                 HIP_CHECK(hipGetLastError());
-                nchunk++;
             }
-            pt->samples += npix * ns;
-            k += ns;
+            pt->samples += (int64_t)ns * ch.pix_count;
         }
         for (int l = 0; l < L - 1; l++) {
             HIP_CHECK(hipEventRecord(pt->lend[l], pt->lstream[l]));
