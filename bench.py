@@ -523,8 +523,10 @@ def batch_bench(n=64, seed=6400, depth=8):
     """Batched scenes: n sphere-field scenes (triangles log-uniform in [1k, 200k],
     resolution in {128, 256, 512}^2, spp in {4, 16, 64}) rendered (a) one scene at a
     time with its own launches and (b) as one batch replayed from a captured hipGraph
-    after the pass-0 autotuner. Both are timed wall-clock after a warm-up render;
-    every batch film must equal its single-scene film bit for bit."""
+    with the schedule the autotuner keeps (the wavefront's queue kernels or the one
+    persistent-kernel launch; the other schedule is timed beside it). Timed wall-clock
+    after a warm-up render; every batch film must equal its single-scene film bit for
+    bit, under both schedules."""
     import crystal2_amd as c2
     from crystal2_amd import scenes
     t0 = time.time()
@@ -559,15 +561,34 @@ def batch_bench(n=64, seed=6400, depth=8):
     batch_s = time.perf_counter() - a
     exact = sum(int(np.array_equal(b.film(i), r)) for i, r in enumerate(ref))
     st = b.stats()
+    # the schedule the autotuner did not keep, timed the same way
+    other = "wavefront" if tune["schedule"] == "persistent" else "persistent"
+    b.configure(other, block_threads=tune["block_threads"], waves_per_simd=tune["waves_per_simd"])
+    b.render(True)
+    b.sync()
+    a = time.perf_counter()
+    b.render(True)
+    b.sync()
+    other_s = time.perf_counter() - a
+    exact_other = sum(int(np.array_equal(b.film(i), r)) for i, r in enumerate(ref))
+    other_nodes = b.stats()["graph_nodes"]
     b.close()
     out = {"scenes": n, "triangles": int(sum(s["triangles"] for s in specs)), "samples": int(samples),
            "depth": depth, "per_scene_s": round(single_s, 4), "batch_graph_s": round(batch_s, 4),
            "per_scene_msamples_s": round(samples / single_s / 1e6, 2),
            "batch_graph_msamples_s": round(samples / batch_s / 1e6, 2),
            "speedup": round(single_s / batch_s, 3), "films_bit_identical": f"{exact}/{n}",
+           "schedule": tune["schedule"], "graph_nodes": st["graph_nodes"],
            "autotune": {"paths_in_flight": tune["paths_in_flight"], "queue_blocks": tune["queue_blocks"],
-                        "candidates": tune["candidates"], "pass0_ms": round(tune["pass_ms"], 3)},
-           "graph_nodes": st["graph_nodes"]}
+                        "candidates": tune["candidates"], "pass0_ms": round(tune["pass_ms"], 3),
+                        "persistent_block_threads": tune["block_threads"],
+                        "persistent_waves_per_simd": tune["waves_per_simd"],
+                        "persistent_sample0_ms": round(tune["sample0_ms_persistent"], 3),
+                        "render_ms_wavefront": round(tune["render_ms_wavefront"], 2),
+                        "render_ms_persistent": round(tune["render_ms_persistent"], 2)},
+           "other_schedule": {"schedule": other, "graph_s": round(other_s, 4),
+                              "msamples_s": round(samples / other_s / 1e6, 2), "graph_nodes": other_nodes,
+                              "films_bit_identical": f"{exact_other}/{n}"}}
     return out
 
 

@@ -393,7 +393,7 @@ class BatchRenderer:
     """
 
     def __init__(self, items, max_depth: int, arg_order: str = "rtl", film: str = "f32",
-                 paths_in_flight: int = 0, queue_blocks: int = 0, integrator: str = "path"):
+                 paths_in_flight: int = 0, queue_blocks: int = 0, integrator: str = "path", accel: str = "bvh"):
         self._lib = load_library()
         self.items = [(sc, int(w), int(h), int(spp)) for sc, w, h, spp in items]
         self._descs = [sc.desc() for sc, _, _, _ in self.items]  # kept alive for the call
@@ -408,13 +408,30 @@ class BatchRenderer:
         o.paths_in_flight = paths_in_flight
         o.queue_blocks = queue_blocks
         o.integrator = WavefrontPathIntegrator.INTEGRATORS[integrator]
+        o.accel = GpuBVH.ACCELS[accel]
         self._h = C.c_void_p()
         check(self._lib.c2_batch_create(arr, len(self.items), C.byref(o), C.byref(self._h)))
 
+    SCHEDULES = {"wavefront": _abi.C2_SCHED_WAVEFRONT, "persistent": _abi.C2_SCHED_PERSISTENT}
+
     def autotune(self) -> dict:
+        """Tunes both schedules and keeps the faster (``c2_batch_autotune``)."""
         t = _abi.c2_batch_tuning()
         check(self._lib.c2_batch_autotune(self._h, C.byref(t)))
-        return {k: getattr(t, k) for k, _ in t._fields_}
+        out = {k: getattr(t, k) for k, _ in t._fields_}
+        out["schedule"] = {v: k for k, v in self.SCHEDULES.items()}[out["schedule"]]
+        return out
+
+    def configure(self, schedule: str = "wavefront", block_threads: int = 256, waves_per_simd: int = 0,
+                  paths_in_flight: int = 0, queue_blocks: int = 0):
+        """Sets the schedule ("wavefront" or "persistent") and launch shape explicitly."""
+        t = _abi.c2_batch_tuning()
+        t.schedule = self.SCHEDULES[schedule]
+        t.block_threads = block_threads
+        t.waves_per_simd = waves_per_simd
+        t.paths_in_flight = paths_in_flight
+        t.queue_blocks = queue_blocks
+        check(self._lib.c2_batch_configure(self._h, C.byref(t)))
 
     def render(self, use_graph: bool = True):
         check(self._lib.c2_batch_render(self._h, int(use_graph)))
@@ -431,7 +448,9 @@ class BatchRenderer:
     def stats(self) -> dict:
         s = _abi.c2_batch_stats()
         check(self._lib.c2_batch_get_stats(self._h, C.byref(s)))
-        return {k: getattr(s, k) for k, _ in s._fields_}
+        out = {k: getattr(s, k) for k, _ in s._fields_}
+        out["schedule"] = {v: k for k, v in self.SCHEDULES.items()}[out["schedule"]]
+        return out
 
     def close(self):
         if self._h:

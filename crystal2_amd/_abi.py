@@ -100,15 +100,21 @@ class c2_batch_item(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32)]
 
 
+C2_SCHED_WAVEFRONT, C2_SCHED_PERSISTENT = 0, 1
+
+
 class c2_batch_tuning(C.Structure):
     _fields_ = [("paths_in_flight", C.c_int32), ("queue_blocks", C.c_int32), ("pass_ms", C.c_double),
-                ("candidates", C.c_int32)]
+                ("candidates", C.c_int32), ("schedule", C.c_int32), ("block_threads", C.c_int32),
+                ("waves_per_simd", C.c_int32), ("sample0_ms_persistent", C.c_double), ("render_ms_wavefront", C.c_double),
+                ("render_ms_persistent", C.c_double)]
 
 
 class c2_batch_stats(C.Structure):
     _fields_ = [("samples", C.c_int64), ("samples_per_render", C.c_int64), ("extension_rays", C.c_int64),
                 ("shadow_rays", C.c_int64), ("mis_rays", C.c_int64), ("passes", C.c_int32),
-                ("paths_in_flight", C.c_int32), ("queue_blocks", C.c_int32), ("graph_nodes", C.c_int64)]
+                ("paths_in_flight", C.c_int32), ("queue_blocks", C.c_int32), ("graph_nodes", C.c_int64),
+                ("schedule", C.c_int32), ("block_threads", C.c_int32), ("waves_per_simd", C.c_int32)]
 
 
 class c2_kernel_times(C.Structure):
@@ -181,6 +187,7 @@ SIGNATURES = [
     ("c2_batch_create", C.c_int, [C.POINTER(c2_batch_item), C.c_int32, C.POINTER(c2_render_opts), C.POINTER(P)]),
     ("c2_batch_destroy", None, [P]),
     ("c2_batch_autotune", C.c_int, [P, C.POINTER(c2_batch_tuning)]),
+    ("c2_batch_configure", C.c_int, [P, C.POINTER(c2_batch_tuning)]),
     ("c2_batch_render", C.c_int, [P, C.c_int32]),
     ("c2_batch_sync", C.c_int, [P]),
     ("c2_batch_get_film", C.c_int, [P, C.c_int32, _f32p]),

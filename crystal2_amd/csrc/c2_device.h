@@ -216,13 +216,13 @@ __device__ __forceinline__ void load_prim(const DevScene& S, int slot, float4& a
 // Closest hit (ANY = false) or any hit (ANY = true). Closest hit: tmin = 0 and
 // tmax = ray tmax (the reference passes 0, +inf). Returns true on a hit.
 //
-// The traversal stack: the top K entries of each lane live in LDS (a [K][kTravBlock]
+// The traversal stack: the top K entries of each lane live in LDS (a [K][block size]
 // slab per block, slot-major, so the lanes of a wavefront touch consecutive words
 // and no two lanes share a bank beyond the wave64 halves); entries deeper than K
 // move to a private overflow array, which only deep traversals touch. The stack
 // never holds more than kStackSize entries (at most 3 pushes per level of a tree
 // at most kMaxBvh4Depth deep).
-constexpr int kTravBlock = 256;  // block size of every kernel that traverses
+constexpr int kTravBlock = 256;  // block size of the wavefront traversal kernels (the stack default)
 #ifndef C2_TRAV_K_ANY
 #define C2_TRAV_K_ANY 16
 #endif
@@ -233,11 +233,13 @@ template <bool ANY>
 struct TravK {
     static constexpr int value = ANY ? C2_TRAV_K_ANY : C2_TRAV_K_CLOSEST;
 };
-// LDS backing of the stack for one block; declare as `__shared__ TravLds<ANY> lds;`.
-template <bool ANY>
+// LDS backing of the stack for one block of BS threads; declare as
+// `__shared__ TravLds<ANY> lds;` (BS = kTravBlock) or `TravLds<ANY, BS>`.
+template <bool ANY, int BS = kTravBlock>
 struct TravLds {
-    int32_t idx[TravK<ANY>::value * kTravBlock];
-    float t[ANY ? 1 : TravK<ANY>::value * kTravBlock];
+    static constexpr int kBS = BS;
+    int32_t idx[TravK<ANY>::value * BS];
+    float t[ANY ? 1 : TravK<ANY>::value * BS];
 };
 
 // Overflow part of a lane's stack, in private memory: the entries pushed beyond the K
@@ -250,7 +252,7 @@ struct TravOverflow {
 };
 // The stack proper holds only scalars and pointers, so that the stack pointer and lane
 // stay in registers (an array member would put the whole object in private memory).
-template <bool ANY>
+template <bool ANY, int BS = kTravBlock>
 struct TravStack {
     static constexpr int K = TravK<ANY>::value;
     static_assert((K & (K - 1)) == 0 && K > 0 && K <= kStackSize, "LDS stack depth must be a power of two");
@@ -260,10 +262,10 @@ struct TravStack {
     float* ot;
     int lane;
     int sp;
-    __device__ __forceinline__ TravStack(TravLds<ANY>& L, TravOverflow<ANY>& O)
+    __device__ __forceinline__ TravStack(TravLds<ANY, BS>& L, TravOverflow<ANY>& O)
         : li(L.idx), lt(L.t), oi(O.i), ot(O.t), lane((int)threadIdx.x), sp(0) {}
     __device__ __forceinline__ void push(int32_t e, float t) {
-        int off = (sp & (K - 1)) * kTravBlock + lane;
+        int off = (sp & (K - 1)) * BS + lane;
         if (sp >= K) {  // evict the oldest resident entry (same ring slot) to the overflow array
             oi[sp - K] = li[off];
             if (!ANY) ot[sp - K] = lt[off];
@@ -274,7 +276,7 @@ struct TravStack {
     }
     __device__ __forceinline__ void pop(int32_t& e, float& t) {
         sp--;
-        int off = (sp & (K - 1)) * kTravBlock + lane;
+        int off = (sp & (K - 1)) * BS + lane;
         e = li[off];
         t = ANY ? 0.f : lt[off];
         if (sp >= K) {  // refill the freed ring slot with the entry K below the new top
@@ -286,16 +288,16 @@ struct TravStack {
 
 // root: Node4 index where the traversal starts (a batch keeps every scene's tree in
 // one node array); a negative root is a scene without entities (always a miss).
-template <bool ANY>
+template <bool ANY, int BS = kTravBlock>
 __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, V3 d, float tmin, float tmax,
-                                         int ignore, Hit& hit, TravLds<ANY>& lds) {
+                                         int ignore, Hit& hit, TravLds<ANY, BS>& lds) {
     hit.entity = -1;
     if (root < 0) return false;
     RayPre r = make_ray(o, d);
     // byte offset of each axis's near plane within its (lo, hi) pair
     const uint32_t oX = r.neg[0] ? 16u : 0u, oY = r.neg[1] ? 16u : 0u, oZ = r.neg[2] ? 16u : 0u;
     TravOverflow<ANY> ovf;
-    TravStack<ANY> st(lds, ovf);
+    TravStack<ANY, BS> st(lds, ovf);
     int32_t cur = root;
     float best = INFINITY;
     int32_t bestE = -1;

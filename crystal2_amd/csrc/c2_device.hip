@@ -227,6 +227,7 @@ struct PathBuffers {
 // has one view; a batch has one per scene, all sharing one concatenated scene.
 struct View {
     int32_t width, height;
+    int32_t spp;             // samples per pixel of the view
     int32_t root;            // Node4 index of the scene's BVH root, -1 = no entities
     int32_t elig_begin;      // eligible lights (Flux != 0): elig[elig_begin, + num_elig)
     int32_t num_elig;
@@ -294,6 +295,273 @@ __device__ __forceinline__ int32_t halton_index_dev(const View& P, int x, int y,
         off += (int64_t)(r1 * (uint64_t)(int64_t)P.bscale0 * (uint64_t)(int64_t)P.mult1);
     }
     return (int32_t)(off + (int64_t)(k * P.stride0));
+}
+
+// ---------------------------------------------------------------------------- path steps
+// The arithmetic of one path's steps, shared by the wavefront kernels (which keep a
+// path's state in the path buffers between launches) and the persistent kernel (which
+// keeps it in registers), so both form the same floats in the same order.
+
+// Pixel of linear index lin of a view's window. Bands of 8 rows walked column by
+// column, so the 64 paths of a wavefront start in one 8 x 8 tile (coherent primary and
+// shadow rays). A bijection onto the window for any size; the last band holds the
+// leftover rows.
+__device__ __forceinline__ void window_pixel(const View& V, int lin, int& x, int& y) {
+    const int winH = V.win_h;
+    const int fb = winH >> 3;
+    const int band = lin / (V.win_w * 8);
+    const int hb = band < fb ? 8 : winH - fb * 8;
+    const int idxb = lin - band * V.win_w * 8;
+    x = V.win_x0 + idxb / hb;
+    y = V.win_y0 + band * 8 + idxb % hb;
+}
+
+// Camera ray direction of Halton sample k at pixel (x, y), and the sample's Halton index:
+// SamplerIntegrator.cpp:42-52 with PerspCamera::GenerateRay (PerspCamera.cpp:23-29).
+__device__ __forceinline__ V3 camera_dir(const DevScene& S, const View& V, int x, int y, int k, int32_t& idx) {
+    const HaltonDev hd{V.exp0, V.scale1};
+    idx = halton_index_dev(V, x, y, k);
+    float s0 = halton_value(S, hd, idx, 0), s1 = halton_value(S, hd, idx, 1);
+    float px = (float)x + s0, py = (float)y + s1;
+    px = px / V.width;
+    py = py / V.height;
+    px = px * 2 - 1;
+    py = py * 2 - 1;
+    px *= V.camW, py *= V.camH;
+    return normalize(V.right * px + V.up * py + V.look * 1.0f);
+}
+
+// Radiance the hit's own light emits back along the ray (0 from its back side), added
+// when the previous bounce was specular or transmissive, PathTracingIntegrator.cpp:54-57.
+__device__ __forceinline__ V3 hit_emission(const DevScene& S, const Inter& it, V3 d) {
+    if (it.light < 0) return V3{0, 0, 0};
+    const DevLight& Ld = S.lights[it.light];
+    V3 nact = it.front ? it.n : -it.n;
+    return dot(nact, -d) < 0 ? V3{0, 0, 0} : V3{Ld.L[0], Ld.L[1], Ld.L[2]};
+}
+
+// A surface vertex (a hit with a material) after shade_surface: what the light loop
+// needs, and the continuation.
+struct SurfVertex {
+    float uL0, uL1;     // UniformSampleAllLights' light-sample dimensions
+    float pdfB;         // pdf of the BSDF-sampled MIS direction
+    int fBnz;           // that direction carries f != 0 (the MIS ray may be needed)
+    V3 wiB, misF;       // the MIS direction and f * cos along it
+    int32_t dim;        // sampler dimension after the vertex
+    bool cont, lp;      // the path continues; through a specular / transmissive sample
+    V3 beta2, no, wiC;  // throughput after the continuation, and the next ray
+};
+
+// Vertex stage of a surface hit: the light-sample and MIS-direction dimensions
+// (UniformSampleAllLights, :175-278: uL = Get2D(), uB = Get2D(), one Get1D per eligible
+// non-delta light), the BSDF-sampled MIS direction (shared by all lights), and for the
+// path tracer the continuation sample and Russian roulette (:79-102).
+__device__ __forceinline__ void shade_surface(const DevScene& S, const View& V, const Inter& it, V3 d, V3 beta,
+                                              int32_t idx, int32_t dim, int bounce, int32_t texel, int integrator,
+                                              int arg_order, SurfVertex& sv) {
+    const HaltonDev hd{V.exp0, V.scale1};
+    Bsdf bs = make_bsdf(S, it, (uint32_t)idx, (uint32_t)(dim + 2), texel);
+    V3 wo = -d;
+    float uL0 = halton_value(S, hd, idx, dim), uL1 = halton_value(S, hd, idx, dim + 1);
+    float uB0 = halton_value(S, hd, idx, dim + 2), uB1 = halton_value(S, hd, idx, dim + 3);
+    const bool direct = integrator == C2_INTEGRATOR_DIRECT;
+    int own = it.light;
+    // the path tracer skips the hit's own light (:181-184), direct lighting does not
+    // (DirectLightingIntegrator.cpp:68-71)
+    int ownElig = (!direct && own >= 0 && S.lights[own].eligible) ? 1 : 0;
+    int nGet1D = V.nd_elig - ownElig;  // one Get1D per eligible non-delta light, :240
+    V3 misF{0, 0, 0}, wiB{0, 0, 0};
+    float pdfB = 0.f;
+    int fBnz = 0;
+    if (nGet1D > 0) {
+        int typeB = 0, stB = kSampleOk;
+        V3 wiFail{0, 0, 0};
+        V3 fB = bsdf_sample(bs, it, wo, uB0, uB1, T_ALL & ~T_SPEC, wiB, pdfB, typeB, &stB, &wiFail);
+        if (!is_zero(fB)) {
+            fBnz = 1;
+            misF = fB * fmax_ref(0.f, dot(it.n, wiB));
+        } else if (direct && stB == kSampleZero && pdfB == 0.f) {
+            // Direct lighting keeps going after a zero sample
+            // (DirectLightingIntegrator.cpp:120-150): the BxDF's local-frame wi is traced
+            // as it is, and f * Li * w / pdf is 0 / 0 when the ray reaches the light. With
+            // a nonzero pdf the term is 0, and with an unwritten pdf the reference reads an
+            // uninitialised value: both add nothing and are not traced.
+            fBnz = 1;
+            wiB = wiFail;
+            misF = fB * fmax_ref(0.f, dot(it.n, wiB));
+        }
+    }
+    sv.uL0 = uL0;
+    sv.uL1 = uL1;
+    sv.pdfB = pdfB;
+    sv.fBnz = fBnz;
+    sv.wiB = wiB;
+    sv.misF = misF;
+    dim += 4 + nGet1D;
+    sv.cont = false;
+    sv.lp = false;
+    sv.beta2 = beta;
+    sv.no = V3{0, 0, 0};
+    sv.wiC = V3{0, 0, 0};
+    if (!direct) {
+        // the path tracer continues (the direct-lighting path ends here):
+        // sampler->Get1D() and sampler->Get2D() as arguments of one call, :79
+        float u2x, u2y;
+        uint32_t sdim;
+        if (arg_order == C2_ARGORDER_RTL) {
+            sdim = (uint32_t)dim;
+            u2x = halton_value(S, hd, idx, dim);
+            u2y = halton_value(S, hd, idx, dim + 1);
+        } else {
+            sdim = (uint32_t)(dim + 1);
+            u2x = halton_value(S, hd, idx, dim + 1);
+            u2y = halton_value(S, hd, idx, dim + 2);
+        }
+        // (the Get1D value only selects the BxDF; every supported material has one)
+        dim += 3;
+        bs.rdim = sdim;
+        V3 wiC{0, 0, 0};
+        float pdfC = 0.f;
+        int typeC = 0;
+        V3 fC = bsdf_sample(bs, it, wo, u2x, u2y, T_ALL, wiC, pdfC, typeC);
+        if (!(fabsf(pdfC) == 0.f || is_zero(fC))) {
+            bool specular = (typeC & T_SPEC) != 0;
+            float cosine = specular ? 1.0f : fmax_ref(0.f, (typeC & T_TRANS) ? dot(-it.n, wiC) : dot(it.n, wiC));
+            V3 beta2 = beta * (fC * cosine / pdfC);
+            bool cont = true;
+            if (bounce > 3) {  // Russian roulette, :96-102
+                float qq = fmax_ref(.05f, 1.f - length(beta2));
+                float ur = halton_value(S, hd, idx, dim);
+                dim++;
+                if (ur < qq) cont = false;
+                else beta2 = beta2 / (1 - qq);
+            }
+            sv.beta2 = beta2;
+            if (cont) {
+                sv.cont = true;
+                sv.lp = specular || (typeC & T_TRANS);
+                sv.no = spawn_origin(it.p, it.n, wiC);
+                sv.wiC = wiC;
+            }
+        }
+    }
+    sv.dim = dim;
+}
+
+// The vertex as the light loop sees it (rebuilt from the vertex records by k_nee): the
+// bitangent is n x dpdu.
+__device__ __forceinline__ Inter nee_inter(V3 p, V3 n, V3 dpdu, V3 dir, int32_t entity, int32_t material,
+                                           int32_t light, bool front) {
+    Inter it{};
+    it.p = p;
+    it.n = n;
+    it.dpdu = dpdu;
+    it.dpdv = cross(n, dpdu);
+    it.dir = dir;
+    it.entity = entity;
+    it.material = material;
+    it.light = light;
+    it.front = front;
+    return it;
+}
+
+// Light sampling half of EsimateDirect for one light (:199-230). When the sample carries
+// f * Li != 0, calls use(so, sd, C) with the shadow ray (origin, unnormalised direction to
+// the sampled point) and the unshadowed contribution, and returns true. (A callback rather
+// than out-parameters: the caller's stores stay inside the branch, which keeps k_nee at
+// 168 instead of 177 VGPRs.)
+template <class Use>
+__device__ __forceinline__ bool light_term(const DevScene& S, const DevLight& Lt, bool delta, const Inter& it,
+                                           const Bsdf& bs, V3 wo, float uL0, float uL1, Use&& use) {
+    V3 lightPos;
+    float pdfL;
+    V3 Li = sample_li(S, Lt, it.p, it.n, uL0, uL1, lightPos, pdfL);
+    V3 wi = normalize(lightPos - it.p);
+    if (pdfL != 0.f && !is_zero(Li)) {
+        V3 f = bsdf_f(bs, it, wo, wi) * fmax_ref(0.f, dot(it.n, wi));
+        float pB = bsdf_pdf(bs, it, wo, wi, T_ALL & ~T_SPEC);
+        if (!is_zero(f)) {
+            V3 C = delta ? f * Li / pdfL : f * Li * power_heuristic(pdfL, pB) / pdfL;
+            bool back = dot(lightPos - it.p, it.n) < 0;  // SpawnRayTo
+            V3 so = it.p + (back ? -it.n : it.n) * 1e-4f;
+            V3 sd = lightPos - so;
+            use(so, sd, C);
+            return true;
+        }
+    }
+    return false;
+}
+
+// The shared MIS ray, Scene::Intersect(isec.SpawnRay(wi)) from the vertex (sp, sn): only
+// the light owning the hit entity can receive f * Le * w / pdf_bsdf (:255-273). Returns
+// that light (-1 none) and the term in Lb.
+template <bool KD, class Lds>
+__device__ __forceinline__ int mis_term(const DevScene& S, const View& V, V3 sp, V3 sn, V3 wi, V3 f, float pdfB,
+                                        int own, int integrator, Lds& lds, V3& Lb) {
+    V3 o = spawn_origin(sp, sn, wi);
+    Hit h;
+    int lh = -1;
+    Lb = V3{0, 0, 0};
+    bool found;
+    if constexpr (KD)
+        found = kd_traverse<false>(S, V.root, V.kd_lo, V.kd_hi, o, wi, 0.f, INFINITY, -1, h, lds);
+    else
+        found = traverse<false>(S, V.root, o, wi, 0.f, INFINITY, -1, h, lds);
+    if (found) {
+        int4 e = S.entities[h.entity];
+        int li = e.w;
+        if (li >= 0 && (integrator == C2_INTEGRATOR_DIRECT || li != own)) {
+            const DevLight& Lt = S.lights[li];
+            if (Lt.eligible && Lt.type == C2_LIGHT_AREA) {
+                float pdfL = pdf_light(S, Lt, sp, sn, wi);
+                if (pdfL != 0.f) {
+                    float weight = power_heuristic(pdfB, pdfL);
+                    // lightIsec.Le(-wi) with the hit's un-flipped normal
+                    V3 hp, hn, hdu;
+                    bool front;
+                    if (e.x == C2_SHAPE_TRIANGLE) {
+                        tri_frame(S, e.y, wi, h.u, h.v, hn, hdu, front);
+                    } else {
+                        analytic_frame(e.x, S.spheres + (size_t)e.y * 22, o, wi, h.t, hp, hn, hdu, front);
+                    }
+                    V3 nact = front ? hn : -hn;
+                    V3 Li = dot(nact, -wi) < 0 ? V3{0, 0, 0} : V3{Lt.L[0], Lt.L[1], Lt.L[2]};
+                    if (!is_zero(Li)) {
+                        Lb = f * Li * weight / pdfB;
+                        lh = li;
+                    }
+                }
+            }
+        }
+    }
+    return lh;
+}
+
+// FrameBuffer::AddSample, FrameBuffer.cpp:20-32 (weight 1); a NaN or Inf sample adds 0.
+__device__ __forceinline__ void film_add(int mode, float4* film, float4* comp, double4* film64, int pixel, V3 c) {
+    if (any_nan(c)) c = V3{0, 0, 0};
+    if (any_inf(c)) c = V3{0, 0, 0};
+    if (mode == C2_FILM_F64) {
+        double4 f = film64[pixel];
+        f.x += (double)c.x;
+        f.y += (double)c.y;
+        f.z += (double)c.z;
+        f.w += 1.0;
+        film64[pixel] = f;
+    } else if (mode == C2_FILM_F32_KAHAN) {
+        float4 f = film[pixel], cp = comp[pixel];
+        float y, tt;
+        y = c.x - cp.x; tt = f.x + y; cp.x = (tt - f.x) - y; f.x = tt;
+        y = c.y - cp.y; tt = f.y + y; cp.y = (tt - f.y) - y; f.y = tt;
+        y = c.z - cp.z; tt = f.z + y; cp.z = (tt - f.z) - y; f.z = tt;
+        f.w += 1.f;
+        film[pixel] = f;
+        comp[pixel] = cp;
+    } else {
+        float4 f = film[pixel];
+        V3 acc = V3{f.x, f.y, f.z} + c * 1.f;
+        film[pixel] = make_float4(acc.x, acc.y, acc.z, f.w + 1.f);
+    }
 }
 
 // Block-aggregated appends: one atomic per block of kBlock threads instead of one per
@@ -377,15 +645,8 @@ __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, 
         const int v = P.pv[lo].x;
         const View& V = P.views[v];
         lin -= P.pv[lo].y;
-        // Pixel order: bands of 8 rows walked column by column, so the 64 paths of a
-        // wavefront start in one 8 x 8 tile (coherent primary and shadow rays). A
-        // bijection onto the window for any size; the last band holds the leftover rows.
-        const int winH = V.win_h;
-        const int fb = winH >> 3;
-        const int band = lin / (V.win_w * 8);
-        const int hb = band < fb ? 8 : winH - fb * 8;
-        const int idxb = lin - band * V.win_w * 8;
-        int x = V.win_x0 + idxb / hb, y = V.win_y0 + band * 8 + idxb % hb;
+        int x, y;
+        window_pixel(V, lin, x, y);
         int pixel = y * V.width + x;
         // training pixels: a phase per sample, so the samples of one launch train different pixels
         const int phase = P.cache_bounce >= 0
@@ -395,17 +656,8 @@ __global__ void __launch_bounds__(kBlock) k_generate(DevScene S, PathBuffers B, 
         int tp = P.cache_bounce >= 0 ? block_append(&B.counters[5], train, sm) : -1;  // uniform condition
         if (tp >= P.train_paths) train = false;  // beyond the record capacity: an ordinary path
         if (!live) continue;
-        HaltonDev hd{V.exp0, V.scale1};
-        int32_t idx = halton_index_dev(V, x, y, P.sample + js);
-        float s0 = halton_value(S, hd, idx, 0), s1 = halton_value(S, hd, idx, 1);
-        float px = (float)x + s0, py = (float)y + s1;
-        px = px / V.width;
-        py = py / V.height;
-        // PerspCamera::GenerateRay
-        px = px * 2 - 1;
-        py = py * 2 - 1;
-        px *= V.camW, py *= V.camH;
-        V3 d = normalize(V.right * px + V.up * py + V.look * 1.0f);
+        int32_t idx;
+        const V3 d = camera_dir(S, V, x, y, P.sample + js, idx);
         B.ray_o[i] = make_float4(V.eye.x, V.eye.y, V.eye.z, 0.f);
         B.ray_d[i] = make_float4(d.x, d.y, d.z, 0.f);
         B.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
@@ -588,7 +840,6 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
         if (live) {
             int4 meta = B.meta[p];
             const View& V = P.views[B.vid[p]];
-            const HaltonDev hd{V.exp0, V.scale1};
             float4 hh = B.hit[p];
             int ent = __float_as_int(hh.w);
             int flags = 0;
@@ -604,13 +855,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                 // emission when the previous bounce was specular / transmissive, :54-57
                 float4 Lpre = B.L[p];
                 if (meta.w & F_LIGHTPATH) {
-                    V3 Le{0, 0, 0};
-                    if (it.light >= 0) {
-                        const DevLight& Ld = S.lights[it.light];
-                        V3 nact = it.front ? it.n : -it.n;
-                        Le = dot(nact, -d) < 0 ? V3{0, 0, 0} : V3{Ld.L[0], Ld.L[1], Ld.L[2]};
-                    }
-                    V3 L = V3{Lpre.x, Lpre.y, Lpre.z} + beta * Le;
+                    V3 L = V3{Lpre.x, Lpre.y, Lpre.z} + beta * hit_emission(S, it, d);
                     Lpre = make_float4(L.x, L.y, L.z, 0.f);
                     B.L[p] = Lpre;
                 }
@@ -640,92 +885,25 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                     }
                     // the Kd texel k_surface found, read only for a textured material
                     const int32_t texel = S.materials[it.material].kd_texture > 0 ? B.texel[p] : 0;
-                    Bsdf bs = make_bsdf(S, it, (uint32_t)idx, (uint32_t)(dim + 2), texel);
-                    V3 wo = -d;
-                    // UniformSampleAllLights: uL = Get2D(), uB = Get2D(), :179-180
-                    float uL0 = halton_value(S, hd, idx, dim), uL1 = halton_value(S, hd, idx, dim + 1);
-                    float uB0 = halton_value(S, hd, idx, dim + 2), uB1 = halton_value(S, hd, idx, dim + 3);
-                    const bool direct = P.integrator == C2_INTEGRATOR_DIRECT;
-                    int own = it.light;
-                    // the path tracer skips the hit's own light (:181-184), direct lighting
-                    // does not (DirectLightingIntegrator.cpp:68-71)
-                    int ownElig = (!direct && own >= 0 && S.lights[own].eligible) ? 1 : 0;
-                    int nGet1D = V.nd_elig - ownElig;  // one Get1D per eligible non-delta light, :240
-                    V3 misF{0, 0, 0}, wiB{0, 0, 0};
-                    float pdfB = 0.f;
-                    int fBnz = 0;
-                    if (nGet1D > 0) {
-                        int typeB = 0, stB = kSampleOk;
-                        V3 wiFail{0, 0, 0};
-                        V3 fB = bsdf_sample(bs, it, wo, uB0, uB1, T_ALL & ~T_SPEC, wiB, pdfB, typeB, &stB, &wiFail);
-                        if (!is_zero(fB)) {
-                            fBnz = 1;
-                            misF = fB * fmax_ref(0.f, dot(it.n, wiB));
-                        } else if (direct && stB == kSampleZero && pdfB == 0.f) {
-                            // Direct lighting keeps going after a zero sample
-                            // (DirectLightingIntegrator.cpp:120-150): the BxDF's local-frame
-                            // wi is traced as it is, and f * Li * w / pdf is 0 / 0 when the
-                            // ray reaches the light. With a nonzero pdf the term is 0, and
-                            // with an unwritten pdf the reference reads an uninitialised
-                            // value: both add nothing and are not traced.
-                            fBnz = 1;
-                            wiB = wiFail;
-                            misF = fB * fmax_ref(0.f, dot(it.n, wiB));
-                        }
-                    }
-                    B.vtx_p[p] = make_float4(it.p.x, it.p.y, it.p.z, uL0);
-                    B.vtx_n[p] = make_float4(it.n.x, it.n.y, it.n.z, uL1);
-                    B.vtx_du[p] = make_float4(it.dpdu.x, it.dpdu.y, it.dpdu.z, __int_as_float(fBnz));
-                    B.vtx_d[p] = make_float4(d.x, d.y, d.z, pdfB);
-                    B.vtx_i[p] = make_int4(ent, it.material, own, it.front ? 1 : 0);
-                    B.mis_w[p] = make_float4(wiB.x, wiB.y, wiB.z, 0.f);
-                    B.mis_f[p] = make_float4(misF.x, misF.y, misF.z, 0.f);
+                    SurfVertex sv;
+                    shade_surface(S, V, it, d, beta, idx, dim, bounce, texel, P.integrator, P.arg_order, sv);
+                    B.vtx_p[p] = make_float4(it.p.x, it.p.y, it.p.z, sv.uL0);
+                    B.vtx_n[p] = make_float4(it.n.x, it.n.y, it.n.z, sv.uL1);
+                    B.vtx_du[p] = make_float4(it.dpdu.x, it.dpdu.y, it.dpdu.z, __int_as_float(sv.fBnz));
+                    B.vtx_d[p] = make_float4(d.x, d.y, d.z, sv.pdfB);
+                    B.vtx_i[p] = make_int4(ent, it.material, it.light, it.front ? 1 : 0);
+                    B.mis_w[p] = make_float4(sv.wiB.x, sv.wiB.y, sv.wiB.z, 0.f);
+                    B.mis_f[p] = make_float4(sv.misF.x, sv.misF.y, sv.misF.z, 0.f);
                     B.misflag[p] = 0;
                     B.mis_hit[p] = -1;
-                    dim += 4 + nGet1D;
+                    dim = sv.dim;
                     flags = F_NEE;
-                    if (!direct) {
-                        // the path tracer continues: the direct-lighting path ends here
-                        // continuation: sampler->Get1D() and sampler->Get2D() as arguments of one call, :79
-                        float u2x, u2y;
-                        uint32_t sdim;
-                        if (P.arg_order == C2_ARGORDER_RTL) {
-                            sdim = (uint32_t)dim;
-                            u2x = halton_value(S, hd, idx, dim);
-                            u2y = halton_value(S, hd, idx, dim + 1);
-                        } else {
-                            sdim = (uint32_t)(dim + 1);
-                            u2x = halton_value(S, hd, idx, dim + 1);
-                            u2y = halton_value(S, hd, idx, dim + 2);
-                        }
-                        // (the Get1D value only selects the BxDF; every supported material has one)
-                        dim += 3;
-                        bs.rdim = sdim;
-                        V3 wiC{0, 0, 0};
-                        float pdfC = 0.f;
-                        int typeC = 0;
-                        V3 fC = bsdf_sample(bs, it, wo, u2x, u2y, T_ALL, wiC, pdfC, typeC);
-                        if (!(fabsf(pdfC) == 0.f || is_zero(fC))) {
-                            bool specular = (typeC & T_SPEC) != 0;
-                            float cosine = specular ? 1.0f
-                                                    : fmax_ref(0.f, (typeC & T_TRANS) ? dot(-it.n, wiC) : dot(it.n, wiC));
-                            bool lp = specular || (typeC & T_TRANS);
-                            beta2 = beta * (fC * cosine / pdfC);
-                            cont = true;
-                            if (bounce > 3) {  // Russian roulette, :96-102
-                                float qq = fmax_ref(.05f, 1.f - length(beta2));
-                                float ur = halton_value(S, hd, idx, dim);
-                                dim++;
-                                if (ur < qq) cont = false;
-                                else beta2 = beta2 / (1 - qq);
-                            }
-                            if (cont) {
-                                V3 no = spawn_origin(it.p, it.n, wiC);
-                                B.ray_o[p] = make_float4(no.x, no.y, no.z, 0.f);
-                                B.ray_d[p] = make_float4(wiC.x, wiC.y, wiC.z, 0.f);
-                                if (lp) flags |= F_LIGHTPATH;
-                            }
-                        }
+                    beta2 = sv.beta2;
+                    if (sv.cont) {
+                        cont = true;
+                        B.ray_o[p] = make_float4(sv.no.x, sv.no.y, sv.no.z, 0.f);
+                        B.ray_d[p] = make_float4(sv.wiC.x, sv.wiC.y, sv.wiC.z, 0.f);
+                        if (sv.lp) flags |= F_LIGHTPATH;
                     }
                 }
             }
@@ -813,15 +991,8 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
             int4 meta = B.meta[p];
             int4 vi = B.vtx_i[p];
             float4 vp = B.vtx_p[p], vn = B.vtx_n[p], vd = B.vtx_du[p], vdir = B.vtx_d[p];
-            it.p = V3{vp.x, vp.y, vp.z};
-            it.n = V3{vn.x, vn.y, vn.z};
-            it.dpdu = V3{vd.x, vd.y, vd.z};
-            it.dpdv = cross(it.n, it.dpdu);
-            it.dir = V3{vdir.x, vdir.y, vdir.z};
-            it.entity = vi.x;
-            it.material = vi.y;
-            it.light = vi.z;
-            it.front = vi.w != 0;
+            it = nee_inter(V3{vp.x, vp.y, vp.z}, V3{vn.x, vn.y, vn.z}, V3{vd.x, vd.y, vd.z},
+                           V3{vdir.x, vdir.y, vdir.z}, vi.x, vi.y, vi.z, vi.w != 0);
             // B.texel still holds this vertex's texel (k_surface rewrites it next bounce)
             if (it.material >= 0 && S.materials[it.material].kd_texture > 0) texel = B.texel[p];
             hidx = (uint32_t)meta.y;
@@ -851,25 +1022,13 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
                 // live across the loop: fewer registers, more waves per SIMD
                 const Bsdf bs = make_bsdf(S, it, hidx, 0u, texel);
                 // (i) light sampling, :199-230
-                V3 lightPos;
-                float pdfL;
-                V3 Li = sample_li(S, Lt, it.p, it.n, uL0, uL1, lightPos, pdfL);
-                V3 wi = normalize(lightPos - it.p);
-                if (pdfL != 0.f && !is_zero(Li)) {
-                    V3 f = bsdf_f(bs, it, wo, wi) * fmax_ref(0.f, dot(it.n, wi));
-                    float pB = bsdf_pdf(bs, it, wo, wi, T_ALL & ~T_SPEC);
-                    if (!is_zero(f)) {
-                        V3 C = delta ? f * Li / pdfL : f * Li * power_heuristic(pdfL, pB) / pdfL;
-                        bool back = dot(lightPos - it.p, it.n) < 0;  // SpawnRayTo
-                        V3 so = it.p + (back ? -it.n : it.n) * 1e-4f;
-                        V3 sd = lightPos - so;
-                        B.sh_o[slot] = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
-                        B.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, __int_as_float(root));
-                        B.sh_c[slot] = make_float4(C.x, C.y, C.z, 0.f);
-                        v = 2;
-                        shadow = true;
-                    }
-                }
+                light_term(S, Lt, delta, it, bs, wo, uL0, uL1, [&](V3 so, V3 sd, V3 C) {
+                    B.sh_o[slot] = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
+                    B.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, __int_as_float(root));
+                    B.sh_c[slot] = make_float4(C.x, C.y, C.z, 0.f);
+                    v = 2;
+                    shadow = true;
+                });
                 // (ii) BSDF sampling needs the MIS ray when Pdf_Li != 0, :235-257
                 if (!delta && misCand && !misHit && pdf_light(S, Lt, it.p, it.n, misDir) != 0.f) misHit = true;
             }
@@ -920,43 +1079,9 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassP
         V3 sp{mp.x, mp.y, mp.z}, sn{mn.x, mn.y, mn.z}, wi{mw.x, mw.y, mw.z}, f{mf.x, mf.y, mf.z};
         float pdfB = B.vtx_d[p].w;
         int own = B.vtx_i[p].z;
-        V3 o = spawn_origin(sp, sn, wi);
-        Hit h;
-        int lh = -1;
-        V3 Lb{0, 0, 0};
         const View& V = P.views[B.vid[p]];
-        bool found;
-        if constexpr (KD)
-            found = kd_traverse<false>(S, V.root, V.kd_lo, V.kd_hi, o, wi, 0.f, INFINITY, -1, h, lds);
-        else
-            found = traverse<false>(S, V.root, o, wi, 0.f, INFINITY, -1, h, lds);
-        if (found) {
-            int4 e = S.entities[h.entity];
-            int li = e.w;
-            if (li >= 0 && (P.integrator == C2_INTEGRATOR_DIRECT || li != own)) {
-                const DevLight& Lt = S.lights[li];
-                if (Lt.eligible && Lt.type == C2_LIGHT_AREA) {
-                    float pdfL = pdf_light(S, Lt, sp, sn, wi);
-                    if (pdfL != 0.f) {
-                        float weight = power_heuristic(pdfB, pdfL);
-                        // lightIsec.Le(-wi) with the hit's un-flipped normal
-                        V3 hp, hn, hdu;
-                        bool front;
-                        if (e.x == C2_SHAPE_TRIANGLE) {
-                            tri_frame(S, e.y, wi, h.u, h.v, hn, hdu, front);
-                        } else {
-                            analytic_frame(e.x, S.spheres + (size_t)e.y * 22, o, wi, h.t, hp, hn, hdu, front);
-                        }
-                        V3 nact = front ? hn : -hn;
-                        V3 Li = dot(nact, -wi) < 0 ? V3{0, 0, 0} : V3{Lt.L[0], Lt.L[1], Lt.L[2]};
-                        if (!is_zero(Li)) {
-                            Lb = f * Li * weight / pdfB;
-                            lh = li;
-                        }
-                    }
-                }
-            }
-        }
+        V3 Lb;
+        const int lh = mis_term<KD>(S, V, sp, sn, wi, f, pdfB, own, P.integrator, lds, Lb);
         B.mis_L[p] = make_float4(Lb.x, Lb.y, Lb.z, 0.f);
         B.mis_hit[p] = lh;
     }
@@ -1003,8 +1128,8 @@ __global__ void __launch_bounds__(kBlock) k_resolve(PathBuffers B, PassParams P,
     }
 }
 
-// FrameBuffer::AddSample, FrameBuffer.cpp:20-32 (weight 1, in sample order)
-// The film adds of a chunk's samples of one pixel run in one thread in sample order, so
+// FrameBuffer::AddSample, FrameBuffer.cpp:20-32 (weight 1, in sample order): the film
+// adds of a chunk's samples of one pixel run in one thread in sample order, so
 // every pixel's sum is formed in the reference's order (sample 0, 1, ...).
 __global__ void __launch_bounds__(kBlock) k_accumulate(PathBuffers B, PassParams P, int mode, float4* film,
                                                        float4* comp, double4* film64) {
@@ -1012,34 +1137,208 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(PathBuffers B, PassParams
         for (int js = 0; js < P.nsamp; js++) {
             const int i = js * P.pix_count + t;
             float4 L4 = B.L[i];
-            V3 c{L4.x, L4.y, L4.z};
-            if (any_nan(c)) c = V3{0, 0, 0};
-            if (any_inf(c)) c = V3{0, 0, 0};
-            int pixel = B.meta[i].x;
-            if (mode == C2_FILM_F64) {
-                double4 f = film64[pixel];
-                f.x += (double)c.x;
-                f.y += (double)c.y;
-                f.z += (double)c.z;
-                f.w += 1.0;
-                film64[pixel] = f;
-            } else if (mode == C2_FILM_F32_KAHAN) {
-                float4 f = film[pixel], cp = comp[pixel];
-                float y, tt;
-                y = c.x - cp.x; tt = f.x + y; cp.x = (tt - f.x) - y; f.x = tt;
-                y = c.y - cp.y; tt = f.y + y; cp.y = (tt - f.y) - y; f.y = tt;
-                y = c.z - cp.z; tt = f.z + y; cp.z = (tt - f.z) - y; f.z = tt;
-                f.w += 1.f;
-                film[pixel] = f;
-                comp[pixel] = cp;
-            } else {
-                float4 f = film[pixel];
-                V3 acc = V3{f.x, f.y, f.z} + c * 1.f;
-                film[pixel] = make_float4(acc.x, acc.y, acc.z, f.w + 1.f);
-            }
+            film_add(mode, film, comp, film64, B.meta[i].x, V3{L4.x, L4.y, L4.z});
         }
     }
 }
+
+// ============================================================================ persistent kernel
+
+// One persistent launch: every pixel of the listed views is one work item, handed out in
+// list order from a device counter; a lane that takes a pixel traces samples
+// [sample_begin, min(spp, sample_end)) of it one after another, each path from the camera
+// ray to its end in registers, and adds each sample to the film itself, so a pixel's
+// samples are added in sample order, as k_accumulate adds them.
+struct PersistParams {
+    const View* views;
+    const int2* pv;        // (view, first item), ascending by first item
+    int32_t npv;
+    int32_t nitems;        // pixels of the listed views' windows
+    int32_t* counter;      // next unclaimed item; 0 when the launch starts
+    int32_t sample_begin, sample_end;
+    int32_t steps;         // path steps (direct lighting: the transparent chain limit)
+    int32_t arg_order, integrator, film_mode;
+    float4* film;
+    float4* comp;
+    double4* film64;
+    const int32_t* elig;
+    unsigned long long* totals;  // extension, shadow, MIS rays
+};
+
+// The closest-hit and any-hit stacks of a lane are never live at once, so they share
+// the block's LDS (a lane's entries lie at offsets = lane mod BS in both).
+template <int BS>
+union PersistLds {
+    TravLds<false, BS> closest;
+    TravLds<true, BS> any;
+};
+
+// BS threads per block; W > 0 limits the registers so that at least W waves fit on a SIMD
+// (the rest spill to scratch), W = 0 gives the kernel the registers it needs.
+template <int BS, int W>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W > 0 ? W : 1)))
+k_persist(DevScene S, PersistParams Q) {
+    __shared__ PersistLds<BS> lds;
+    const int lane = __lane_id();
+    const uint64_t lower = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    const bool direct = Q.integrator == C2_INTEGRATOR_DIRECT;
+    // the lane's pixel
+    int v = 0, x = 0, y = 0, pixel = 0, k = 0, kend = 0;
+    bool done = false, live = false;
+    // the lane's path
+    V3 o{0, 0, 0}, d{0, 0, 0}, beta{1, 1, 1}, L{0, 0, 0};
+    int32_t idx = 0, dim = 0, b = 0;
+    bool lightpath = false;
+    unsigned long long next = 0, nsh = 0, nmis = 0;
+    while (true) {
+        // lanes whose pixel has no sample left take the next items, one atomic per wave
+        const bool want = !done && !live && k >= kend;
+        const uint64_t m = __ballot(want);
+        if (m) {
+            const int first = __ffsll((unsigned long long)m) - 1;
+            int base = 0;
+            if (lane == first) base = atomicAdd(Q.counter, __popcll(m));
+            base = __shfl(base, first);
+            if (want) {
+                const int item = base + __popcll(m & lower);
+                if (item >= Q.nitems) {
+                    done = true;
+                } else {
+                    int lo = 0, hi = Q.npv - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (Q.pv[mid].y <= item) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    v = Q.pv[lo].x;
+                    const View& V = Q.views[v];
+                    window_pixel(V, item - Q.pv[lo].y, x, y);
+                    pixel = (int)V.film_off + y * V.width + x;
+                    k = Q.sample_begin;
+                    kend = min(V.spp, Q.sample_end);
+                }
+            }
+        }
+        if (!__ballot(!done)) break;  // the queue is drained and every path has ended
+        if (!done && !live && k < kend) {  // next sample of the pixel: the camera ray
+            const View& V = Q.views[v];
+            d = camera_dir(S, V, x, y, k, idx);
+            o = V.eye;
+            beta = V3{1.f, 1.f, 1.f};
+            L = V3{0.f, 0.f, 0.f};
+            dim = 2;
+            b = 0;
+            lightpath = true;
+            live = true;
+        }
+        if (!live) continue;
+        const View& V = Q.views[v];
+        bool cont = false;
+        if (b < Q.steps) {
+            // extension ray (k_extend)
+            Hit h;
+            const bool hit = traverse<false, BS>(S, V.root, o, d, 0.f, INFINITY, -1, h, lds.closest);
+            next++;
+            const int ent = hit ? h.entity : -1;
+            h.entity = ent;
+            if (ent < 0) {
+                // a ray that left the scene: the skybox (k_surface)
+                if (S.tex && V.sky[0] > 0) L = L + beta * sky_lookup(S, V.sky, d);
+            } else {
+                const int4 e = S.entities[ent];
+                int32_t texel = 0;  // Kd texel of a textured material (k_surface)
+                if (S.tex && e.z >= 0) {
+                    const int kd = S.materials[e.z].kd_texture;
+                    if (kd > 0) {
+                        float u, vv;
+                        surface_uv(S, e, o, d, h, u, vv);
+                        texel = texel_index(S, kd, u, vv);
+                    }
+                }
+                // vertex (k_shade)
+                Inter it = make_inter(S, o, d, h);
+                if (lightpath) L = L + beta * hit_emission(S, it, d);
+                if (it.material < 0) {
+                    // transparent: SpawnRay(dir), no dimensions consumed
+                    if (!direct || it.light < 0) {
+                        o = spawn_origin(it.p, it.n, d);
+                        cont = true;
+                    }
+                } else {
+                    SurfVertex sv;
+                    shade_surface(S, V, it, d, beta, idx, dim, b, texel, Q.integrator, Q.arg_order, sv);
+                    dim = sv.dim;
+                    // light loop (k_nee, k_mis, k_occlude, k_resolve)
+                    const Inter itn = nee_inter(it.p, it.n, it.dpdu, d, ent, it.material, it.light, it.front);
+                    const V3 wo = -itn.dir;
+                    const int ne = V.num_elig, eb = V.elig_begin;
+                    bool misHit = false;  // some light has Pdf_Li != 0 along the MIS direction
+                    if (sv.fBnz)
+                        for (int j = 0; j < ne && !misHit; j++) {
+                            const int li = Q.elig[eb + j];
+                            const DevLight Lt = S.lights[li];
+                            if ((direct || itn.light != li) && Lt.type != C2_LIGHT_POINT &&
+                                pdf_light(S, Lt, itn.p, itn.n, sv.wiB) != 0.f)
+                                misHit = true;
+                        }
+                    int lh = -1;
+                    V3 Lb{0, 0, 0};
+                    if (misHit) {
+                        lh = mis_term<false>(S, V, itn.p, itn.n, sv.wiB, sv.misF, sv.pdfB, itn.light, Q.integrator,
+                                             lds.closest, Lb);
+                        nmis++;
+                    }
+                    // sum over the eligible lights in light order (k_resolve's order)
+                    V3 U{0, 0, 0};
+                    for (int j = 0; j < ne; j++) {
+                        const int li = Q.elig[eb + j];
+                        V3 E{0, 0, 0};
+                        bool any = false;
+                        if (direct || itn.light != li) {
+                            const DevLight Lt = S.lights[li];
+                            const bool delta = Lt.type == C2_LIGHT_POINT;
+                            const Bsdf bs = make_bsdf(S, itn, (uint32_t)idx, 0u, texel);
+                            light_term(S, Lt, delta, itn, bs, wo, sv.uL0, sv.uL1, [&](V3 so, V3 sd, V3 C) {
+                                nsh++;
+                                Hit hs;
+                                if (!traverse<true, BS>(S, V.root, so, sd, 0.f, 1 - 1e-5f, delta ? -1 : Lt.entity,
+                                                        hs, lds.any)) {
+                                    E = E + C;
+                                    any = true;
+                                }
+                            });
+                        }
+                        if (li == lh) {
+                            E = E + Lb;
+                            any = true;
+                        }
+                        if (any) U = U + E;
+                    }
+                    L = L + beta * U;
+                    lightpath = false;
+                    if (sv.cont) {
+                        cont = true;
+                        o = sv.no;
+                        d = sv.wiC;
+                        beta = sv.beta2;
+                        lightpath = sv.lp;
+                    }
+                }
+            }
+        }
+        b++;
+        if (!cont || b >= Q.steps) {  // the path has ended: FrameBuffer::AddSample
+            film_add(Q.film_mode, Q.film, Q.comp, Q.film64, pixel, L);
+            k++;
+            live = false;
+        }
+    }
+    if (next) atomicAdd(&Q.totals[0], next);
+    if (nsh) atomicAdd(&Q.totals[1], nsh);
+    if (nmis) atomicAdd(&Q.totals[2], nmis);
+}
+
+__global__ void k_set_i32(int32_t* p, int32_t v) { *p = v; }
 
 __global__ void k_zero4(float4* p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1692,6 +1991,12 @@ struct c2_pt {
     };
     std::vector<PassTab> passes;  // a single renderer uses passes[0] for every sample
     int2* pv = nullptr;
+    // persistent kernel: every view as (view, first item), views in order of falling spp
+    // (the longest pixels are handed out first), the items and the item counter
+    int2* ppv = nullptr;
+    int32_t nppv = 0;
+    int64_t pitems = 0;
+    int32_t* pcounter = nullptr;
     int64_t film_pixels = 0;
     float4* film = nullptr;
     float4* comp = nullptr;
@@ -1843,6 +2148,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
         View V{};
         V.width = v.width;
         V.height = v.height;
+        V.spp = v.spp;
         V.root = descs[k]->num_entities > 0 ? SG.offs[k].node : -1;
         if (SG.kd) {
             V.root = SG.kdm[k].root;
@@ -1930,6 +2236,21 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     DeviceArena& A = pt->arena;
     View* dviews = A.upload(pt->views);
     pt->pv = A.upload(pv);
+    {
+        std::vector<int> order(n);
+        for (int k = 0; k < n; k++) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return vs[a].spp > vs[c].spp; });
+        std::vector<int2> ppv;
+        int64_t items = 0;
+        for (int k : order) {
+            ppv.push_back(make_int2(k, (int32_t)items));
+            items += view_pixels(k);
+        }
+        pt->pitems = items;
+        pt->nppv = n;
+        pt->ppv = A.upload(ppv);
+        pt->pcounter = A.alloc<int32_t>(1);
+    }
     P.views = dviews;
     if (elig.empty()) elig.push_back(-1);
     const int32_t* delig = A.upload(elig);
@@ -2255,6 +2576,93 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
         }
 }
 
+// The persistent kernel serves the BVH accelerator with the path tracer or direct
+// lighting; the radiance cache (training between launches) and ray capture (one bounce of
+// a pass) need the wavefront.
+bool persistent_ok(const c2_pt* pt) {
+    return !pt->accel->scene.kd && pt->pp.cache_bounce < 0 && pt->cap_bounce < 0;
+}
+
+template <int BS, int W>
+int persist_grid() {
+    // one block per free slot of every CU (the kernel loops until the queue is drained),
+    // per device
+    static int grid[64] = {};
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) fail(C2_ERR_UNSUPPORTED, "device index beyond 63");
+    if (!grid[dev]) {
+        int cus = 0, nb = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_persist<BS, W>), BS, 0));
+        grid[dev] = std::max(1, nb) * std::max(1, cus);
+    }
+    return grid[dev];
+}
+
+// Enqueues samples [sample_begin, + sample_count) of every view as one persistent launch
+// (after a one-thread kernel zeroes the item counter: no memset node in a captured graph).
+template <int BS, int W>
+void launch_persist(hipStream_t s, const DevScene& S, const PersistParams& Q) {
+    hipLaunchKernelGGL((k_persist<BS, W>), dim3(persist_grid<BS, W>()), dim3(BS), 0, s, S, Q);
+}
+
+// Launch shapes of the persistent kernel: threads per block x minimum waves per SIMD.
+struct PersistShape {
+    int block, waves;
+    void (*launch)(hipStream_t, const DevScene&, const PersistParams&);
+};
+const PersistShape kPersistShapes[] = {
+    {128, 0, &launch_persist<128, 0>}, {128, 4, &launch_persist<128, 4>}, {128, 6, &launch_persist<128, 6>},
+    {128, 8, &launch_persist<128, 8>}, {256, 0, &launch_persist<256, 0>}, {256, 4, &launch_persist<256, 4>},
+    {256, 6, &launch_persist<256, 6>}, {256, 8, &launch_persist<256, 8>}, {512, 0, &launch_persist<512, 0>},
+    {512, 4, &launch_persist<512, 4>}, {512, 6, &launch_persist<512, 6>}, {512, 8, &launch_persist<512, 8>},
+};
+
+const PersistShape* persist_shape(int block, int waves) {
+    for (const PersistShape& p : kPersistShapes)
+        if (p.block == block && p.waves == waves) return &p;
+    return nullptr;
+}
+
+void enqueue_persistent(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipStream_t s, int block,
+                        int waves) {
+    if (!persistent_ok(pt)) fail(C2_ERR_UNSUPPORTED, "persistent kernel: needs the BVH, no radiance cache, no capture");
+    if (sample_begin < 0 || sample_count < 0) fail(C2_ERR_INVALID, "render: negative sample range");
+    if (pt->pitems >= (1ll << 30)) fail(C2_ERR_UNSUPPORTED, "persistent kernel: more than 2^30 pixels");
+    const PassParams& P = pt->pp;
+    const DevScene S = pt->accel->scene.dev;
+    PersistParams Q{};
+    Q.views = P.views;
+    Q.pv = pt->ppv;
+    Q.npv = pt->nppv;
+    Q.nitems = (int32_t)pt->pitems;
+    Q.counter = pt->pcounter;
+    Q.sample_begin = sample_begin;
+    Q.sample_end = (int32_t)std::min<int64_t>((int64_t)sample_begin + sample_count, INT32_MAX);
+    Q.steps = S.num_entities == 0 ? 0
+              : P.integrator == C2_INTEGRATOR_DIRECT ? (P.max_depth > 0 ? kDlChainLimit : 0)
+                                                     : P.max_depth;
+    Q.arg_order = P.arg_order;
+    Q.integrator = P.integrator;
+    Q.film_mode = pt->opts.film_mode;
+    Q.film = pt->film;
+    Q.comp = pt->comp;
+    Q.film64 = pt->film64;
+    Q.elig = pt->B.elig;
+    Q.totals = pt->B.totals;
+    const PersistShape* shape = persist_shape(block, waves);
+    if (!shape) fail(C2_ERR_INVALID, "persistent kernel: no such launch shape (threads per block, waves per SIMD)");
+    hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, pt->pcounter, 0);
+    shape->launch(s, S, Q);
+    HIP_CHECK(hipGetLastError());
+    pt->launches += 2;
+    for (const View& V : pt->views) {
+        const int64_t ns = std::max<int64_t>(0, std::min<int64_t>(V.spp, Q.sample_end) - sample_begin);
+        pt->samples += ns * V.win_w * V.win_h;
+    }
+}
+
 // Film of view v as FrameBuffer::GetImageDataRGB32F returns it: color / weight,
 // rows flipped (y = 0 at the top).
 void read_film(c2_pt* pt, int v, float* rgb) {
@@ -2503,6 +2911,9 @@ extern "C" int c2_util_stream_copy(int64_t bytes, int32_t reps, double* avg_ms) 
 // once into a hipGraph and replayed with one launch.
 struct c2_batch {
     std::unique_ptr<c2_pt> pt;
+    int32_t schedule = C2_SCHED_WAVEFRONT;
+    int32_t block = 256;  // persistent kernel: threads per block
+    int32_t waves = 0;    // persistent kernel: minimum waves per SIMD (0: as the registers allow)
     int32_t max_spp = 0;
     int64_t samples_per_render = 0;  // sum over scenes of width * height * spp
     int64_t renders = 0;
@@ -2519,6 +2930,17 @@ struct c2_batch {
         drop_graph();
     }
 };
+
+namespace {
+// The whole render of a batch on stream s: the film reset, then every sample pass of the
+// wavefront or the one persistent launch.
+void enqueue_batch(c2_batch* b, hipStream_t s) {
+    c2_pt* pt = b->pt.get();
+    enqueue_film_reset(pt, s);
+    if (b->schedule == C2_SCHED_PERSISTENT) enqueue_persistent(pt, 0, b->max_spp, s, b->block, b->waves);
+    else enqueue_passes(pt, 0, b->max_spp, s, true);
+}
+}  // namespace
 
 extern "C" {
 
@@ -2550,41 +2972,81 @@ int c2_batch_autotune(c2_batch* b, c2_batch_tuning* out) {
         if (!b) fail(C2_ERR_INVALID, "null batch");
         c2_pt* pt = b->pt.get();
         hipStream_t s = pt->stream;
-        std::vector<int32_t> chunks;
-        for (int32_t c = pt->chunk_cap; c >= (1 << 16) && chunks.size() < 3; c /= 2) chunks.push_back(c);
-        if (chunks.empty()) chunks.push_back(pt->chunk_cap);
-        const int32_t qbs[] = {4096, 8192, 16384};
         hipEvent_t e0, e1;
         HIP_CHECK(hipEventCreate(&e0));
         HIP_CHECK(hipEventCreate(&e1));
         const uint32_t saveProf = pt->prof_mask;
         pt->prof_mask = 0;
-        float best = INFINITY;
-        int32_t bestC = pt->chunk, bestQ = pt->opts.queue_blocks;
+        auto timed = [&](auto&& enqueue) {
+            HIP_CHECK(hipEventRecord(e0, s));
+            enqueue();
+            HIP_CHECK(hipEventRecord(e1, s));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return ms;
+        };
         int tried = 0;
+        // 1. the wavefront: sample pass 0 over paths in flight x queue-kernel grid caps
+        std::vector<int32_t> chunks;
+        for (int32_t c = pt->chunk_cap; c >= (1 << 16) && chunks.size() < 3; c /= 2) chunks.push_back(c);
+        if (chunks.empty()) chunks.push_back(pt->chunk_cap);
+        const int32_t qbs[] = {4096, 8192, 16384};
+        float bestW = INFINITY;
+        int32_t bestC = pt->chunk, bestQ = pt->opts.queue_blocks;
         enqueue_passes(pt, 0, 1, s, true);  // warm-up: code objects, caches
         for (int32_t c : chunks)
             for (int32_t q : qbs) {
                 pt->chunk = c;
                 pt->opts.queue_blocks = q;
-                (void)hipEventRecord(e0, s);
-                enqueue_passes(pt, 0, 1, s, true);
-                (void)hipEventRecord(e1, s);
-                HIP_CHECK(hipEventSynchronize(e1));
-                float ms = 0;
-                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                const float ms = timed([&] { enqueue_passes(pt, 0, 1, s, true); });
                 tried++;
-                if (ms < best) {
-                    best = ms;
+                if (ms < bestW) {
+                    bestW = ms;
                     bestC = c;
                     bestQ = q;
                 }
             }
+        pt->chunk = bestC;
+        pt->opts.queue_blocks = bestQ;
+        // 2. the persistent kernel: sample 0 of every pixel over the block sizes
+        const bool pok = persistent_ok(pt);
+        float bestP = INFINITY;
+        int32_t bestB = 256, bestWv = 0;
+        if (pok) {
+            for (const PersistShape& sh : kPersistShapes) {
+                enqueue_persistent(pt, 0, 1, s, sh.block, sh.waves);  // warm-up
+                const float ms = timed([&] { enqueue_persistent(pt, 0, 1, s, sh.block, sh.waves); });
+                tried++;
+                if (ms < bestP) {
+                    bestP = ms;
+                    bestB = sh.block;
+                    bestWv = sh.waves;
+                }
+            }
+        }
+        // 3. the two winners over the whole render, each as the captured graph that
+        //    c2_batch_render(b, 1) replays (captured, replayed once to warm, then timed)
+        auto graph_ms = [&](int32_t sched) {
+            b->schedule = sched;
+            b->block = bestB;
+            b->waves = bestWv;
+            b->drop_graph();
+            int rc = c2_batch_render(b, 1);
+            if (rc != C2_OK) fail(rc, c2_last_error());
+            HIP_CHECK(hipStreamSynchronize(s));
+            return timed([&] { HIP_CHECK(hipGraphLaunch(b->exec, s)); });
+        };
+        b->renders = 0;
+        const float wMs = graph_ms(C2_SCHED_WAVEFRONT);
+        const float pMs = pok ? graph_ms(C2_SCHED_PERSISTENT) : INFINITY;
+        b->schedule = pMs < wMs ? C2_SCHED_PERSISTENT : C2_SCHED_WAVEFRONT;
+        b->block = bestB;
+        b->waves = bestWv;
+        b->renders = 0;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         pt->prof_mask = saveProf;
-        pt->chunk = bestC;
-        pt->opts.queue_blocks = bestQ;
         pt->samples = 0;
         b->drop_graph();  // a captured graph holds the old launch shapes
         enqueue_film_reset(pt, s);
@@ -2592,9 +3054,39 @@ int c2_batch_autotune(c2_batch* b, c2_batch_tuning* out) {
         if (out) {
             out->paths_in_flight = bestC;
             out->queue_blocks = bestQ;
-            out->pass_ms = best;
+            out->pass_ms = bestW;
             out->candidates = tried;
+            out->schedule = b->schedule;
+            out->block_threads = bestB;
+            out->waves_per_simd = bestWv;
+            out->sample0_ms_persistent = pok ? bestP : -1.0;
+            out->render_ms_wavefront = wMs;
+            out->render_ms_persistent = pok ? pMs : -1.0;
         }
+    });
+}
+
+int c2_batch_configure(c2_batch* b, const c2_batch_tuning* cfg) {
+    return guard([&] {
+        if (!b || !cfg) fail(C2_ERR_INVALID, "null argument");
+        c2_pt* pt = b->pt.get();
+        if (cfg->schedule != C2_SCHED_WAVEFRONT && cfg->schedule != C2_SCHED_PERSISTENT)
+            fail(C2_ERR_INVALID, "schedule: C2_SCHED_WAVEFRONT or C2_SCHED_PERSISTENT");
+        if (cfg->schedule == C2_SCHED_PERSISTENT) {
+            if (!persist_shape(cfg->block_threads, cfg->waves_per_simd))
+                fail(C2_ERR_INVALID, "persistent launch shape: 128, 256 or 512 threads x 0, 4, 6 or 8 waves per SIMD");
+            if (!persistent_ok(pt))
+                fail(C2_ERR_UNSUPPORTED, "persistent kernel: needs the BVH, no radiance cache");
+        }
+        if (cfg->paths_in_flight < 0 || cfg->queue_blocks < 0) fail(C2_ERR_INVALID, "negative launch shape");
+        b->schedule = cfg->schedule;
+        if (cfg->schedule == C2_SCHED_PERSISTENT) {
+            b->block = cfg->block_threads;
+            b->waves = cfg->waves_per_simd;
+        }
+        if (cfg->paths_in_flight > 0) pt->chunk = std::min(cfg->paths_in_flight, pt->chunk_cap);
+        if (cfg->queue_blocks > 0) pt->opts.queue_blocks = cfg->queue_blocks;
+        b->drop_graph();
     });
 }
 
@@ -2604,16 +3096,14 @@ int c2_batch_render(c2_batch* b, int32_t use_graph) {
         c2_pt* pt = b->pt.get();
         hipStream_t s = pt->stream;
         if (!use_graph) {
-            enqueue_film_reset(pt, s);
-            enqueue_passes(pt, 0, b->max_spp, s, true);
+            enqueue_batch(b, s);
         } else {
             if (!b->exec) {
                 const uint32_t saveProf = pt->prof_mask;
                 pt->prof_mask = 0;  // no timing events inside the graph
                 HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
                 try {
-                    enqueue_film_reset(pt, s);
-                    enqueue_passes(pt, 0, b->max_spp, s, true);
+                    enqueue_batch(b, s);
                 } catch (...) {
                     hipGraph_t g = nullptr;
                     (void)hipStreamEndCapture(s, &g);
@@ -2659,6 +3149,9 @@ int c2_batch_get_stats(const c2_batch* b, c2_batch_stats* out) {
         out->passes = b->max_spp;
         out->paths_in_flight = b->pt->chunk;
         out->queue_blocks = b->pt->opts.queue_blocks;
+        out->schedule = b->schedule;
+        out->block_threads = b->block;
+        out->waves_per_simd = b->waves;
         out->graph_nodes = 0;
         if (b->graph) {
             size_t nn = 0;

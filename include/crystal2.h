@@ -401,23 +401,45 @@ typedef struct c2_batch_item {
     const c2_scene_desc* scene;
     int32_t width, height, spp;
 } c2_batch_item;
+/* Schedules of a batch render. WAVEFRONT: every sample pass as the queue kernels
+ * (generate, extend, shade, nee, occlude, mis, resolve, accumulate per bounce).
+ * PERSISTENT: one kernel launch whose lanes take pixels from a device counter and
+ * trace each sample's path from the camera to its end in registers (BVH accelerator,
+ * path tracer or direct lighting). Both give the same films bit for bit. */
+#define C2_SCHED_WAVEFRONT  0
+#define C2_SCHED_PERSISTENT 1
 typedef struct c2_batch_tuning {
-    int32_t paths_in_flight;  /* chosen paths per launch                         */
-    int32_t queue_blocks;     /* chosen grid cap of the queue kernels            */
-    double pass_ms;           /* sample pass 0 with the chosen values            */
+    int32_t paths_in_flight;  /* wavefront: paths per launch                     */
+    int32_t queue_blocks;     /* wavefront: grid cap of the queue kernels        */
+    double pass_ms;           /* wavefront: sample pass 0 with the chosen values */
     int32_t candidates;       /* configurations timed                            */
+    int32_t schedule;         /* C2_SCHED_*                                      */
+    int32_t block_threads;    /* persistent: threads per block (128, 256, 512)   */
+    int32_t waves_per_simd;   /* persistent: registers limited so that at least  */
+                              /* this many waves fit a SIMD (2, 4), or 0: none   */
+    double sample0_ms_persistent;  /* persistent: sample 0 of every pixel (-1: n/a) */
+    double render_ms_wavefront;    /* whole render, captured graph, each schedule's */
+    double render_ms_persistent;   /* best values (-1: persistent not applicable)   */
 } c2_batch_tuning;
 typedef struct c2_batch_stats {
     int64_t samples, samples_per_render;
     int64_t extension_rays, shadow_rays, mis_rays;
     int32_t passes, paths_in_flight, queue_blocks;
     int64_t graph_nodes;      /* nodes of the captured graph (0 before capture)  */
+    int32_t schedule, block_threads, waves_per_simd;
 } c2_batch_stats;
 int  c2_batch_create(const c2_batch_item* items, int32_t n, const c2_render_opts* opts, c2_batch** out);
 void c2_batch_destroy(c2_batch* batch);
-/* Times sample pass 0 over a small grid of paths-in-flight x queue_blocks values,
- * keeps the fastest (drops a captured graph) and clears the films. */
+/* Tunes each schedule on sample 0 (wavefront: pass 0 over paths-in-flight x
+ * queue_blocks; persistent: 128 / 256 / 512 threads per block x 0 / 2 / 4 waves per
+ * SIMD), then times both winners over
+ * the whole render as captured graphs and keeps the faster. Drops a captured graph and
+ * clears the films. */
 int  c2_batch_autotune(c2_batch* batch, c2_batch_tuning* out);
+/* Sets the schedule and launch shape explicitly (fields of c2_batch_tuning: schedule,
+ * block_threads and waves_per_simd, and paths_in_flight / queue_blocks when > 0; the
+ * timing fields are ignored). Drops a captured graph. */
+int  c2_batch_configure(c2_batch* batch, const c2_batch_tuning* cfg);
 /* Clears the films and renders every sample of every scene; asynchronous. */
 int  c2_batch_render(c2_batch* batch, int32_t use_graph);
 int  c2_batch_sync(c2_batch* batch);

@@ -1,7 +1,8 @@
 """Batched scenes: every film of a batch render (direct launches and a replayed
-hipGraph) is bit-identical to the same scene rendered alone with the same
-options. The items differ in size, spp, light count (2, 8, a point light) and
-materials, so the views share launches with different trip counts."""
+hipGraph, the wavefront schedule and the persistent kernel) is bit-identical to the
+same scene rendered alone with the same options. The items differ in size, spp, light
+count (2, 8, a point light) and materials, so the views share launches with different
+trip counts."""
 import os
 
 import numpy as np
@@ -100,6 +101,74 @@ def test_batch_direct_lighting(gpu_available):
     b.close()
 
 
+@pytest.mark.parametrize("block,waves", [(128, 0), (256, 0), (512, 0), (256, 4), (128, 6), (512, 8)])
+def test_persistent_films_equal_single_renders(gpu_available, block, waves):
+    """The persistent kernel (one launch; lanes take pixels from a device counter and
+    trace each path to its end) gives every film of the wavefront bit for bit, and traces
+    the same extension, shadow and MIS rays."""
+    items = _items()
+    ref = [_alone(*it) for it in items]
+    w = c2.BatchRenderer(items, DEPTH, arg_order="rtl")
+    w.render(False)
+    w.sync()
+    ws = w.stats()
+    w.close()
+    b = c2.BatchRenderer(items, DEPTH, arg_order="rtl")
+    b.configure("persistent", block_threads=block, waves_per_simd=waves)
+    for rep, use_graph in enumerate((False, True, True)):
+        print(f"case persistent block={block} waves={waves} use_graph={use_graph}", flush=True)
+        b.render(use_graph)
+        b.sync()
+        for i, r in enumerate(ref):
+            assert np.array_equal(b.film(i), r), f"item {i} block={block} graph={use_graph}"
+        if rep == 0:
+            st = b.stats()
+            assert st["schedule"] == "persistent" and st["block_threads"] == block
+            assert st["waves_per_simd"] == waves
+            for k in ("extension_rays", "shadow_rays", "mis_rays"):
+                assert st[k] == ws[k], k
+    st = b.stats()
+    assert 0 < st["graph_nodes"] <= 4  # film reset, counter reset, the persistent launch
+    b.close()
+
+
+@pytest.mark.parametrize("film", ["f64", "kahan"])
+def test_persistent_film_modes_and_direct_lighting(gpu_available, film):
+    items = _items()[:3]
+    for integrator in ("path", "direct"):
+        print(f"case persistent film={film} integrator={integrator}", flush=True)
+        ref = []
+        for sc, w, h, spp in items:
+            pt = c2.WavefrontPathIntegrator(sc, w, h, spp, DEPTH, arg_order="rtl", integrator=integrator, film=film)
+            pt.render()
+            pt.sync()
+            ref.append(pt.film())
+            pt.close()
+        b = c2.BatchRenderer(items, DEPTH, arg_order="rtl", integrator=integrator, film=film)
+        b.configure("persistent", block_threads=256)
+        b.render(True)
+        b.sync()
+        for i, r in enumerate(ref):
+            assert np.array_equal(b.film(i), r), f"item {i}"
+        b.close()
+
+
+def test_persistent_refuses_the_kdtree(gpu_available):
+    print("case persistent kdtree refused", flush=True)
+    b = c2.BatchRenderer(_items()[:2], DEPTH, arg_order="rtl", accel="kdtree")
+    with pytest.raises(c2.Crystal2Error) as e:
+        b.configure("persistent", block_threads=256)
+    assert e.value.code == -6  # C2_ERR_UNSUPPORTED
+    b.configure("wavefront", block_threads=100)  # the wavefront ignores the block size
+    b.close()
+    b = c2.BatchRenderer(_items()[:2], DEPTH, arg_order="rtl")
+    for bs, wv in ((100, 0), (256, 3)):
+        with pytest.raises(c2.Crystal2Error) as e:
+            b.configure("persistent", block_threads=bs, waves_per_simd=wv)
+        assert e.value.code == -1  # C2_ERR_INVALID
+    b.close()
+
+
 def test_batch_64_scene_workload(gpu_available):
     """The batched-scenes workload at its configured size: 64 sphere-field scenes with
     triangle counts log-uniform in 1k-200k, resolutions in {128, 256, 512}^2 and spp in
@@ -117,10 +186,19 @@ def test_batch_64_scene_workload(gpu_available):
         pt.close()
     b = c2.BatchRenderer(items, 8, arg_order="rtl")
     t = b.autotune()
-    assert t["candidates"] >= 3
+    print(f"autotune: {t}", flush=True)
+    assert t["candidates"] == 21 and t["render_ms_wavefront"] > 0 and t["render_ms_persistent"] > 0
     for rep in range(2):
         b.render(True)
         b.sync()
         bad = [i for i, r in enumerate(ref) if not np.array_equal(b.film(i), r)]
         assert not bad, f"replay {rep}: films {bad} differ"
+    # the schedule the autotuner did not keep gives the same films
+    other = "wavefront" if t["schedule"] == "persistent" else "persistent"
+    print(f"case batch 64-scene workload, {other}", flush=True)
+    b.configure(other, block_threads=t["block_threads"], waves_per_simd=t["waves_per_simd"])
+    b.render(True)
+    b.sync()
+    bad = [i for i, r in enumerate(ref) if not np.array_equal(b.film(i), r)]
+    assert not bad, f"{other}: films {bad} differ"
     b.close()
