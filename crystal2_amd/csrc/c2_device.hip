@@ -178,32 +178,47 @@ enum : int32_t {
     F_MIS = 8,        // MIS ray queued for this vertex
     F_TRAIN = 16,     // radiance-cache training path (index in bits 8..31)
 };
+// Flags of a bounce's vertex record (vtx_i.w)
+enum : int32_t {
+    V_FRONT = 1,     // the ray hit the front face
+    V_NEE = 2,       // the light loop runs at this vertex
+    V_CONTINUE = 4,  // the path continues (resolve sets beta <- beta2)
+};
 
+// Path state (ray, throughput, radiance, sampler state, hit) is indexed by path p and
+// lives across bounces. A bounce's vertex records are indexed by the vertex's position i
+// in the bounce's queue (the order k_surface, k_shade, k_nee and k_resolve all walk), so
+// those kernels read and write them coalesced although p is scattered once the queue is
+// grouped by shading class. Shadow rays are packed per wavefront: the 64 vertices
+// i = 64w .. 64w + 63 own slots [64w * lmax, (64w + 64) * lmax), and their rays fill them
+// from the start, light by light, each light's rays in lane order (see wave_slot).
 struct PathBuffers {
-    float4* ray_o;   // o.xyz
-    float4* ray_d;   // d.xyz
-    float4* beta;    // beta at the current vertex
-    float4* beta2;   // beta after the continuation (applied by resolve)
-    float4* L;       // path radiance
-    int4* meta;      // film pixel (all views' films concatenated), halton index, dimension, flags
-    int32_t* vid;    // view (scene of a batch) of the path
-    float4* hit;     // t, u, v, entity bits
-    float4* vtx_p;   // vertex position, uL.x
-    float4* vtx_n;   // facing shading normal, uL.y
-    float4* vtx_du;  // dpdu, (BSDF-sampled f != 0) bits
-    float4* vtx_d;   // incoming ray direction, pdf of the BSDF-sampled direction
-    int4* vtx_i;     // entity, material, attached light, front face
-    int32_t* texel;  // Kd texel of the extension hit (textured materials only)
-    float4* mis_w;   // BSDF-sampled direction (shared by all lights)
-    float4* mis_f;   // f * cos for that direction
-    float4* mis_L;   // resolved MIS term
-    int32_t* misflag;  // 1 = some light has Pdf_Li != 0 -> MIS ray traced
-    int32_t* mis_hit;  // light owning the MIS hit, -1 none
+    float4* ray_o;   // [p] o.xyz
+    float4* ray_d;   // [p] d.xyz
+    float4* beta;    // [p] beta at the current vertex
+    float4* L;       // [p] path radiance
+    int4* meta;      // [p] film pixel (all views' films concatenated), halton index, dimension, flags
+    int32_t* vid;    // [p] view (scene of a batch) of the path
+    float4* hit;     // [p] t, u, v, entity bits
+    float4* beta2;   // [i] beta after the continuation (applied by resolve)
+    float4* vtx_p;   // [i] vertex position, uL.x
+    float4* vtx_n;   // [i] facing shading normal, uL.y
+    float4* vtx_du;  // [i] dpdu, (BSDF-sampled f != 0) bits
+    float4* vtx_d;   // [i] incoming ray direction, pdf of the BSDF-sampled direction
+    int4* vtx_i;     // [i] entity, material, attached light, V_* flags (every queued vertex)
+    int2* vtx_h;     // [i] Halton index, view
+    int32_t* texel;  // [i] Kd texel of the extension hit (textured materials only)
+    float4* mis_w;   // [i] BSDF-sampled direction (shared by all lights)
+    float4* mis_f;   // [i] f * cos for that direction
+    float4* mis_L;   // [i] resolved MIS term
+    int32_t* mis_hit;  // [i] light owning the MIS hit, -1 none
+    int32_t* nee_out;  // [i] 1 = the MIS ray is traced
+    uint32_t* sh_mask; // [g * cap + i] bit j: light 32g + j of vertex i emitted a shadow ray
     const int32_t* elig;  // eligible lights (Flux != 0) in light order
-    float4* sh_o;    // shadow ray origin, ignore entity bits
-    float4* sh_d;    // shadow ray direction
-    float4* sh_c;    // unshadowed contribution
-    uint8_t* sh_vis; // 0 = none / occluded, 2 = pending, 1 = unoccluded
+    float4* sh_o;    // [slot] shadow ray origin, ignore entity bits
+    float4* sh_d;    // [slot] shadow ray direction, BVH root bits
+    float4* sh_c;    // [slot] unshadowed contribution
+    uint8_t* sh_vis; // [slot] 1 = unoccluded, 0 = occluded
     int32_t* queue[2];
     // radiance cache
     float4* nrc_feat;   // [path][4]: query features
@@ -213,11 +228,12 @@ struct PathBuffers {
     uint8_t* trec_valid;
     float4* bfeat;      // training batch features [cap][4]
     float4* btarget;    // training batch targets [cap]
-    int32_t* sh_list;
-    int32_t* mis_list;
+    int32_t* sh_list;   // slots of the bounce's shadow rays
+    int32_t* mis_list;  // vertices (queue positions) tracing the MIS ray
     int32_t* queue_s;   // this bounce's queue grouped by shading class (see k_class_count)
     uint8_t* cls;       // shading class per queue index
     int32_t* ccount;    // [kClasses] paths per class, [kClasses] scatter cursors
+    int32_t cap;        // paths the buffers hold (sh_mask's group stride)
     int32_t* counters;  // [0..1] queue counts, [2] shadow, [3] mis, [4] cache queries, [5] training
                         // paths, [6] captured rays, [7] training rows
     unsigned long long* totals;  // extension, shadow, mis rays (64-bit running totals)
@@ -595,11 +611,22 @@ __device__ __forceinline__ int32_t block_append(int32_t* counter, bool pred, int
     return sm[kWaves] + sm[w] + __popcll(m & lower);
 }
 
-// Appends list entries p * E + jbase + j for the set bits j < nb of `bits`, in
-// light-major order within the block (all of the block's light-jbase rays, then the
-// next light's, ...). sm: kWaves * kAppendLights + 1 ints of LDS.
+// Slot of a shadow ray in its wavefront's packed region: the region of the 64 vertices
+// of queue positions i & ~63 starts at (i & ~63) * lmax; `off` is the wave's rays of the
+// lights before this one, `m` the ballot of the lanes whose vertex emits this light's ray.
+__device__ __forceinline__ int32_t wave_slot(int32_t i, int lmax, int32_t off, uint64_t m) {
+    const int lane = __lane_id();
+    const uint64_t lower = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    return (i & ~63) * lmax + off + __popcll(m & lower);
+}
+
+// Appends the shadow rays of a group of nb lights (bit j of `bits`: the group's light j
+// emits one) to `list` in light-major order within the block (all of the block's rays of
+// the group's first light, then the next light's, ...). The entries are the rays' slots
+// (wave_slot, with `off` the wave's rays before the group). sm: kWaves * kAppendLights + 1
+// ints of LDS.
 __device__ __forceinline__ void block_append_lights(int32_t* counter, uint32_t bits, int nb, int32_t* sm,
-                                                    int32_t* list, int p, int E, int jbase) {
+                                                    int32_t* list, int32_t i, int lmax, int32_t off) {
     const int lane = __lane_id(), w = (int)threadIdx.x >> 6;
     __syncthreads();
     for (int j = 0; j < nb; j++) {
@@ -621,8 +648,9 @@ __device__ __forceinline__ void block_append_lights(int32_t* counter, uint32_t b
     const uint64_t lower = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
     const int32_t base = sm[kWaves * kAppendLights];
     for (int j = 0; j < nb; j++) {
-        uint64_t m = __ballot((bits >> j) & 1u);
-        if ((bits >> j) & 1u) list[base + sm[w * kAppendLights + j] + __popcll(m & lower)] = p * E + jbase + j;
+        const uint64_t m = __ballot((bits >> j) & 1u);
+        if ((bits >> j) & 1u) list[base + sm[w * kAppendLights + j] + __popcll(m & lower)] = wave_slot(i, lmax, off, m);
+        off += __popcll(m);
     }
 }
 
@@ -818,7 +846,7 @@ __global__ void __launch_bounds__(kBlock) k_surface(DevScene S, PathBuffers B, P
         float4 ro = B.ray_o[p];
         float u, v;
         surface_uv(S, e, V3{ro.x, ro.y, ro.z}, d, Hit{ent, hh.x, hh.y, hh.z}, u, v);
-        B.texel[p] = texel_index(S, kd, u, v);
+        B.texel[i] = texel_index(S, kd, u, v);
     }
 }
 
@@ -843,6 +871,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
             float4 hh = B.hit[p];
             int ent = __float_as_int(hh.w);
             int flags = 0;
+            int32_t vent = -1, vmat = -1, vown = -1, vfl = 0;  // vertex record
             int32_t idx = meta.y, dim = meta.z;
             float4 b4 = B.beta[p];
             V3 beta{b4.x, b4.y, b4.z};
@@ -862,7 +891,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                 const bool training = (meta.w & F_TRAIN) != 0;
                 if (it.material >= 0 && P.cache_bounce >= 0 && !training && bounce == P.cache_bounce) {
                     // the path ends in the radiance cache: L += beta * cache(x, wo) after this kernel
-                    cache_features(S, it, d, B.nrc_feat + (int64_t)p * 4, B.texel + p);
+                    cache_features(S, it, d, B.nrc_feat + (int64_t)p * 4, B.texel + i);
                     query = true;
                 } else if (it.material < 0) {
                     // transparent: SpawnRay(dir), no dimensions consumed, :63-68. The
@@ -878,24 +907,26 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                     if (training) {  // training row: features, throughput and radiance so far
                         int64_t r = ((int64_t)(meta.w >> 8) * P.max_depth + bounce);
                         float4* rec = B.trec + r * 6;
-                        cache_features(S, it, d, rec, B.texel + p);
+                        cache_features(S, it, d, rec, B.texel + i);
                         rec[4] = make_float4(beta.x, beta.y, beta.z, 0.f);
                         rec[5] = Lpre;
                         B.trec_valid[r] = 1;
                     }
                     // the Kd texel k_surface found, read only for a textured material
-                    const int32_t texel = S.materials[it.material].kd_texture > 0 ? B.texel[p] : 0;
+                    const int32_t texel = S.materials[it.material].kd_texture > 0 ? B.texel[i] : 0;
                     SurfVertex sv;
                     shade_surface(S, V, it, d, beta, idx, dim, bounce, texel, P.integrator, P.arg_order, sv);
-                    B.vtx_p[p] = make_float4(it.p.x, it.p.y, it.p.z, sv.uL0);
-                    B.vtx_n[p] = make_float4(it.n.x, it.n.y, it.n.z, sv.uL1);
-                    B.vtx_du[p] = make_float4(it.dpdu.x, it.dpdu.y, it.dpdu.z, __int_as_float(sv.fBnz));
-                    B.vtx_d[p] = make_float4(d.x, d.y, d.z, sv.pdfB);
-                    B.vtx_i[p] = make_int4(ent, it.material, it.light, it.front ? 1 : 0);
-                    B.mis_w[p] = make_float4(sv.wiB.x, sv.wiB.y, sv.wiB.z, 0.f);
-                    B.mis_f[p] = make_float4(sv.misF.x, sv.misF.y, sv.misF.z, 0.f);
-                    B.misflag[p] = 0;
-                    B.mis_hit[p] = -1;
+                    B.vtx_p[i] = make_float4(it.p.x, it.p.y, it.p.z, sv.uL0);
+                    B.vtx_n[i] = make_float4(it.n.x, it.n.y, it.n.z, sv.uL1);
+                    B.vtx_du[i] = make_float4(it.dpdu.x, it.dpdu.y, it.dpdu.z, __int_as_float(sv.fBnz));
+                    B.vtx_d[i] = make_float4(d.x, d.y, d.z, sv.pdfB);
+                    B.vtx_h[i] = make_int2(idx, B.vid[p]);
+                    B.mis_w[i] = make_float4(sv.wiB.x, sv.wiB.y, sv.wiB.z, 0.f);
+                    B.mis_f[i] = make_float4(sv.misF.x, sv.misF.y, sv.misF.z, 0.f);
+                    vent = ent;
+                    vmat = it.material;
+                    vown = it.light;
+                    vfl = (it.front ? V_FRONT : 0) | V_NEE;
                     dim = sv.dim;
                     flags = F_NEE;
                     beta2 = sv.beta2;
@@ -910,7 +941,12 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
             if (cont) flags |= F_CONTINUE;
             flags |= meta.w & ~0xF;  // keep the training tag
             B.meta[p] = make_int4(meta.x, idx, dim, flags);
-            B.beta2[p] = make_float4(beta2.x, beta2.y, beta2.z, 0.f);
+            // the vertex record of every queued vertex (k_nee and k_resolve read its flags)
+            if (cont) {
+                vfl |= V_CONTINUE;
+                B.beta2[i] = make_float4(beta2.x, beta2.y, beta2.z, 0.f);
+            }
+            B.vtx_i[i] = make_int4(vent, vmat, vown, vfl);
         }
         int qpos = block_append(&B.counters[q ^ 1], live && cont, sm);
         if (qpos >= 0) next[qpos] = p;
@@ -958,29 +994,24 @@ __global__ void __launch_bounds__(kBlock) k_cache_collect(PathBuffers B, PassPar
 }
 
 // Light loop of UniformSampleAllLights / EsimateDirect (:181-278), one thread per
-// path vertex looping over the eligible lights in light order: the vertex record
-// and its BSDF are loaded and built once, then per light j the light sample
-// (shadow slot p * E + j) and the test whether the shared BSDF-sampled direction
+// vertex looping over the eligible lights in light order: the vertex record is loaded
+// and the BSDF built once per light, then per light j the light sample (a shadow ray in
+// the vertex's next free slot) and the test whether the shared BSDF-sampled direction
 // needs the MIS ray (Pdf_Li != 0).
-__global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
+// (Three waves per SIMD asked for: without it the allocator settles at 169 VGPRs, one past
+// the 168 of three waves; with it 168 and the same 20 B of scratch.)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
     __shared__ int32_t sm[kWaves * kAppendLights + 1];
     int n = B.counters[q];
-    const int32_t* queue = B.queue[q];
     const int E = P.lmax;
     int stride = gridDim.x * blockDim.x;
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + (int)threadIdx.x;
         bool live = i < n;
-        int p = live ? queue[i] : 0;
-        bool nee = false;
+        int4 vi = live ? B.vtx_i[i] : make_int4(-1, -1, -1, 0);
+        const bool nee = (vi.w & V_NEE) != 0;
         int eb = 0, ne = 0, root = -1;
-        if (live) {
-            nee = (B.meta[p].w & F_NEE) != 0;
-            const View& V = P.views[B.vid[p]];
-            eb = V.elig_begin;
-            ne = V.num_elig;
-            root = V.root;
-        }
         Inter it{};
         int32_t texel = 0;
         uint32_t hidx = 0;  // Halton index: the BSDF's Phong redraw stream
@@ -988,34 +1019,37 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
         float uL0 = 0.f, uL1 = 0.f;
         bool misCand = false;
         if (nee) {
-            int4 meta = B.meta[p];
-            int4 vi = B.vtx_i[p];
-            float4 vp = B.vtx_p[p], vn = B.vtx_n[p], vd = B.vtx_du[p], vdir = B.vtx_d[p];
+            const int2 vh = B.vtx_h[i];
+            const View& V = P.views[vh.y];
+            eb = V.elig_begin;
+            ne = V.num_elig;
+            root = V.root;
+            float4 vp = B.vtx_p[i], vn = B.vtx_n[i], vd = B.vtx_du[i], vdir = B.vtx_d[i];
             it = nee_inter(V3{vp.x, vp.y, vp.z}, V3{vn.x, vn.y, vn.z}, V3{vd.x, vd.y, vd.z},
-                           V3{vdir.x, vdir.y, vdir.z}, vi.x, vi.y, vi.z, vi.w != 0);
-            // B.texel still holds this vertex's texel (k_surface rewrites it next bounce)
-            if (it.material >= 0 && S.materials[it.material].kd_texture > 0) texel = B.texel[p];
-            hidx = (uint32_t)meta.y;
+                           V3{vdir.x, vdir.y, vdir.z}, vi.x, vi.y, vi.z, (vi.w & V_FRONT) != 0);
+            // this bounce's texel (k_surface wrote it for this queue position)
+            if (it.material >= 0 && S.materials[it.material].kd_texture > 0) texel = B.texel[i];
+            hidx = (uint32_t)vh.x;
             wo = -it.dir;
             uL0 = vp.w;
             uL1 = vn.w;
             misCand = __float_as_int(vd.w) != 0;
             if (misCand) {
-                float4 mw = B.mis_w[p];
+                float4 mw = B.mis_w[i];
                 misDir = V3{mw.x, mw.y, mw.z};
             }
         }
         bool misHit = false;
         uint32_t shbits = 0;
+        int32_t woff = 0, goff = 0;  // the wave's shadow rays so far; before this group of lights
         // block-uniform trip count (the appends below need every thread); lanes whose
         // view has fewer eligible lights idle through the tail
         for (int j = 0; j < E; j++) {
             const bool lj = j < ne;
             const int li = lj ? B.elig[eb + j] : -1;
             const bool delta = lj && S.lights[li].type == C2_LIGHT_POINT;
-            const int slot = p * E + j;
             bool shadow = false;
-            uint8_t v = 0;
+            float4 ro{}, rd{}, rc{};
             if (nee && lj && (P.integrator == C2_INTEGRATOR_DIRECT || it.light != li)) {
                 const DevLight Lt = S.lights[li];
                 // rebuilt per light from the (cached) material record rather than kept
@@ -1023,26 +1057,34 @@ __global__ void __launch_bounds__(kBlock) k_nee(DevScene S, PathBuffers B, PassP
                 const Bsdf bs = make_bsdf(S, it, hidx, 0u, texel);
                 // (i) light sampling, :199-230
                 light_term(S, Lt, delta, it, bs, wo, uL0, uL1, [&](V3 so, V3 sd, V3 C) {
-                    B.sh_o[slot] = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
-                    B.sh_d[slot] = make_float4(sd.x, sd.y, sd.z, __int_as_float(root));
-                    B.sh_c[slot] = make_float4(C.x, C.y, C.z, 0.f);
-                    v = 2;
+                    ro = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
+                    rd = make_float4(sd.x, sd.y, sd.z, __int_as_float(root));
+                    rc = make_float4(C.x, C.y, C.z, 0.f);
                     shadow = true;
                 });
                 // (ii) BSDF sampling needs the MIS ray when Pdf_Li != 0, :235-257
                 if (!delta && misCand && !misHit && pdf_light(S, Lt, it.p, it.n, misDir) != 0.f) misHit = true;
             }
-            if (live && lj) B.sh_vis[slot] = v;
+            // the ray goes to the wave's next free slots, in lane order
+            const uint64_t mj = __ballot(shadow);
+            if (shadow) {
+                const int32_t slot = wave_slot(i, E, woff, mj);
+                B.sh_o[slot] = ro;
+                B.sh_d[slot] = rd;
+                B.sh_c[slot] = rc;
+            }
+            woff += __popcll(mj);
             shbits |= shadow ? (1u << (j % kAppendLights)) : 0u;
             if (j % kAppendLights == kAppendLights - 1 || j == E - 1) {
-                block_append_lights(&B.counters[2], shbits, j % kAppendLights + 1, sm, B.sh_list, p, E,
-                                    j - j % kAppendLights);
+                block_append_lights(&B.counters[2], shbits, j % kAppendLights + 1, sm, B.sh_list, i, E, goff);
+                if (live) B.sh_mask[(j / kAppendLights) * B.cap + i] = shbits;
+                goff = woff;
                 shbits = 0;
             }
         }
-        if (misHit) B.misflag[p] = 1;
+        if (live) B.nee_out[i] = misHit ? 1 : 0;
         int mpos = block_append(&B.counters[3], misHit, sm);
-        if (mpos >= 0) B.mis_list[mpos] = p;
+        if (mpos >= 0) B.mis_list[mpos] = i;
     }
 }
 
@@ -1074,57 +1116,80 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassP
     int n = B.counters[3];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[2], (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int p = B.mis_list[i];
-        float4 mp = B.vtx_p[p], mn = B.vtx_n[p], mw = B.mis_w[p], mf = B.mis_f[p];
+        const int v = B.mis_list[i];  // the vertex's queue position
+        float4 mp = B.vtx_p[v], mn = B.vtx_n[v], mw = B.mis_w[v], mf = B.mis_f[v];
         V3 sp{mp.x, mp.y, mp.z}, sn{mn.x, mn.y, mn.z}, wi{mw.x, mw.y, mw.z}, f{mf.x, mf.y, mf.z};
-        float pdfB = B.vtx_d[p].w;
-        int own = B.vtx_i[p].z;
-        const View& V = P.views[B.vid[p]];
+        float pdfB = B.vtx_d[v].w;
+        int own = B.vtx_i[v].z;
+        const View& V = P.views[B.vtx_h[v].y];
         V3 Lb;
         const int lh = mis_term<KD>(S, V, sp, sn, wi, f, pdfB, own, P.integrator, lds, Lb);
-        B.mis_L[p] = make_float4(Lb.x, Lb.y, Lb.z, 0.f);
-        B.mis_hit[p] = lh;
+        B.mis_L[v] = make_float4(Lb.x, Lb.y, Lb.z, 0.f);
+        B.mis_hit[v] = lh;
     }
 }
 
 // L += beta * sum over eligible lights in light order of EsimateDirect's result:
-// the unoccluded light sample (0 + C) plus, for the light owning the MIS hit, Lb.
+// the unoccluded light sample (0 + C) plus, for the light owning the MIS hit, Lb. The
+// slots of the wave's shadow rays are found again by replaying k_nee's ballots over the
+// vertices' light masks (the grid-stride loop over block-uniform bases keeps vertex i in
+// lane i % 64, as in k_nee).
 __global__ void __launch_bounds__(kBlock) k_resolve(PathBuffers B, PassParams P, int q) {
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int p = queue[i];
-        int4 meta = B.meta[p];
-        if (meta.w & F_NEE) {
-            int lh = B.misflag[p] ? B.mis_hit[p] : -1;
-            V3 Lb{0, 0, 0};
+    const int E = P.lmax;
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {  // wave-uniform trip count
+        const int i = base + (int)threadIdx.x;
+        const bool live = i < n;
+        const int vfl = live ? B.vtx_i[i].w : 0;
+        const bool nee = (vfl & V_NEE) != 0;
+        int p = 0, lh = -1, eb = 0, ne = 0;
+        V3 Lb{0, 0, 0};
+        if (nee) {
+            p = queue[i];
+            if (B.nee_out[i]) lh = B.mis_hit[i];
             if (lh >= 0) {
-                float4 m = B.mis_L[p];
+                float4 m = B.mis_L[i];
                 Lb = V3{m.x, m.y, m.z};
             }
-            V3 U{0, 0, 0};
-            int slot0 = p * P.lmax;
-            const View& V = P.views[B.vid[p]];
-            for (int j = 0; j < V.num_elig; j++) {
-                int li = B.elig[V.elig_begin + j];
-                V3 E{0, 0, 0};
+            const View& V = P.views[B.vtx_h[i].y];
+            eb = V.elig_begin;
+            ne = V.num_elig;
+        }
+        V3 U{0, 0, 0};
+        int32_t woff = 0;
+        uint32_t bits = 0;
+        for (int j = 0; j < E; j++) {
+            if (j % kAppendLights == 0) bits = live ? B.sh_mask[(j / kAppendLights) * B.cap + i] : 0u;
+            const bool ray = (bits >> (j % kAppendLights)) & 1u;
+            const uint64_t mj = __ballot(ray);
+            if (nee && j < ne) {
+                const int li = B.elig[eb + j];
+                V3 E3{0, 0, 0};
                 bool any = false;
-                if (B.sh_vis[slot0 + j] == 1) {
-                    float4 c = B.sh_c[slot0 + j];
-                    E = E + V3{c.x, c.y, c.z};
-                    any = true;
+                if (ray) {
+                    const int32_t slot = wave_slot(i, E, woff, mj);
+                    if (B.sh_vis[slot] == 1) {
+                        const float4 c = B.sh_c[slot];
+                        E3 = E3 + V3{c.x, c.y, c.z};
+                        any = true;
+                    }
                 }
                 if (li == lh) {
-                    E = E + Lb;
+                    E3 = E3 + Lb;
                     any = true;
                 }
-                if (any) U = U + E;
+                if (any) U = U + E3;
             }
+            woff += __popcll(mj);
+        }
+        if (nee) {
             float4 b4 = B.beta[p], L4 = B.L[p];
             V3 L = V3{L4.x, L4.y, L4.z} + V3{b4.x, b4.y, b4.z} * U;
             B.L[p] = make_float4(L.x, L.y, L.z, 0.f);
         }
-        if (meta.w & F_CONTINUE) B.beta[p] = B.beta2[p];
+        if (vfl & V_CONTINUE) B.beta[queue[i]] = B.beta2[i];  // (a flagged vertex is live)
     }
 }
 
@@ -2090,7 +2155,10 @@ void alloc_path_buffers(PathBuffers& B, DeviceArena& A, size_t ch, int lmax, con
     B.vtx_du = A.alloc<float4>(ch);
     B.vtx_d = A.alloc<float4>(ch);
     B.vtx_i = A.alloc<int4>(ch);
-    B.misflag = A.alloc<int32_t>(ch);
+    B.vtx_h = A.alloc<int2>(ch);
+    B.nee_out = A.alloc<int32_t>(ch);
+    B.cap = (int32_t)ch;
+    B.sh_mask = A.alloc<uint32_t>(ch * (size_t)((lmax + kAppendLights - 1) / kAppendLights));
     B.mis_hit = A.alloc<int32_t>(ch);
     B.elig = elig;
     B.texel = A.alloc<int32_t>(ch);
@@ -2228,7 +2296,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : dflt;
     // keep the path buffers within ~16 GiB: 8 GiB in each of two lanes, or 16 GiB in a
     // batch's one lane
-    int64_t perPath = (int64_t)P.lmax * (3 * 16 + 1 + 4) + 16 * 14 + 16;
+    int64_t perPath = (int64_t)P.lmax * (3 * 16 + 4 + 1) + 16 * 15 + 48 + 4 * ((P.lmax + 31) / 32);
     int64_t maxChunk = std::max<int64_t>(4096, ((int64_t)(per_sample_passes ? 16 : 8) << 30) / perPath);
     chunk = std::max<int64_t>(1, std::min(std::min(chunk, std::max(maxPass, dflt)), maxChunk));
     pt->chunk = pt->chunk_cap = (int32_t)chunk;
