@@ -1,6 +1,8 @@
 // Headless driver for the reference renderer compiled from /root/reference.
 //
-// Test infrastructure only (oracle/_ref). It replaces the reference's GUI
+// Test infrastructure only (oracle/_ref). Built with -DC2_GPU_PLUGIN (make refgpu), the
+// render and hits commands hand the scene file to the GPU plugins first, so a config may
+// name "GPUBVH" and "WavefrontPathTracer". It replaces the reference's GUI
 // entry point (Crystal2.cpp:8-37 + src/CrystalEngine.cpp) with a command line
 // that calls the reference's own loaders, accelerator, sampler and integrator
 // and dumps raw results, so the CPU restatement in oracle/ and the GPU path can
@@ -49,8 +51,12 @@
 #include <SJson/SJson.h>
 #include <stbi/stb_image.h>
 #include <stbi/stb_image_write.h>
+#ifdef C2_GPU_PLUGIN
+#include "../crystal2_amd/plugin/crystal2_gpu.h"
+#endif
 
 #include <chrono>
+#include <stdexcept>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -77,6 +83,9 @@ Loaded load(const char* cfgPath, const char* scenePath) {
 
 int cmdRender(int argc, char** argv) {
     if (argc < 5) return 2;
+#ifdef C2_GPU_PLUGIN
+    crystal2_gpu::SetSceneFile(argv[3]);  // before Scene::CreateScene builds a "GPUBVH"
+#endif
     Loaded L = load(argv[2], argv[3]);
     auto camera = Camera::CreateCamera(L.sceneNode->GetMember("Camera"), L.info);
     auto integrator = crystal::Integrator::LoadIntegrator(L.cfgNode.get(), L.info);
@@ -99,6 +108,9 @@ int cmdRender(int argc, char** argv) {
 
 int cmdHits(int argc, char** argv) {
     if (argc < 6) return 2;
+#ifdef C2_GPU_PLUGIN
+    crystal2_gpu::SetSceneFile(argv[3]);
+#endif
     Loaded L = load(argv[2], argv[3]);
     auto objects = L.scene->GetObjects();
     std::unordered_map<const crystal::Entity*, int> index;
@@ -262,12 +274,24 @@ int main(int argc, char** argv) {
     }
     std::string cmd = argv[1];
     int rc = 2;
-    if (cmd == "render") rc = cmdRender(argc, argv);
-    else if (cmd == "hits") rc = cmdHits(argc, argv);
-    else if (cmd == "raycast") rc = cmdRaycast(argc, argv);
-    else if (cmd == "halton") rc = cmdHalton(argc, argv);
-    else if (cmd == "decode") rc = cmdDecode(argc, argv);
-    else if (cmd == "writehdr") rc = cmdWriteHdr(argc, argv);
+    try {
+        if (cmd == "render") rc = cmdRender(argc, argv);
+        else if (cmd == "hits") rc = cmdHits(argc, argv);
+        else if (cmd == "raycast") rc = cmdRaycast(argc, argv);
+        else if (cmd == "halton") rc = cmdHalton(argc, argv);
+        else if (cmd == "decode") rc = cmdDecode(argc, argv);
+        else if (cmd == "writehdr") rc = cmdWriteHdr(argc, argv);
+    } catch (const std::logic_error& e) {  // the plugins' std::invalid_argument
+        std::fprintf(stderr, "error: %s\n", e.what());
+        std::fflush(stderr);
+        std::_Exit(4);
+    } catch (const std::runtime_error& e) {
+        // the reference's `std::exception("msg")` (ref_prelude.h spells `exception` as
+        // runtime_error in this translation unit too, so it is named as such here)
+        std::fprintf(stderr, "error: %s\n", e.what());
+        std::fflush(stderr);
+        std::_Exit(4);
+    }
     if (rc == 2) std::fprintf(stderr, "bad arguments for %s\n", cmd.c_str());
     std::fflush(stdout);
     // The reference's FixedThreadPool workers are detached and still blocked on
