@@ -33,6 +33,11 @@ HIPFLAGS = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", f"--offload-arch=
             f"-I{os.path.join(HERE, '..', 'include')}"]
 
 
+# The MLP kernels keep their MFMA accumulators in VGPRs (the compiler's default puts them in
+# AGPRs, and every bias init and ReLU then copies them across: 2 instructions per element).
+EXTRA_HIPFLAGS = {"c2_nrc.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _run(cmd):
     print("+", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
@@ -53,7 +58,7 @@ def build(verbose: bool = True) -> str:
         if not os.path.exists(p):
             continue
         o = os.path.join(OUT_DIR, "obj", src + ".o")
-        _run([HIPCC, *HIPFLAGS, "-c", p, "-o", o])
+        _run([HIPCC, *HIPFLAGS, *EXTRA_HIPFLAGS.get(src, []), "-c", p, "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-pthread"])

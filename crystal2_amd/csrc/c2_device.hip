@@ -43,6 +43,7 @@
 #include "c2_sched.h"
 #include "c2_device.h"
 #include "c2_host.h"
+#include "c2_nrc.h"
 #include "c2_shade.h"
 
 namespace c2 {
@@ -221,12 +222,11 @@ struct PathBuffers {
     uint8_t* sh_vis; // [slot] 1 = unoccluded, 0 = occluded
     int32_t* queue[2];
     // radiance cache
-    float4* nrc_feat;   // [path][4]: query features
-    int32_t* nrc_list;  // paths querying the cache this bounce
+    float4* nrc_feat;   // [query][kNrcRec4]: packed query features, throughput and path, by query slot
     int32_t* tlist;     // training path -> path
-    float4* trec;       // [training path][depth][6]: features x4, beta, L after emission
+    float4* trec;       // [training path][depth][kNrcQ4 + 2]: packed query, beta, L after emission
     uint8_t* trec_valid;
-    float4* bfeat;      // training batch features [cap][4]
+    float4* bfeat;      // training batch features [cap][kNrcQ4]
     float4* btarget;    // training batch targets [cap]
     int32_t* sh_list;   // slots of the bounce's shadow rays
     int32_t* mis_list;  // vertices (queue positions) tracing the MIS ray
@@ -275,10 +275,13 @@ struct PassParams {
     int32_t train_paths;             // training path record capacity
 };
 
-// Radiance-cache query features: position, outgoing direction, facing normal,
-// roughness (diffuse 1, specular 0, GGX sqrt(ax ay), Phong sqrt(2 / (K + 2))), albedo.
-__device__ __forceinline__ void cache_features(const DevScene& S, const Inter& it, V3 d, float4* out,
-                                               const int32_t* texel) {
+// Radiance-cache query in the MLP kernels' packed layout (kNrcQ4 float4: position and
+// roughness; azimuth and cos theta of the outgoing direction and of the facing normal;
+// albedo and 1). Roughness: diffuse 1, specular 0, GGX sqrt(ax ay), Phong sqrt(2 / (K + 2)).
+struct CacheQuery {
+    float4 q[kNrcQ4];
+};
+__device__ __forceinline__ CacheQuery cache_query(const DevScene& S, const Inter& it, V3 d, const int32_t* texel) {
     float rough = 1.f;
     V3 alb{0.f, 0.f, 0.f};
     if (it.material >= 0) {
@@ -290,12 +293,12 @@ __device__ __forceinline__ void cache_features(const DevScene& S, const Inter& i
             rough = sqrtf(m.roughness[0] * m.roughness[1]);
         else if (m.type == C2_MAT_PHONG) rough = sqrtf(2.f / (m.phong_k + 2.f));
     }
-    out[0] = make_float4(it.p.x, it.p.y, it.p.z, -d.x);
-    out[1] = make_float4(-d.y, -d.z, it.n.x, it.n.y);
-    out[2] = make_float4(it.n.z, rough, alb.x, alb.y);
-    // azimuths of the outgoing direction and the normal, for the cache's input encoding
-    out[3] = make_float4(alb.z, atan2f(-d.z, -d.x) * 0.15915494f + 0.5f, atan2f(it.n.z, it.n.x) * 0.15915494f + 0.5f,
-                         0.f);
+    CacheQuery c;
+    c.q[0] = make_float4(it.p.x, it.p.y, it.p.z, rough);
+    c.q[1] = make_float4(atan2f(-d.z, -d.x) * 0.15915494f + 0.5f, -d.y * 0.5f + 0.5f,
+                         atan2f(it.n.z, it.n.x) * 0.15915494f + 0.5f, it.n.y * 0.5f + 0.5f);
+    c.q[2] = make_float4(alb.x, alb.y, alb.z, 1.f);
+    return c;
 }
 
 __device__ __forceinline__ int32_t halton_index_dev(const View& P, int x, int y, int k) {
@@ -863,8 +866,19 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
     for (int base = blockIdx.x * blockDim.x; base < n; base += stride) {
         int i = base + (int)threadIdx.x;
         bool live = i < n;
-        bool cont = false, query = false;
+        bool cont = false;
         int p = live ? queue[i] : 0;
+        // a path that ends in the radiance cache here gets its query slot first, so that its
+        // query is written where it is computed
+        int cpos = -1;
+        if (P.cache_bounce == bounce) {  // uniform over the launch
+            bool query = false;
+            if (live) {
+                const int ent = __float_as_int(B.hit[p].w);
+                query = ent >= 0 && S.entities[ent].z >= 0 && (B.meta[p].w & F_TRAIN) == 0;
+            }
+            cpos = block_append(&B.counters[4], query, sm);
+        }
         if (live) {
             int4 meta = B.meta[p];
             const View& V = P.views[B.vid[p]];
@@ -889,10 +903,14 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                     B.L[p] = Lpre;
                 }
                 const bool training = (meta.w & F_TRAIN) != 0;
-                if (it.material >= 0 && P.cache_bounce >= 0 && !training && bounce == P.cache_bounce) {
+                if (cpos >= 0) {
                     // the path ends in the radiance cache: L += beta * cache(x, wo) after this kernel
-                    cache_features(S, it, d, B.nrc_feat + (int64_t)p * 4, B.texel + i);
-                    query = true;
+                    // (cpos >= 0 exactly when it.material >= 0, the path is not a training path
+                    // and this is the cache bounce)
+                    const CacheQuery cq = cache_query(S, it, d, B.texel + i);
+                    float4* rec = B.nrc_feat + (int64_t)cpos * kNrcRec4;
+                    for (int k = 0; k < kNrcQ4; k++) rec[k] = cq.q[k];
+                    rec[kNrcQ4] = make_float4(beta.x, beta.y, beta.z, __int_as_float(p));
                 } else if (it.material < 0) {
                     // transparent: SpawnRay(dir), no dimensions consumed, :63-68. The
                     // direct-lighting integrator ends the path at a transparent emitter
@@ -906,10 +924,11 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                 } else {
                     if (training) {  // training row: features, throughput and radiance so far
                         int64_t r = ((int64_t)(meta.w >> 8) * P.max_depth + bounce);
-                        float4* rec = B.trec + r * 6;
-                        cache_features(S, it, d, rec, B.texel + i);
-                        rec[4] = make_float4(beta.x, beta.y, beta.z, 0.f);
-                        rec[5] = Lpre;
+                        float4* rec = B.trec + r * (kNrcQ4 + 2);
+                        const CacheQuery tq = cache_query(S, it, d, B.texel + i);
+                        for (int k = 0; k < kNrcQ4; k++) rec[k] = tq.q[k];
+                        rec[kNrcQ4] = make_float4(beta.x, beta.y, beta.z, 0.f);
+                        rec[kNrcQ4 + 1] = Lpre;
                         B.trec_valid[r] = 1;
                     }
                     // the Kd texel k_surface found, read only for a textured material
@@ -950,46 +969,74 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
         }
         int qpos = block_append(&B.counters[q ^ 1], live && cont, sm);
         if (qpos >= 0) next[qpos] = p;
-        if (P.cache_bounce >= 0) {  // uniform over the launch
-            int cpos = block_append(&B.counters[4], live && query, sm);
-            if (cpos >= 0) B.nrc_list[cpos] = p;
-        }
     }
 }
 
 // Training rows of this launch: target = (L_final - L after the vertex's emission) / beta.
-// One thread per (training path, bounce) record, so every load is independent and the
-// records are read in their memory order; one append per wavefront (ballot + one atomic),
-// not one atomic per row: every row of a launch appends to the same counter.
+// A thread takes kCollectPer (training path, bounce) records, kBlock apart, so every load is
+// coalesced over the block's records; the block appends its valid rows with one atomic on the
+// row counter (a wave scan of the per-thread counts, then the waves' sums in LDS), and each
+// thread writes its rows in record order from its offset. One atomic per wavefront instead
+// serialised ~21 k atomics on the counter per launch.
+constexpr int kCollectPer = 8;
 __global__ void __launch_bounds__(kBlock) k_cache_collect(PathBuffers B, PassParams P) {
+    __shared__ int32_t wsum[kBlock / 64];
+    __shared__ int32_t bbase;
     const int64_t nr = (int64_t)min(B.counters[5], P.train_paths) * P.max_depth;
     const int cap = P.train_cap * P.nsamp;
-    const int lane = __lane_id();
-    const uint64_t lower = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x + (int)threadIdx.x - lane; r0 < nr; r0 += stride) {  // wave-uniform
-        const int64_t r = r0 + lane;
-        bool ok = r < nr && B.trec_valid[r];
-        const float4* rec = B.trec + r * 6;
-        float4 tgt = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (ok) {
-            const float4 Lf = B.L[B.tlist[r / P.max_depth]];
-            const float4 be = rec[4], lp = rec[5];
-            const float t0v = be.x > 0.f ? (Lf.x - lp.x) / be.x : 0.f;
-            const float t1v = be.y > 0.f ? (Lf.y - lp.y) / be.y : 0.f;
-            const float t2v = be.z > 0.f ? (Lf.z - lp.z) / be.z : 0.f;
-            ok = isfinite(t0v) && isfinite(t1v) && isfinite(t2v);
-            tgt = make_float4(fmaxf(t0v, 0.f), fmaxf(t1v, 0.f), fmaxf(t2v, 0.f), 0.f);
+    const int lane = __lane_id(), wv = (int)threadIdx.x >> 6;
+    const int64_t span = (int64_t)kBlock * kCollectPer;
+    for (int64_t r0 = (int64_t)blockIdx.x * span; r0 < nr; r0 += (int64_t)gridDim.x * span) {  // block-uniform
+        float4 tgt[kCollectPer];
+        uint32_t okm = 0;
+#pragma unroll
+        for (int k = 0; k < kCollectPer; k++) {
+            const int64_t r = r0 + (int64_t)k * kBlock + threadIdx.x;
+            tgt[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r < nr && B.trec_valid[r]) {
+                const float4* rec = B.trec + r * (kNrcQ4 + 2);
+                const float4 Lf = B.L[B.tlist[r / P.max_depth]];
+                const float4 be = rec[kNrcQ4], lp = rec[kNrcQ4 + 1];
+                const float t0v = be.x > 0.f ? (Lf.x - lp.x) / be.x : 0.f;
+                const float t1v = be.y > 0.f ? (Lf.y - lp.y) / be.y : 0.f;
+                const float t2v = be.z > 0.f ? (Lf.z - lp.z) / be.z : 0.f;
+                if (isfinite(t0v) && isfinite(t1v) && isfinite(t2v)) {
+                    okm |= 1u << k;
+                    tgt[k] = make_float4(fmaxf(t0v, 0.f), fmaxf(t1v, 0.f), fmaxf(t2v, 0.f), 0.f);
+                }
+            }
         }
-        const uint64_t m = __ballot(ok);
-        if (!m) continue;
-        int base = 0;
-        if (lane == 0) base = atomicAdd(&B.counters[7], __popcll(m));
-        base = __shfl(base, 0);
-        const int slot = base + __popcll(m & lower);
-        if (!ok || slot >= cap) continue;
-        for (int k = 0; k < 4; k++) B.bfeat[(int64_t)slot * 4 + k] = rec[k];
-        B.btarget[slot] = tgt;
+        const int cnt = __popc(okm);
+        int incl = cnt;  // inclusive scan over the wavefront
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d);
+            if (lane >= d) incl += v;
+        }
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int w = 0; w < kBlock / 64; w++) {
+                const int t = wsum[w];
+                wsum[w] = tot;
+                tot += t;
+            }
+            bbase = tot ? atomicAdd(&B.counters[7], tot) : 0;
+        }
+        __syncthreads();
+        int slot = bbase + wsum[wv] + incl - cnt;
+#pragma unroll
+        for (int k = 0; k < kCollectPer; k++) {
+            if (!((okm >> k) & 1u)) continue;
+            if (slot < cap) {
+                const float4* rec = B.trec + (r0 + (int64_t)k * kBlock + threadIdx.x) * (kNrcQ4 + 2);
+                for (int j = 0; j < kNrcQ4; j++) B.bfeat[(int64_t)slot * kNrcQ4 + j] = rec[j];
+                B.btarget[slot] = tgt[k];
+            }
+            slot++;
+        }
+        __syncthreads();  // wsum and bbase are rewritten by the next iteration
     }
 }
 
@@ -2103,8 +2150,7 @@ struct c2_pt {
 };
 
 namespace c2 {
-void nrc_launch_infer(c2_nrc* c, const int32_t* list, const int32_t* countp, int32_t maxn, const float4* feats,
-                      float4* L, const float4* beta, hipStream_t s);
+void nrc_launch_infer(c2_nrc* c, const int32_t* countp, int32_t maxn, const float4* recs, float4* L, hipStream_t s);
 void nrc_launch_train(c2_nrc* c, const float4* feats, const float4* targets, const int32_t* countp, int32_t js,
                       int32_t ns, int32_t maxn, hipStream_t s, bool adam);
 }  // namespace c2
@@ -2393,6 +2439,11 @@ int c2_pt_reset(c2_pt* pt) {
 }  // extern "C"
 
 namespace {
+// Events that only time launches on the GPU: without the system-scope release and acquire a
+// default event record carries (the host never reads memory through them), so a record adds
+// no cache write-back to the launch it brackets.
+inline void timing_event(hipEvent_t* e) { HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableSystemFence)); }
+
 struct KTimer {
     c2_pt* pt;
     int cls;
@@ -2400,8 +2451,8 @@ struct KTimer {
     hipEvent_t a = nullptr, b = nullptr;
     KTimer(c2_pt* p, int c, hipStream_t st) : pt(p), cls(c), s(st) {
         if (!(pt->prof_mask & (1u << c))) return;
-        HIP_CHECK(hipEventCreate(&a));
-        HIP_CHECK(hipEventCreate(&b));
+        timing_event(&a);
+        timing_event(&b);
         HIP_CHECK(hipEventRecord(a, s));
     }
     ~KTimer() {
@@ -2558,11 +2609,10 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         }
                         if (P.cache_bounce == b) {
                             hipEvent_t e0, e1;
-                            HIP_CHECK(hipEventCreate(&e0));
-                            HIP_CHECK(hipEventCreate(&e1));
+                            timing_event(&e0);
+                            timing_event(&e1);
                             HIP_CHECK(hipEventRecord(e0, s));
-                            nrc_launch_infer(pt->nrc, LB.nrc_list, LB.counters + 4, (int32_t)npath, LB.nrc_feat,
-                                             LB.L, LB.beta, s);
+                            nrc_launch_infer(pt->nrc, LB.counters + 4, (int32_t)npath, LB.nrc_feat, LB.L, s);
                             HIP_CHECK(hipEventRecord(e1, s));
                             pt->cache_events.push_back({0, {e0, e1}});
                         }
@@ -2611,10 +2661,11 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                     // each record owns its two events (c2_pt_get_cache_times and the destructor
                     // destroy both of every record)
                     hipEvent_t e0, e1, e2, e3;
-                    for (hipEvent_t* e : {&e0, &e1, &e2, &e3}) HIP_CHECK(hipEventCreate(e));
+                    for (hipEvent_t* e : {&e0, &e1, &e2, &e3}) timing_event(e);
                     HIP_CHECK(hipEventRecord(e0, s));
                     const int64_t pairs = (int64_t)P.train_paths * P.max_depth;  // (training path, bounce)
-                    hipLaunchKernelGGL(k_cache_collect, dim3(grid_for(pairs)), dim3(kBlock), 0, s, LB, P);
+                    hipLaunchKernelGGL(k_cache_collect, dim3(grid_for((pairs + kCollectPer - 1) / kCollectPer)),
+                                       dim3(kBlock), 0, s, LB, P);
                     HIP_CHECK(hipEventRecord(e1, s));
                     HIP_CHECK(hipEventRecord(e2, s));
                     // one Adam step per sample of the launch, each on an equal slice of the rows
@@ -2866,12 +2917,11 @@ int c2_pt_set_cache(c2_pt* pt, c2_nrc* nrc, int32_t cache_bounce, int32_t train_
         int64_t tpaths = (int64_t)pt->chunk / train_ratio + pt->views[0].height + 64;
         PathBuffers& B = pt->B;
         DeviceArena& A = pt->cache_arena;
-        B.nrc_feat = A.alloc<float4>((size_t)pt->chunk * 4);
-        B.nrc_list = A.alloc<int32_t>(pt->chunk);
+        B.nrc_feat = A.alloc<float4>((size_t)pt->chunk * kNrcRec4);
         B.tlist = A.alloc<int32_t>(tpaths);
-        B.trec = A.alloc<float4>((size_t)tpaths * pt->pp.max_depth * 6);
+        B.trec = A.alloc<float4>((size_t)tpaths * pt->pp.max_depth * (kNrcQ4 + 2));
         B.trec_valid = A.alloc<uint8_t>((size_t)tpaths * pt->pp.max_depth);
-        B.bfeat = A.alloc<float4>((size_t)cap * kCacheSamplesPerChunk * 4);
+        B.bfeat = A.alloc<float4>((size_t)cap * kCacheSamplesPerChunk * kNrcQ4);
         B.btarget = A.alloc<float4>((size_t)cap * kCacheSamplesPerChunk);
         (void)st;
         pt->nrc = nrc;

@@ -26,7 +26,19 @@ constexpr int kNrcIn = 64;
 constexpr int kNrcW = 64;       // hidden width
 constexpr int kNrcLayers = 5;   // 4 hidden + output
 constexpr int kNrcOut = 3;
-constexpr int kNrcFeat = 16;    // raw floats per query
+constexpr int kNrcFeat = 16;    // raw floats per query (host API)
+// Packed query on the device, kNrcQ4 float4 per query, written by the renderer and by the
+// host conversion of raw queries:
+//   q0 = (position xyz, roughness)
+//   q1 = (azimuth of wo, cos theta of wo * 0.5 + 0.5, azimuth of n, cos theta of n * 0.5 + 0.5)
+//   q2 = (albedo rgb, 1)
+// (azimuth = atan2(z, x) / 2 pi + 0.5; cos theta = the y component). These are the raw
+// encoding inputs every lane group of the MLP kernels needs: q0 for the position bands and
+// the roughness one-blob, q1 and q2 for features 48..51 and 56..59, so a lane loads two.
+constexpr int kNrcQ4 = 3;
+// The renderer's query record adds a fourth float4: (throughput rgb, path index bits); the
+// inference kernel adds throughput x prediction to that path's radiance.
+constexpr int kNrcRec4 = kNrcQ4 + 1;
 constexpr int kNrcParams = kNrcLayers * kNrcW * kNrcW + kNrcLayers * kNrcW;  // padded storage
 
 // Forward FLOP per query: 4 x (64 x 64 x 2) + 64 x 3 x 2
