@@ -2625,28 +2625,32 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                                                P.lmax, pt->cap_n, pt->cap_rays, pt->cap_ignore,
                                                LB.counters + 6);
                         }
-                        // The rare MIS rays (closest hit) run on the side stream, concurrently
-                        // with the shadow rays; both only read what k_nee wrote, and k_resolve
-                        // waits for both.
+                        // The shadow rays run on the side stream and the rare MIS rays (closest
+                        // hit, a small grid) on the lane's stream, concurrently; both only read
+                        // what k_nee wrote, and k_resolve waits for both. The MIS launch directly
+                        // follows k_nee in its queue, so its blocks are resident before the
+                        // shadow-ray grid (behind the cross-stream wait) fills the device: on the
+                        // side stream instead, its blocks waited for slots that k_occlude's
+                        // blocks freed, and its launches spanned most of k_occlude's.
                         HIP_CHECK(hipEventRecord(evf, s));
                         HIP_CHECK(hipStreamWaitEvent(side, evf, 0));
-                        {
-                            KTimer kt(pt, C2_K_MIS, side);
-                            if (SGkd)
-                                hipLaunchKernelGGL(k_mis<true>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0,
-                                                   side, S, LB, P);
-                            else
-                                hipLaunchKernelGGL(k_mis<false>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0,
-                                                   side, S, LB, P);
-                        }
-                        HIP_CHECK(hipEventRecord(evj, side));
                         int gs = std::min(grid_for(npath * P.lmax), qcap);
                         {
-                            KTimer kt(pt, C2_K_OCCLUDE, s);
+                            KTimer kt(pt, C2_K_OCCLUDE, side);
                             if (SGkd)
-                                hipLaunchKernelGGL(k_occlude<true>, dim3(gs), dim3(kBlock), 0, s, S, LB);
+                                hipLaunchKernelGGL(k_occlude<true>, dim3(gs), dim3(kBlock), 0, side, S, LB);
                             else
-                                hipLaunchKernelGGL(k_occlude<false>, dim3(gs), dim3(kBlock), 0, s, S, LB);
+                                hipLaunchKernelGGL(k_occlude<false>, dim3(gs), dim3(kBlock), 0, side, S, LB);
+                        }
+                        HIP_CHECK(hipEventRecord(evj, side));
+                        {
+                            KTimer kt(pt, C2_K_MIS, s);
+                            if (SGkd)
+                                hipLaunchKernelGGL(k_mis<true>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, s, S,
+                                                   LB, P);
+                            else
+                                hipLaunchKernelGGL(k_mis<false>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, s,
+                                                   S, LB, P);
                         }
                         HIP_CHECK(hipStreamWaitEvent(s, evj, 0));
                         {
