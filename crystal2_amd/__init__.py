@@ -222,10 +222,12 @@ class GpuBVH:
     """Accelerator on the GPU (src/Accelerators/Accelerator.h:8-22): Intersect / IntersectTest."""
 
     ACCELS = {"bvh": _abi.C2_ACCEL_BVH, "BVH": _abi.C2_ACCEL_BVH, "GPUBVH": _abi.C2_ACCEL_BVH,
-              "kdtree": _abi.C2_ACCEL_KDTREE, "KDTree": _abi.C2_ACCEL_KDTREE}
+              "kdtree": _abi.C2_ACCEL_KDTREE, "KDTree": _abi.C2_ACCEL_KDTREE,
+              "bvh8": _abi.C2_ACCEL_BVH8, "GPUBVH8": _abi.C2_ACCEL_BVH8}
 
     def __init__(self, scene: Scene, accel: str = "bvh"):
-        """``accel``: "bvh" (the GPU BVH) or "kdtree" (the reference's KD-tree, "KDTree")."""
+        """``accel``: "bvh" (the GPU BVH, 4 wide), "bvh8" (the same tree 8 wide) or "kdtree"
+        (the reference's KD-tree, "KDTree")."""
         self._lib = load_library()
         self._scene = scene
         self._h = C.c_void_p()
@@ -289,7 +291,7 @@ class WavefrontPathIntegrator:
     ``film``: "f32" (reference FrameBuffer), "kahan" or "f64".
     ``integrator``: "path" (PathTracingIntegrator) or "direct" (DirectLightingIntegrator,
     DirectLightingIntegrator.cpp:22-156, on the same kernels).
-    ``accel``: "bvh" or "kdtree" (config "Acceleration": the reference's KD-tree).
+    ``accel``: "bvh", "bvh8" or "kdtree" (config "Acceleration": the reference's KD-tree).
     """
 
     FILMS = {"f32": _abi.C2_FILM_F32, "kahan": _abi.C2_FILM_F32_KAHAN, "f64": _abi.C2_FILM_F64}
@@ -415,7 +417,8 @@ class BatchRenderer:
     SCHEDULES = {"wavefront": _abi.C2_SCHED_WAVEFRONT, "persistent": _abi.C2_SCHED_PERSISTENT}
 
     def autotune(self) -> dict:
-        """Tunes both schedules and keeps the faster (``c2_batch_autotune``)."""
+        """Tunes both schedules (the wavefront's also over BVH widths 4 and 8) and keeps the
+        faster (``c2_batch_autotune``)."""
         t = _abi.c2_batch_tuning()
         check(self._lib.c2_batch_autotune(self._h, C.byref(t)))
         out = {k: getattr(t, k) for k, _ in t._fields_}
@@ -423,14 +426,16 @@ class BatchRenderer:
         return out
 
     def configure(self, schedule: str = "wavefront", block_threads: int = 256, waves_per_simd: int = 0,
-                  paths_in_flight: int = 0, queue_blocks: int = 0):
-        """Sets the schedule ("wavefront" or "persistent") and launch shape explicitly."""
+                  paths_in_flight: int = 0, queue_blocks: int = 0, bvh_width: int = 0):
+        """Sets the schedule ("wavefront" or "persistent"), launch shape and, for the
+        wavefront, the BVH width (4 or 8; 0 keeps it) explicitly."""
         t = _abi.c2_batch_tuning()
         t.schedule = self.SCHEDULES[schedule]
         t.block_threads = block_threads
         t.waves_per_simd = waves_per_simd
         t.paths_in_flight = paths_in_flight
         t.queue_blocks = queue_blocks
+        t.bvh_width = bvh_width
         check(self._lib.c2_batch_configure(self._h, C.byref(t)))
 
     def render(self, use_graph: bool = True):

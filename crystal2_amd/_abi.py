@@ -19,7 +19,7 @@ C2_LIGHT_AREA, C2_LIGHT_POINT = 0, 1
 C2_ARGORDER_RTL, C2_ARGORDER_LTR = 0, 1
 C2_FILM_F32, C2_FILM_F32_KAHAN, C2_FILM_F64 = 0, 1, 2
 C2_INTEGRATOR_PATH, C2_INTEGRATOR_DIRECT = 0, 1
-C2_ACCEL_BVH, C2_ACCEL_KDTREE = 0, 1
+C2_ACCEL_BVH, C2_ACCEL_KDTREE, C2_ACCEL_BVH8 = 0, 1, 2
 
 
 class c2_material(C.Structure):
@@ -107,7 +107,7 @@ class c2_batch_tuning(C.Structure):
     _fields_ = [("paths_in_flight", C.c_int32), ("queue_blocks", C.c_int32), ("pass_ms", C.c_double),
                 ("candidates", C.c_int32), ("schedule", C.c_int32), ("block_threads", C.c_int32),
                 ("waves_per_simd", C.c_int32), ("sample0_ms_persistent", C.c_double), ("render_ms_wavefront", C.c_double),
-                ("render_ms_persistent", C.c_double)]
+                ("render_ms_persistent", C.c_double), ("bvh_width", C.c_int32), ("pass_ms_width8", C.c_double)]
 
 
 class c2_batch_stats(C.Structure):

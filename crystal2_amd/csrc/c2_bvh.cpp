@@ -1,5 +1,5 @@
 // Host BVH builder: binned SAH binary BVH (leaf <= 4) collapsed into 4-wide
-// 128-byte nodes (layout documented in c2_bvh.h).
+// 128-byte nodes or 8-wide 256-byte nodes (layouts documented in c2_bvh.h).
 //
 // The reference builds a median-split binary BVH with leaf <= 2
 // (src/Accelerators/BVH.cpp:156-188, its SAH path at :281-325 is compiled out).
@@ -152,16 +152,17 @@ struct Builder {
     }
 };
 
+// Collapse of the binary tree into W-wide nodes (Node4 or Node8). Open the shallowest
+// inner child first (ties: largest surface area), so every W-wide level spans log2(W)
+// binary levels wherever the tree allows.
+template <int W, class Node>
 struct Collapser {
     const std::vector<Bvh2Node>& b2;
-    std::vector<Node4>& out;
+    std::vector<Node>& out;
     int maxDepth = 0;
 
     int collapse(int i, int depth) {
         maxDepth = std::max(maxDepth, depth);
-        // Open the shallowest inner child first (ties: largest surface area), so
-        // every 4-wide level spans two binary levels wherever the tree allows and
-        // depth4 <= ceil(depth2 / 2).
         std::vector<int> kids, kdepth;
         if (b2[i].leaf()) {
             kids.push_back(i);
@@ -172,7 +173,7 @@ struct Collapser {
             kdepth.push_back(1);
             kdepth.push_back(1);
         }
-        while (kids.size() < 4) {
+        while ((int)kids.size() < W) {
             int best = -1;
             float bestA = -1;
             for (size_t k = 0; k < kids.size(); k++) {
@@ -191,25 +192,26 @@ struct Collapser {
             kdepth.insert(kdepth.begin() + best + 1, d + 1);
         }
         int me = (int)out.size();
-        out.push_back(Node4{});
-        Node4 n{};
-        n.count = 0;
-        for (int s = 0; s < 4; s++) {
+        out.push_back(Node{});
+        Node n{};
+        for (int s = 0; s < W; s++) {
             n.lo_x[s] = n.lo_y[s] = n.lo_z[s] = INFINITY;
             n.hi_x[s] = n.hi_y[s] = n.hi_z[s] = -INFINITY;
             n.child[s] = -1;
         }
+        uint32_t count = 0;
         for (size_t s = 0; s < kids.size(); s++) {
             const Bvh2Node& c = b2[kids[s]];
             n.lo_x[s] = c.box.lo[0]; n.lo_y[s] = c.box.lo[1]; n.lo_z[s] = c.box.lo[2];
             n.hi_x[s] = c.box.hi[0]; n.hi_y[s] = c.box.hi[1]; n.hi_z[s] = c.box.hi[2];
             if (c.leaf()) {
                 n.child[s] = ~(int32_t)((uint32_t)c.left | ((uint32_t)(-c.right - 1) << kSlotBits));
-                n.count |= (uint32_t)(-c.right) << (8 * s);
+                if (s < 4) count |= (uint32_t)(-c.right) << (8 * s);
             } else {
                 n.child[s] = -1;  // patched below
             }
         }
+        if constexpr (W == 4) n.count = count;
         for (size_t s = 0; s < kids.size(); s++) {
             const Bvh2Node& c = b2[kids[s]];
             if (!c.leaf()) n.child[s] = collapse(kids[s], depth + 1);
@@ -237,12 +239,24 @@ void build_bvh(const std::vector<Box3>& boxes, BuiltBvh& out, int leaf_size) {
     out.bvh2.push_back(Bvh2Node{});
     b.build(0, 0, (int)n, 0);
     out.depth2 = b.maxDepth;
-    Collapser c{out.bvh2, out.nodes};
+    Collapser<4, Node4> c{out.bvh2, out.nodes};
     out.nodes.reserve(out.bvh2.size() / 2 + 2);
     c.collapse(0, 0);
     out.depth4 = c.maxDepth;
     if (out.depth4 > kMaxBvh4Depth)
         fail(C2_ERR_UNSUPPORTED, "BVH4 depth " + std::to_string(out.depth4) + " exceeds the traversal stack bound");
+}
+
+void collapse_bvh8(BuiltBvh& out) {
+    out.nodes8.clear();
+    out.depth8 = 0;
+    if (out.bvh2.empty()) return;
+    Collapser<8, Node8> c{out.bvh2, out.nodes8};
+    out.nodes8.reserve(out.bvh2.size() / 6 + 2);
+    c.collapse(0, 0);
+    out.depth8 = c.maxDepth;
+    if (out.depth8 > kMaxBvh8Depth)
+        fail(C2_ERR_UNSUPPORTED, "BVH8 depth " + std::to_string(out.depth8) + " exceeds the traversal stack bound");
 }
 
 }  // namespace c2

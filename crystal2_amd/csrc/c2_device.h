@@ -35,6 +35,7 @@ struct DevLight {
 
 struct DevScene {
     const Node4* nodes;
+    const Node8* nodes8;       // the 8-wide tree (C2_ACCEL_BVH8), null otherwise
     const float4* prims;       // 3 float4 per slot
     const int4* entities;      // (kind, shape, material, light)
     const float* tri_pos;      // [ntri][9]
@@ -244,25 +245,25 @@ struct TravLds {
 
 // Overflow part of a lane's stack, in private memory: the entries pushed beyond the K
 // that the LDS ring holds.
-template <bool ANY>
+template <bool ANY, int SS = kStackSize>
 struct TravOverflow {
     static constexpr int K = TravK<ANY>::value;
-    int32_t i[kStackSize - K];
-    float t[ANY ? 1 : kStackSize - K];
+    int32_t i[SS - K];
+    float t[ANY ? 1 : SS - K];
 };
 // The stack proper holds only scalars and pointers, so that the stack pointer and lane
 // stay in registers (an array member would put the whole object in private memory).
-template <bool ANY, int BS = kTravBlock>
+template <bool ANY, int BS = kTravBlock, int SS = kStackSize>
 struct TravStack {
     static constexpr int K = TravK<ANY>::value;
-    static_assert((K & (K - 1)) == 0 && K > 0 && K <= kStackSize, "LDS stack depth must be a power of two");
+    static_assert((K & (K - 1)) == 0 && K > 0 && K <= SS, "LDS stack depth must be a power of two");
     int32_t* li;
     float* lt;
     int32_t* oi;
     float* ot;
     int lane;
     int sp;
-    __device__ __forceinline__ TravStack(TravLds<ANY, BS>& L, TravOverflow<ANY>& O)
+    __device__ __forceinline__ TravStack(TravLds<ANY, BS>& L, TravOverflow<ANY, SS>& O)
         : li(L.idx), lt(L.t), oi(O.i), ot(O.t), lane((int)threadIdx.x), sp(0) {}
     __device__ __forceinline__ void push(int32_t e, float t) {
         int off = (sp & (K - 1)) * BS + lane;
@@ -285,6 +286,95 @@ struct TravStack {
         }
     }
 };
+
+// The primitives of one leaf child word `cur` (< 0), with the per-entity box pre-clip
+// [tmin, min(tmax, best)] (closest) / [tmin, tmax] (any). Any hit: returns true when one
+// occludes (hit.entity set). Closest hit: updates (best, bestE, bestU, bestV), ties to the
+// lowest entity index.
+template <bool ANY>
+__device__ __forceinline__ bool leaf_prims(const DevScene& S, const RayPre& r, V3 o, V3 d, float tmin, float tmax,
+                                           int ignore, int32_t cur, float& best, int32_t& bestE, float& bestU,
+                                           float& bestV, Hit& hit) {
+    uint32_t enc = (uint32_t)(~cur);
+    int slot = (int)(enc & ((1u << kSlotBits) - 1u));
+    int n = (int)(enc >> kSlotBits) + 1;
+    for (int i = 0; i < n; i++) {
+        float4 a, b, c;
+        load_prim(S, slot + i, a, b, c);
+        int32_t ent = __float_as_int(c.y);
+        int32_t kind = __float_as_int(c.z);
+        float t1 = tmin, t2 = ANY ? tmax : fmin_ref(tmax, best);
+        if (kind == C2_SHAPE_TRIANGLE) {
+            V3 v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
+            // entity box (component-wise min / max of finite vertices: the IEEE
+            // forms equal std::min / std::max up to the sign of a zero, which
+            // no comparison of the slab test sees)
+            float lxx = fminf(fminf(v0.x, v1.x), v2.x), lyy = fminf(fminf(v0.y, v1.y), v2.y);
+            float lzz = fminf(fminf(v0.z, v1.z), v2.z), hxx = fmaxf(fmaxf(v0.x, v1.x), v2.x);
+            float hyy = fmaxf(fmaxf(v0.y, v1.y), v2.y), hzz = fmaxf(fmaxf(v0.z, v1.z), v2.z);
+            if (!slab(r, lxx, lyy, lzz, hxx, hyy, hzz, t1, t2)) continue;
+            if (ANY) {
+                if (ent == ignore) continue;
+                if (tri_test(o, d, v0, v1, v2, t1, t2)) {
+                    hit.entity = ent;
+                    return true;
+                }
+            } else {
+                V3 res = tri_solve(o, d, v0, v1, v2);
+                if (!tri_accept_closest(res)) continue;
+                float dis = res.z;
+                if (dis < best || (dis == best && ent < bestE)) {
+                    best = dis;
+                    bestE = ent;
+                    bestU = res.x;
+                    bestV = res.y;
+                }
+            }
+        } else if (kind == C2_SHAPE_DISK) {
+            // box: centre -+ (r, r * sinTheta, r), Disk::GetBoundingBox (b.y holds r * sinTheta)
+            const float* sp3 = S.spheres + (size_t)__float_as_int(b.x) * 22;
+            float rr = a.w, oy = b.y;
+            if (!slab(r, a.x - rr, a.y - oy, a.z - rr, a.x + rr, a.y + oy, a.z + rr, t1, t2)) continue;
+            if (ANY) {
+                if (ent == ignore) continue;
+                if (disk_test(sp3, o, d, t1, t2)) {
+                    hit.entity = ent;
+                    return true;
+                }
+            } else {
+                float dis;
+                if (!disk_closest(sp3, o, d, dis)) continue;
+                if (dis < best || (dis == best && ent < bestE)) {
+                    best = dis;
+                    bestE = ent;
+                    bestU = 0.f;
+                    bestV = 0.f;
+                }
+            }
+        } else {
+            const float* sp3 = S.spheres + (size_t)__float_as_int(b.x) * 22;
+            float rr = a.w;
+            if (!slab(r, a.x - rr, a.y - rr, a.z - rr, a.x + rr, a.y + rr, a.z + rr, t1, t2)) continue;
+            if (ANY) {
+                if (ent == ignore) continue;
+                if (sphere_test(sp3, o, d, t1, t2)) {
+                    hit.entity = ent;
+                    return true;
+                }
+            } else {
+                float dis;
+                if (!sphere_closest(sp3, o, d, dis)) continue;
+                if (dis < best || (dis == best && ent < bestE)) {
+                    best = dis;
+                    bestE = ent;
+                    bestU = 0.f;
+                    bestV = 0.f;
+                }
+            }
+        }
+    }
+    return false;
+}
 
 // root: Node4 index where the traversal starts (a batch keeps every scene's tree in
 // one node array); a negative root is a scene without entities (always a miss).
@@ -366,87 +456,110 @@ __device__ __forceinline__ bool traverse(const DevScene& S, int32_t root, V3 o, 
                 continue;
             }
         } else {
-            uint32_t enc = (uint32_t)(~cur);
-            int slot = (int)(enc & ((1u << kSlotBits) - 1u));
-            int n = (int)(enc >> kSlotBits) + 1;
-            for (int i = 0; i < n; i++) {
-                float4 a, b, c;
-                load_prim(S, slot + i, a, b, c);
-                int32_t ent = __float_as_int(c.y);
-                int32_t kind = __float_as_int(c.z);
-                float t1 = tmin, t2 = ANY ? tmax : fmin_ref(tmax, best);
-                if (kind == C2_SHAPE_TRIANGLE) {
-                    V3 v0{a.x, a.y, a.z}, v1{a.w, b.x, b.y}, v2{b.z, b.w, c.x};
-                    // entity box (component-wise min / max of finite vertices: the IEEE
-                    // forms equal std::min / std::max up to the sign of a zero, which
-                    // no comparison of the slab test sees)
-                    float lxx = fminf(fminf(v0.x, v1.x), v2.x), lyy = fminf(fminf(v0.y, v1.y), v2.y);
-                    float lzz = fminf(fminf(v0.z, v1.z), v2.z), hxx = fmaxf(fmaxf(v0.x, v1.x), v2.x);
-                    float hyy = fmaxf(fmaxf(v0.y, v1.y), v2.y), hzz = fmaxf(fmaxf(v0.z, v1.z), v2.z);
-                    if (!slab(r, lxx, lyy, lzz, hxx, hyy, hzz, t1, t2)) continue;
-                    if (ANY) {
-                        if (ent == ignore) continue;
-                        if (tri_test(o, d, v0, v1, v2, t1, t2)) {
-                            hit.entity = ent;
-                            return true;
-                        }
-                    } else {
-                        V3 res = tri_solve(o, d, v0, v1, v2);
-                        if (!tri_accept_closest(res)) continue;
-                        float dis = res.z;
-                        if (dis < best || (dis == best && ent < bestE)) {
-                            best = dis;
-                            bestE = ent;
-                            bestU = res.x;
-                            bestV = res.y;
-                        }
-                    }
-                } else if (kind == C2_SHAPE_DISK) {
-                    // box: centre -+ (r, r * sinTheta, r), Disk::GetBoundingBox (b.y holds r * sinTheta)
-                    const float* sp3 = S.spheres + (size_t)__float_as_int(b.x) * 22;
-                    float rr = a.w, oy = b.y;
-                    if (!slab(r, a.x - rr, a.y - oy, a.z - rr, a.x + rr, a.y + oy, a.z + rr, t1, t2)) continue;
-                    if (ANY) {
-                        if (ent == ignore) continue;
-                        if (disk_test(sp3, o, d, t1, t2)) {
-                            hit.entity = ent;
-                            return true;
-                        }
-                    } else {
-                        float dis;
-                        if (!disk_closest(sp3, o, d, dis)) continue;
-                        if (dis < best || (dis == best && ent < bestE)) {
-                            best = dis;
-                            bestE = ent;
-                            bestU = 0.f;
-                            bestV = 0.f;
-                        }
-                    }
-                } else {
-                    const float* sp3 = S.spheres + (size_t)__float_as_int(b.x) * 22;
-                    float rr = a.w;
-                    if (!slab(r, a.x - rr, a.y - rr, a.z - rr, a.x + rr, a.y + rr, a.z + rr, t1, t2)) continue;
-                    if (ANY) {
-                        if (ent == ignore) continue;
-                        if (sphere_test(sp3, o, d, t1, t2)) {
-                            hit.entity = ent;
-                            return true;
-                        }
-                    } else {
-                        float dis;
-                        if (!sphere_closest(sp3, o, d, dis)) continue;
-                        if (dis < best || (dis == best && ent < bestE)) {
-                            best = dis;
-                            bestE = ent;
-                            bestU = 0.f;
-                            bestV = 0.f;
-                        }
-                    }
-                }
-            }
+            if (leaf_prims<ANY>(S, r, o, d, tmin, tmax, ignore, cur, best, bestE, bestU, bestV, hit)) return true;
         }
         // pop; a closest-hit entry whose entry distance is already beyond the best hit
         // cannot contain a closer one (its slab interval starts after `best`).
+        while (true) {
+            if (st.sp == 0) {
+                if (ANY) return false;
+                hit.entity = bestE;
+                hit.t = best;
+                hit.u = bestU;
+                hit.v = bestV;
+                return bestE >= 0;
+            }
+            int32_t e;
+            float et;
+            st.pop(e, et);
+            if (!ANY && et > best) continue;
+            cur = e;
+            break;
+        }
+    }
+}
+
+// The same traversal over the 8-wide tree (Node8, C2_ACCEL_BVH8): the eight children's slab
+// tests from 14 loads (two 128-byte lines), the hit children sorted by entry distance with
+// Batcher's 19-comparator network (as selects), the nearest taken next and the others pushed
+// farthest first. Leaves and ties as in traverse: the result does not depend on the order.
+template <bool ANY, int BS = kTravBlock>
+__device__ __forceinline__ bool traverse8(const DevScene& S, int32_t root, V3 o, V3 d, float tmin, float tmax,
+                                          int ignore, Hit& hit, TravLds<ANY, BS>& lds) {
+    hit.entity = -1;
+    if (root < 0) return false;
+    RayPre r = make_ray(o, d);
+    // byte offset of each axis's near plane within its (lo, hi) pair of 32-byte planes
+    const uint32_t oX = r.neg[0] ? 32u : 0u, oY = r.neg[1] ? 32u : 0u, oZ = r.neg[2] ? 32u : 0u;
+    TravOverflow<ANY, kStackSize8> ovf;
+    TravStack<ANY, BS, kStackSize8> st(lds, ovf);
+    int32_t cur = root;
+    float best = INFINITY;
+    int32_t bestE = -1;
+    float bestU = 0.f, bestV = 0.f;
+    while (true) {
+        if (cur >= 0) {
+            const char* nb = reinterpret_cast<const char*>(S.nodes8);
+            const uint32_t no = (uint32_t)cur * 256u;  // the node array stays below 4 GiB
+#define C2_LD4(OFF) (*reinterpret_cast<const float4*>(nb + (no + (OFF))))
+            const float4 nx0 = C2_LD4(oX), nx1 = C2_LD4(oX + 16u);
+            const float4 fx0 = C2_LD4(32u - oX), fx1 = C2_LD4(48u - oX);
+            const float4 ny0 = C2_LD4(64u + oY), ny1 = C2_LD4(80u + oY);
+            const float4 fy0 = C2_LD4(96u - oY), fy1 = C2_LD4(112u - oY);
+            const float4 nz0 = C2_LD4(128u + oZ), nz1 = C2_LD4(144u + oZ);
+            const float4 fz0 = C2_LD4(160u - oZ), fz1 = C2_LD4(176u - oZ);
+#undef C2_LD4
+            const int4 ch0 = *reinterpret_cast<const int4*>(nb + (no + 192u));
+            const int4 ch1 = *reinterpret_cast<const int4*>(nb + (no + 208u));
+            const float lim = ANY ? tmax : fmin_ref(tmax, best);
+            float k[8];
+            int32_t e[8];
+#define C2_CHILD8(J, NX, NY, NZ, FX, FY, FZ, C, CH)                                                     \
+    {                                                                                                \
+        const float tn = fmaxf(fmaxf(fmaxf(tmin, (NX.C - r.o.x) * r.invD.x), (NY.C - r.o.y) * r.invD.y), \
+                               (NZ.C - r.o.z) * r.invD.z);                                           \
+        const float tf = fminf(fminf(fminf(lim, (FX.C - r.o.x) * r.invD.x), (FY.C - r.o.y) * r.invD.y),  \
+                               (FZ.C - r.o.z) * r.invD.z);                                           \
+        const bool in = !(tf < tn);                                                                  \
+        k[J] = in ? tn : INFINITY;                                                                   \
+        e[J] = CH;                                                                                   \
+    }
+            C2_CHILD8(0, nx0, ny0, nz0, fx0, fy0, fz0, x, ch0.x)
+            C2_CHILD8(1, nx0, ny0, nz0, fx0, fy0, fz0, y, ch0.y)
+            C2_CHILD8(2, nx0, ny0, nz0, fx0, fy0, fz0, z, ch0.z)
+            C2_CHILD8(3, nx0, ny0, nz0, fx0, fy0, fz0, w, ch0.w)
+            C2_CHILD8(4, nx1, ny1, nz1, fx1, fy1, fz1, x, ch1.x)
+            C2_CHILD8(5, nx1, ny1, nz1, fx1, fy1, fz1, y, ch1.y)
+            C2_CHILD8(6, nx1, ny1, nz1, fx1, fy1, fz1, z, ch1.z)
+            C2_CHILD8(7, nx1, ny1, nz1, fx1, fy1, fz1, w, ch1.w)
+#undef C2_CHILD8
+#define C2_CSWAP8(A, B)                                              \
+    {                                                                \
+        const bool sw = k[B] < k[A];                                 \
+        const float lo = sw ? k[B] : k[A], hi = sw ? k[A] : k[B];    \
+        const int32_t elo = sw ? e[B] : e[A], ehi = sw ? e[A] : e[B]; \
+        k[A] = lo;                                                   \
+        k[B] = hi;                                                   \
+        e[A] = elo;                                                  \
+        e[B] = ehi;                                                  \
+    }
+            C2_CSWAP8(0, 1) C2_CSWAP8(2, 3) C2_CSWAP8(4, 5) C2_CSWAP8(6, 7)
+            C2_CSWAP8(0, 2) C2_CSWAP8(1, 3) C2_CSWAP8(4, 6) C2_CSWAP8(5, 7)
+            C2_CSWAP8(1, 2) C2_CSWAP8(5, 6)
+            C2_CSWAP8(0, 4) C2_CSWAP8(1, 5) C2_CSWAP8(2, 6) C2_CSWAP8(3, 7)
+            C2_CSWAP8(2, 4) C2_CSWAP8(3, 5)
+            C2_CSWAP8(1, 2) C2_CSWAP8(3, 4) C2_CSWAP8(5, 6)
+#undef C2_CSWAP8
+#pragma unroll
+            for (int j = 7; j >= 1; j--)
+                if (k[j] != INFINITY) st.push(e[j], k[j]);
+            if (k[0] != INFINITY) {
+                cur = e[0];
+                continue;
+            }
+        } else {
+            if (leaf_prims<ANY>(S, r, o, d, tmin, tmax, ignore, cur, best, bestE, bestU, bestV, hit)) return true;
+        }
         while (true) {
             if (st.sp == 0) {
                 if (ANY) return false;
@@ -536,9 +649,12 @@ struct KdStack {
         }
     }
 };
+// Traversal kinds of the kernels (their template argument ACC): the 4-wide BVH, the KD-tree,
+// the 8-wide BVH.
+constexpr int kAccBvh4 = 0, kAccKd = 1, kAccBvh8 = 2;
 // LDS of a traversal kernel: the KD-tree's stack or the BVH's
-template <bool KD, bool ANY>
-using TravLdsFor = std::conditional_t<KD, KdLds, TravLds<ANY>>;
+template <int ACC, bool ANY>
+using TravLdsFor = std::conditional_t<ACC == kAccKd, KdLds, TravLds<ANY>>;
 
 // KD-tree traversal with the reference's windows (KDTree.cpp:73-81, 195-266), recursion
 // turned into an explicit stack in the same visiting order:
@@ -640,6 +756,18 @@ __device__ __forceinline__ bool kd_traverse(const DevScene& S, int32_t root, con
     hit.u = bestU;
     hit.v = bestV;
     return bestE >= 0;
+}
+
+// One traversal of kind ACC from `root` (KD-tree: its root and union box kdlo / kdhi).
+template <int ACC, bool ANY, class Lds>
+__device__ __forceinline__ bool accel_trace(const DevScene& S, int32_t root, const float* kdlo, const float* kdhi, V3 o,
+                                            V3 d, float tmin, float tmax, int ignore, Hit& h, Lds& lds) {
+    if constexpr (ACC == kAccKd)
+        return kd_traverse<ANY>(S, root, kdlo, kdhi, o, d, tmin, tmax, ignore, h, lds);
+    else if constexpr (ACC == kAccBvh8)
+        return traverse8<ANY>(S, root, o, d, tmin, tmax, ignore, h, lds);
+    else
+        return traverse<ANY>(S, root, o, d, tmin, tmax, ignore, h, lds);
 }
 
 }  // namespace c2

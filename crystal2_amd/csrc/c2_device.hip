@@ -118,39 +118,35 @@ struct KdMaster {
     int32_t root;
 };
 
-template <bool KD>
+template <int ACC>
 __global__ void __launch_bounds__(kBlock) k_intersect(DevScene S, const float* __restrict__ rays, int64_t n,
                                                       int32_t* __restrict__ prim, float* __restrict__ tout, KdMaster M) {
-    __shared__ TravLdsFor<KD, false> lds;
+    __shared__ TravLdsFor<ACC, false> lds;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
         float4 a = r4[0], b = r4[1];
         Hit h;
         bool hit;
-        if constexpr (KD)
-            hit = kd_traverse<false>(S, M.root, M.lo, M.hi, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h, lds);
-        else
-            hit = traverse<false>(S, 0, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, 0.f, b.w, -1, h, lds);
+        hit = accel_trace<ACC, false>(S, ACC == kAccKd ? M.root : 0, M.lo, M.hi, V3{a.x, a.y, a.z},
+                                      V3{a.w, b.x, b.y}, 0.f, b.w, -1, h, lds);
         prim[i] = hit ? h.entity : -1;
         tout[i] = hit ? h.t : INFINITY;
     }
 }
 
-template <bool KD>
+template <int ACC>
 __global__ void __launch_bounds__(kBlock) k_intersect_test(DevScene S, const float* __restrict__ rays,
                                                            const int32_t* __restrict__ ignore, int64_t n,
                                                            uint8_t* __restrict__ out, KdMaster M) {
-    __shared__ TravLdsFor<KD, true> lds;
+    __shared__ TravLdsFor<ACC, true> lds;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float4* r4 = reinterpret_cast<const float4*>(rays + i * 8);
         float4 a = r4[0], b = r4[1];
         Hit h;
         const int ig = ignore ? ignore[i] : -1;
         bool hit;
-        if constexpr (KD)
-            hit = kd_traverse<true>(S, M.root, M.lo, M.hi, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ig, h, lds);
-        else
-            hit = traverse<true>(S, 0, V3{a.x, a.y, a.z}, V3{a.w, b.x, b.y}, b.z, b.w, ig, h, lds);
+        hit = accel_trace<ACC, true>(S, ACC == kAccKd ? M.root : 0, M.lo, M.hi, V3{a.x, a.y, a.z},
+                                     V3{a.w, b.x, b.y}, b.z, b.w, ig, h, lds);
         out[i] = hit ? 1 : 0;
     }
 }
@@ -245,6 +241,7 @@ struct View {
     int32_t width, height;
     int32_t spp;             // samples per pixel of the view
     int32_t root;            // Node4 index of the scene's BVH root, -1 = no entities
+    int32_t root8;           // Node8 index of its 8-wide root (renderers with the 8-wide tree)
     int32_t elig_begin;      // eligible lights (Flux != 0): elig[elig_begin, + num_elig)
     int32_t num_elig;
     int32_t nd_elig;         // eligible non-delta lights (one Get1D each per vertex)
@@ -273,6 +270,7 @@ struct PassParams {
     int32_t train_ratio;
     int32_t train_cap;               // training rows per Adam step (the launch collects nsamp x this)
     int32_t train_paths;             // training path record capacity
+    int32_t wide8;                   // 1: traversals use the 8-wide tree (View::root8)
 };
 
 // Radiance-cache query in the MLP kernels' packed layout (kNrcQ4 float4: position and
@@ -514,7 +512,7 @@ __device__ __forceinline__ bool light_term(const DevScene& S, const DevLight& Lt
 // The shared MIS ray, Scene::Intersect(isec.SpawnRay(wi)) from the vertex (sp, sn): only
 // the light owning the hit entity can receive f * Le * w / pdf_bsdf (:255-273). Returns
 // that light (-1 none) and the term in Lb.
-template <bool KD, class Lds>
+template <int ACC, class Lds>
 __device__ __forceinline__ int mis_term(const DevScene& S, const View& V, V3 sp, V3 sn, V3 wi, V3 f, float pdfB,
                                         int own, int integrator, Lds& lds, V3& Lb) {
     V3 o = spawn_origin(sp, sn, wi);
@@ -522,10 +520,8 @@ __device__ __forceinline__ int mis_term(const DevScene& S, const View& V, V3 sp,
     int lh = -1;
     Lb = V3{0, 0, 0};
     bool found;
-    if constexpr (KD)
-        found = kd_traverse<false>(S, V.root, V.kd_lo, V.kd_hi, o, wi, 0.f, INFINITY, -1, h, lds);
-    else
-        found = traverse<false>(S, V.root, o, wi, 0.f, INFINITY, -1, h, lds);
+    found = accel_trace<ACC, false>(S, ACC == kAccBvh8 ? V.root8 : V.root, V.kd_lo, V.kd_hi, o, wi, 0.f, INFINITY, -1,
+                                    h, lds);
     if (found) {
         int4 e = S.entities[h.entity];
         int li = e.w;
@@ -796,9 +792,9 @@ __global__ void __launch_bounds__(kBlock) k_class_scatter(PathBuffers B, int q) 
     }
 }
 
-template <bool KD>
+template <int ACC>
 __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, PassParams P, int q) {
-    __shared__ TravLdsFor<KD, false> lds;
+    __shared__ TravLdsFor<ACC, false> lds;
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[0], (unsigned long long)n);
@@ -808,11 +804,8 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, Pa
         const View& V = P.views[B.vid[p]];
         Hit h;
         bool hit;
-        if constexpr (KD)
-            hit = kd_traverse<false>(S, V.root, V.kd_lo, V.kd_hi, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY,
-                                     -1, h, lds);
-        else
-            hit = traverse<false>(S, V.root, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, INFINITY, -1, h, lds);
+        hit = accel_trace<ACC, false>(S, ACC == kAccBvh8 ? V.root8 : V.root, V.kd_lo, V.kd_hi, V3{o.x, o.y, o.z},
+                                      V3{d.x, d.y, d.z}, 0.f, INFINITY, -1, h, lds);
         B.hit[p] = make_float4(h.t, h.u, h.v, __int_as_float(hit ? h.entity : -1));
     }
 }
@@ -1070,7 +1063,7 @@ k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
             const View& V = P.views[vh.y];
             eb = V.elig_begin;
             ne = V.num_elig;
-            root = V.root;
+            root = P.wide8 ? V.root8 : V.root;
             float4 vp = B.vtx_p[i], vn = B.vtx_n[i], vd = B.vtx_du[i], vdir = B.vtx_d[i];
             it = nee_inter(V3{vp.x, vp.y, vp.z}, V3{vn.x, vn.y, vn.z}, V3{vd.x, vd.y, vd.z},
                            V3{vdir.x, vdir.y, vdir.z}, vi.x, vi.y, vi.z, (vi.w & V_FRONT) != 0);
@@ -1135,9 +1128,9 @@ k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
     }
 }
 
-template <bool KD>
+template <int ACC>
 __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
-    __shared__ TravLdsFor<KD, true> lds;
+    __shared__ TravLdsFor<ACC, true> lds;
     int n = B.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[1], (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1145,21 +1138,17 @@ __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
         float4 o = B.sh_o[s], d = B.sh_d[s];
         Hit h;
         bool occluded;
-        if constexpr (KD)
-            occluded = kd_traverse<true>(S, __float_as_int(d.w), nullptr, nullptr, V3{o.x, o.y, o.z},
-                                         V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f, __float_as_int(o.w), h, lds);
-        else
-            occluded = traverse<true>(S, __float_as_int(d.w), V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f,
-                                      __float_as_int(o.w), h, lds);
+        occluded = accel_trace<ACC, true>(S, __float_as_int(d.w), nullptr, nullptr, V3{o.x, o.y, o.z},
+                                          V3{d.x, d.y, d.z}, 0.f, 1 - 1e-5f, __float_as_int(o.w), h, lds);
         B.sh_vis[s] = occluded ? 0 : 1;
     }
 }
 
 // The shared MIS ray: Scene::Intersect(isec.SpawnRay(wi)); only the light owning the
 // hit entity can receive f * Le * w / pdf_bsdf, :255-273
-template <bool KD>
+template <int ACC>
 __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassParams P) {
-    __shared__ TravLdsFor<KD, false> lds;
+    __shared__ TravLdsFor<ACC, false> lds;
     int n = B.counters[3];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[2], (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1170,7 +1159,7 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassP
         int own = B.vtx_i[v].z;
         const View& V = P.views[B.vtx_h[v].y];
         V3 Lb;
-        const int lh = mis_term<KD>(S, V, sp, sn, wi, f, pdfB, own, P.integrator, lds, Lb);
+        const int lh = mis_term<ACC>(S, V, sp, sn, wi, f, pdfB, own, P.integrator, lds, Lb);
         B.mis_L[v] = make_float4(Lb.x, Lb.y, Lb.z, 0.f);
         B.mis_hit[v] = lh;
     }
@@ -1396,7 +1385,7 @@ k_persist(DevScene S, PersistParams Q) {
                     int lh = -1;
                     V3 Lb{0, 0, 0};
                     if (misHit) {
-                        lh = mis_term<false>(S, V, itn.p, itn.n, sv.wiB, sv.misF, sv.pdfB, itn.light, Q.integrator,
+                        lh = mis_term<kAccBvh4>(S, V, itn.p, itn.n, sv.wiB, sv.misF, sv.pdfB, itn.light, Q.integrator,
                                              lds.closest, Lb);
                         nmis++;
                     }
@@ -1563,6 +1552,9 @@ struct SceneGPU {
     std::vector<std::array<int32_t, 6>> sky;  // per scene: skybox face ids in the merged table
     // KD-tree accelerator (built instead of being traversed by the BVH when kd is set)
     bool kd = false;
+    // 8-wide tree (nodes8) built beside the 4-wide one
+    bool wide8 = false;
+    std::vector<Node8> nodes8;  // scene s is rooted at offs[s].node8
     std::vector<int4> kd_nodes{make_int4(-1, 0, 0, 0)};  // node 0: "none"
     std::vector<float4> kd_rec;  // per leaf entry: 5 float4 (see DevScene::kd_rec)
     int64_t kd_refs = 0;
@@ -1571,7 +1563,7 @@ struct SceneGPU {
 
     // Start of each scene in the concatenated arrays (a single scene has offsets 0).
     struct Offsets {
-        int32_t node, slot, ent, tri, sph, mat, light, tex;
+        int32_t node, slot, ent, tri, sph, mat, light, tex, node8;
     };
     std::vector<Offsets> offs;
     std::vector<Node4> nodes;  // every scene's tree; scene s is rooted at offs[s].node
@@ -1589,15 +1581,17 @@ struct SceneGPU {
         for (int k = 0; k < n; k++) {
             Offsets o{(int32_t)nodes.size(), (int32_t)(prims.size() / 3), (int32_t)ents.size(),
                       (int32_t)(tri_pos.size() / 9), (int32_t)(spheres.size() / 22), (int32_t)mats.size(),
-                      (int32_t)lights.size(), (int32_t)tex.size()};
+                      (int32_t)lights.size(), (int32_t)tex.size(), (int32_t)nodes8.size()};
             offs.push_back(o);
             append_scene(descs[k], o, k == 0);
             if ((int64_t)ents.size() >= (1ll << kSlotBits) || (int64_t)prims.size() / 3 >= (1ll << kSlotBits))
                 fail(C2_ERR_UNSUPPORTED, "too many entities in the scene set");
-            if ((int64_t)nodes.size() * (int64_t)sizeof(Node4) >= (1ll << 32))
+            if ((int64_t)nodes.size() * (int64_t)sizeof(Node4) >= (1ll << 32) ||
+                (int64_t)nodes8.size() * (int64_t)sizeof(Node8) >= (1ll << 32))
                 fail(C2_ERR_UNSUPPORTED, "too many BVH nodes (the node array must stay below 4 GiB)");
         }
         dev.nodes = arena.upload(nodes);
+        if (wide8) dev.nodes8 = arena.upload(nodes8.empty() ? std::vector<Node8>(1) : nodes8);
         dev.prims = arena.upload(prims);
         dev.entities = arena.upload(ents);
         dev.tri_pos = arena.upload(tri_pos);
@@ -1737,6 +1731,17 @@ struct SceneGPU {
                 else if (c8 != 0) nd.child[c] = ~(~nd.child[c] + o.slot);
             }
             nodes.push_back(nd);
+        }
+        if (wide8) {  // the 8-wide tree, shifted the same way (empty slots: inverted boxes)
+            collapse_bvh8(bv);
+            for (const Node8& nd0 : bv.nodes8) {
+                Node8 nd = nd0;
+                for (int c = 0; c < 8; c++) {
+                    if (nd.child[c] >= 0) nd.child[c] += o.node8;
+                    else if (nd.lo_x[c] <= nd.hi_x[c]) nd.child[c] = ~(~nd.child[c] + o.slot);
+                }
+                nodes8.push_back(nd);
+            }
         }
         for (size_t slot = 0; slot < bv.order.size(); slot++) {
             int e = bv.order[slot];
@@ -1904,9 +1909,11 @@ int c2_accel_build(const c2_scene_desc* desc, c2_accel** out) { return c2_accel_
 int c2_accel_create(const c2_scene_desc* desc, int32_t accel, c2_accel** out) {
     return guard([&] {
         if (!out) fail(C2_ERR_INVALID, "null output");
-        if (accel != C2_ACCEL_BVH && accel != C2_ACCEL_KDTREE) fail(C2_ERR_INVALID, "unknown accelerator kind");
+        if (accel != C2_ACCEL_BVH && accel != C2_ACCEL_KDTREE && accel != C2_ACCEL_BVH8)
+            fail(C2_ERR_INVALID, "unknown accelerator kind");
         std::unique_ptr<c2_accel> a(new c2_accel());
         a->scene.kd = accel == C2_ACCEL_KDTREE;
+        a->scene.wide8 = accel == C2_ACCEL_BVH8;
         a->scene.build(desc);
         a->stats.num_nodes_bvh2 = (int64_t)a->scene.bvh.bvh2.size();
         a->stats.num_nodes_bvh4 = (int64_t)a->scene.nodes.size();
@@ -1937,10 +1944,13 @@ int c2_accel_intersect_device(c2_accel* a, const float* rays, int64_t n, int32_t
             return;
         }
         if (a->scene.kd)
-            hipLaunchKernelGGL(k_intersect<true>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+            hipLaunchKernelGGL(k_intersect<kAccKd>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
                                a->scene.dev, rays, n, prim, t, a->scene.kdm[0]);
+        else if (a->scene.wide8)
+            hipLaunchKernelGGL(k_intersect<kAccBvh8>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                               a->scene.dev, rays, n, prim, t, KdMaster{});
         else
-            hipLaunchKernelGGL(k_intersect<false>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+            hipLaunchKernelGGL(k_intersect<kAccBvh4>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
                                a->scene.dev, rays, n, prim, t, KdMaster{});
         HIP_CHECK(hipGetLastError());
     });
@@ -1957,10 +1967,13 @@ int c2_accel_intersect_test_device(c2_accel* a, const float* rays, const int32_t
             return;
         }
         if (a->scene.kd)
-            hipLaunchKernelGGL(k_intersect_test<true>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+            hipLaunchKernelGGL(k_intersect_test<kAccKd>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
                                a->scene.dev, rays, ignore, n, hit, a->scene.kdm[0]);
+        else if (a->scene.wide8)
+            hipLaunchKernelGGL(k_intersect_test<kAccBvh8>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                               a->scene.dev, rays, ignore, n, hit, KdMaster{});
         else
-            hipLaunchKernelGGL(k_intersect_test<false>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+            hipLaunchKernelGGL(k_intersect_test<kAccBvh4>, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
                                a->scene.dev, rays, ignore, n, hit, KdMaster{});
         HIP_CHECK(hipGetLastError());
     });
@@ -2079,6 +2092,7 @@ int c2_accel_count_visits(const c2_accel* a, const float* rays, int64_t n, int32
 
 struct c2_pt {
     std::unique_ptr<c2_accel> accel;
+    bool use8 = false;  // traverse the 8-wide tree (accelerator C2_ACCEL_BVH8, or a batch's choice)
     bool profiling = false;
     uint32_t prof_mask = 0;  // kernel classes timed with events
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
@@ -2177,7 +2191,7 @@ void check_opts(const c2_render_opts* opts) {
         fail(C2_ERR_INVALID, "render options: negative queue_blocks / paths_in_flight");
     if (opts->integrator != C2_INTEGRATOR_PATH && opts->integrator != C2_INTEGRATOR_DIRECT)
         fail(C2_ERR_INVALID, "render options: bad integrator");
-    if (opts->accel != C2_ACCEL_BVH && opts->accel != C2_ACCEL_KDTREE)
+    if (opts->accel != C2_ACCEL_BVH && opts->accel != C2_ACCEL_KDTREE && opts->accel != C2_ACCEL_BVH8)
         fail(C2_ERR_INVALID, "render options: bad accel");
 }
 
@@ -2242,10 +2256,14 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     std::unique_ptr<c2_accel> acc(new c2_accel());
     acc->scene.leaf_size = kLeafSizeRender;
     acc->scene.kd = opts->accel == C2_ACCEL_KDTREE;
+    // a batch keeps both widths, so that its autotuner can choose (c2_batch_autotune)
+    acc->scene.wide8 = opts->accel == C2_ACCEL_BVH8 || (per_sample_passes && opts->accel == C2_ACCEL_BVH);
     acc->scene.build(descs, n);
     pt->accel = std::move(acc);
     const SceneGPU& SG = pt->accel->scene;
     PassParams& P = pt->pp;
+    pt->use8 = opts->accel == C2_ACCEL_BVH8;
+    P.wide8 = pt->use8 ? 1 : 0;
     P.max_depth = opts->max_depth;
     P.arg_order = opts->arg_order;
     P.integrator = opts->integrator;
@@ -2264,6 +2282,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
         V.height = v.height;
         V.spp = v.spp;
         V.root = descs[k]->num_entities > 0 ? SG.offs[k].node : -1;
+        V.root8 = descs[k]->num_entities > 0 && SG.wide8 ? SG.offs[k].node8 : -1;
         if (SG.kd) {
             V.root = SG.kdm[k].root;
             for (int c = 0; c < 3; c++) {
@@ -2502,7 +2521,8 @@ namespace {
 void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipStream_t s0, bool per_sample_passes) {
         PassParams P = pt->pp;
         DevScene S = pt->accel->scene.dev;
-        const bool SGkd = pt->accel->scene.kd;
+        const int acc = pt->accel->scene.kd ? kAccKd : pt->use8 ? kAccBvh8 : kAccBvh4;
+        P.wide8 = acc == kAccBvh8 ? 1 : 0;
         // lanes: chunk c runs on lane c % L (lane 0 = the caller's stream s0); lane 1
         // starts after the work already queued on s0, and s0 waits for it at the end.
         // One lane with the radiance cache (training between chunks), ray capture and
@@ -2584,10 +2604,12 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         }
                         {
                             KTimer kt(pt, C2_K_EXTEND, s);
-                            if (SGkd)
-                                hipLaunchKernelGGL(k_extend<true>, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
+                            if (acc == kAccKd)
+                                hipLaunchKernelGGL(k_extend<kAccKd>, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
+                            else if (acc == kAccBvh8)
+                                hipLaunchKernelGGL(k_extend<kAccBvh8>, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
                             else
-                                hipLaunchKernelGGL(k_extend<false>, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
+                                hipLaunchKernelGGL(k_extend<kAccBvh4>, dim3(g), dim3(kBlock), 0, s, S, LB, P, q);
                         }
                         // the vertex kernels read this bounce's queue grouped by shading class
                         PathBuffers SB = LB;
@@ -2637,19 +2659,24 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         int gs = std::min(grid_for(npath * P.lmax), qcap);
                         {
                             KTimer kt(pt, C2_K_OCCLUDE, side);
-                            if (SGkd)
-                                hipLaunchKernelGGL(k_occlude<true>, dim3(gs), dim3(kBlock), 0, side, S, LB);
+                            if (acc == kAccKd)
+                                hipLaunchKernelGGL(k_occlude<kAccKd>, dim3(gs), dim3(kBlock), 0, side, S, LB);
+                            else if (acc == kAccBvh8)
+                                hipLaunchKernelGGL(k_occlude<kAccBvh8>, dim3(gs), dim3(kBlock), 0, side, S, LB);
                             else
-                                hipLaunchKernelGGL(k_occlude<false>, dim3(gs), dim3(kBlock), 0, side, S, LB);
+                                hipLaunchKernelGGL(k_occlude<kAccBvh4>, dim3(gs), dim3(kBlock), 0, side, S, LB);
                         }
                         HIP_CHECK(hipEventRecord(evj, side));
                         {
                             KTimer kt(pt, C2_K_MIS, s);
-                            if (SGkd)
-                                hipLaunchKernelGGL(k_mis<true>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, s, S,
+                            if (acc == kAccKd)
+                                hipLaunchKernelGGL(k_mis<kAccKd>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, s, S,
                                                    LB, P);
+                            else if (acc == kAccBvh8)
+                                hipLaunchKernelGGL(k_mis<kAccBvh8>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, s,
+                                                   S, LB, P);
                             else
-                                hipLaunchKernelGGL(k_mis<false>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, s,
+                                hipLaunchKernelGGL(k_mis<kAccBvh4>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, s,
                                                    S, LB, P);
                         }
                         HIP_CHECK(hipStreamWaitEvent(s, evj, 0));
@@ -3131,6 +3158,19 @@ int c2_batch_autotune(c2_batch* b, c2_batch_tuning* out) {
             }
         pt->chunk = bestC;
         pt->opts.queue_blocks = bestQ;
+        // ... and the BVH width at those values (the 8-wide tree exists for batches of BVH scenes)
+        const bool have8 = pt->accel->scene.wide8 && !pt->accel->scene.kd;
+        float ms8 = -1.f;
+        if (have8) {
+            const bool save8 = pt->use8;
+            pt->use8 = !save8;
+            enqueue_passes(pt, 0, 1, s, true);  // warm-up of the other width's kernels
+            const float other = timed([&] { enqueue_passes(pt, 0, 1, s, true); });
+            tried++;
+            ms8 = save8 ? bestW : other;
+            if (other < bestW) bestW = other;
+            else pt->use8 = save8;
+        }
         // 2. the persistent kernel: sample 0 of every pixel over the block sizes
         const bool pok = persistent_ok(pt);
         float bestP = INFINITY;
@@ -3184,6 +3224,8 @@ int c2_batch_autotune(c2_batch* b, c2_batch_tuning* out) {
             out->sample0_ms_persistent = pok ? bestP : -1.0;
             out->render_ms_wavefront = wMs;
             out->render_ms_persistent = pok ? pMs : -1.0;
+            out->bvh_width = pt->use8 ? 8 : 4;
+            out->pass_ms_width8 = ms8;
         }
     });
 }
@@ -3201,7 +3243,12 @@ int c2_batch_configure(c2_batch* b, const c2_batch_tuning* cfg) {
                 fail(C2_ERR_UNSUPPORTED, "persistent kernel: needs the BVH, no radiance cache");
         }
         if (cfg->paths_in_flight < 0 || cfg->queue_blocks < 0) fail(C2_ERR_INVALID, "negative launch shape");
+        if (cfg->bvh_width != 0 && cfg->bvh_width != 4 && cfg->bvh_width != 8)
+            fail(C2_ERR_INVALID, "BVH width: 4 or 8 (0 keeps the current one)");
+        if (cfg->bvh_width == 8 && (!pt->accel->scene.wide8 || pt->accel->scene.kd))
+            fail(C2_ERR_UNSUPPORTED, "BVH width 8: the batch has no 8-wide tree");
         b->schedule = cfg->schedule;
+        if (cfg->bvh_width != 0) pt->use8 = cfg->bvh_width == 8;
         if (cfg->schedule == C2_SCHED_PERSISTENT) {
             b->block = cfg->block_threads;
             b->waves = cfg->waves_per_simd;

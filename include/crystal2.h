@@ -68,7 +68,9 @@ enum { C2_LIGHT_AREA = 0, C2_LIGHT_POINT = 1 };
 /* Accelerator::GetAccelerator names (src/Accelerators/Accelerator.cpp:8-23): "BVH" and
  * "GPUBVH" run the GPU BVH; "KDTree" runs the reference's KD-tree (the same tree, traversed
  * on the GPU with the reference's windows, so results match it rather than the BVH). */
-enum { C2_ACCEL_BVH = 0, C2_ACCEL_KDTREE = 1 };
+/* C2_ACCEL_BVH8: the same SAH tree collapsed 8 wide (256-byte nodes) instead of 4 wide; the
+ * results are those of C2_ACCEL_BVH (closest hits tie to the lowest entity index). */
+enum { C2_ACCEL_BVH = 0, C2_ACCEL_KDTREE = 1, C2_ACCEL_BVH8 = 2 };
 
 typedef struct c2_material {
     int32_t type;          /* C2_MAT_*                                      */
@@ -416,10 +418,15 @@ typedef struct c2_batch_tuning {
     int32_t schedule;         /* C2_SCHED_*                                      */
     int32_t block_threads;    /* persistent: threads per block (128, 256, 512)   */
     int32_t waves_per_simd;   /* persistent: registers limited so that at least  */
-                              /* this many waves fit a SIMD (2, 4), or 0: none   */
+                              /* this many waves fit a SIMD (4, 6, 8), or 0: none */
     double sample0_ms_persistent;  /* persistent: sample 0 of every pixel (-1: n/a) */
     double render_ms_wavefront;    /* whole render, captured graph, each schedule's */
     double render_ms_persistent;   /* best values (-1: persistent not applicable)   */
+    int32_t bvh_width;        /* wavefront: BVH width traversed, 4 or 8 (the     */
+                              /* persistent kernel traverses the 4-wide tree);   */
+                              /* configure: 0 keeps the current width            */
+    double pass_ms_width8;    /* wavefront: sample pass 0 with the 8-wide tree at */
+                              /* the chosen values (-1: not timed)               */
 } c2_batch_tuning;
 typedef struct c2_batch_stats {
     int64_t samples, samples_per_render;
@@ -431,14 +438,14 @@ typedef struct c2_batch_stats {
 int  c2_batch_create(const c2_batch_item* items, int32_t n, const c2_render_opts* opts, c2_batch** out);
 void c2_batch_destroy(c2_batch* batch);
 /* Tunes each schedule on sample 0 (wavefront: pass 0 over paths-in-flight x
- * queue_blocks; persistent: 128 / 256 / 512 threads per block x 0 / 2 / 4 waves per
- * SIMD), then times both winners over
+ * queue_blocks, then the BVH width 4 or 8 at the best of those; persistent: 128 / 256 /
+ * 512 threads per block x 0 / 4 / 6 / 8 waves per SIMD), then times both winners over
  * the whole render as captured graphs and keeps the faster. Drops a captured graph and
  * clears the films. */
 int  c2_batch_autotune(c2_batch* batch, c2_batch_tuning* out);
 /* Sets the schedule and launch shape explicitly (fields of c2_batch_tuning: schedule,
- * block_threads and waves_per_simd, and paths_in_flight / queue_blocks when > 0; the
- * timing fields are ignored). Drops a captured graph. */
+ * block_threads and waves_per_simd, paths_in_flight / queue_blocks when > 0, and
+ * bvh_width when > 0; the timing fields are ignored). Drops a captured graph. */
 int  c2_batch_configure(c2_batch* batch, const c2_batch_tuning* cfg);
 /* Clears the films and renders every sample of every scene; asynchronous. */
 int  c2_batch_render(c2_batch* batch, int32_t use_graph);

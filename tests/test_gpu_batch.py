@@ -63,10 +63,28 @@ def test_batch_autotune_keeps_results(gpu_available):
     b = c2.BatchRenderer(items, DEPTH, arg_order="rtl", film="f64")
     t = b.autotune()
     assert t["candidates"] >= 3 and t["pass_ms"] > 0
+    assert t["bvh_width"] in (4, 8) and t["pass_ms_width8"] > 0
     b.render(True)
     b.sync()
     for i, r in enumerate(ref):
         assert np.array_equal(b.film(i), r)
+    b.close()
+
+
+@pytest.mark.parametrize("width", [8, 4])
+def test_batch_bvh_width(gpu_available, width):
+    """Both BVH widths of a batch (the wavefront traverses the 8-wide or the 4-wide tree)
+    render the single-scene films, in direct launches and the captured graph."""
+    print(f"case batch bvh width {width}", flush=True)
+    items = _items()
+    ref = [_alone(*it) for it in items]
+    b = c2.BatchRenderer(items, DEPTH, arg_order="rtl")
+    b.configure("wavefront", bvh_width=width)
+    for use_graph in (False, True):
+        b.render(use_graph)
+        b.sync()
+        for i, r in enumerate(ref):
+            assert np.array_equal(b.film(i), r), f"item {i} width={width} graph={use_graph}"
     b.close()
 
 
@@ -172,8 +190,9 @@ def test_persistent_refuses_the_kdtree(gpu_available):
 def test_batch_64_scene_workload(gpu_available):
     """The batched-scenes workload at its configured size: 64 sphere-field scenes with
     triangle counts log-uniform in 1k-200k, resolutions in {128, 256, 512}^2 and spp in
-    {4, 16, 64} (198 M samples), depth 8, autotuned, captured into a hipGraph and replayed
-    twice. Every film is bit-identical to the scene rendered alone."""
+    {4, 16, 64} (198 M samples), depth 8, autotuned (schedule, launch shape, BVH width),
+    captured into a hipGraph and replayed twice. Every film is bit-identical to the scene
+    rendered alone."""
     print("case batch 64-scene workload", flush=True)
     specs = scenes.batch_specs(64, 6400)
     items = scenes.build_batch(specs)
@@ -187,7 +206,9 @@ def test_batch_64_scene_workload(gpu_available):
     b = c2.BatchRenderer(items, 8, arg_order="rtl")
     t = b.autotune()
     print(f"autotune: {t}", flush=True)
-    assert t["candidates"] == 21 and t["render_ms_wavefront"] > 0 and t["render_ms_persistent"] > 0
+    # 9 wavefront launch shapes, the other BVH width, 12 persistent launch shapes
+    assert t["candidates"] == 22 and t["render_ms_wavefront"] > 0 and t["render_ms_persistent"] > 0
+    assert t["bvh_width"] in (4, 8) and t["pass_ms_width8"] > 0
     for rep in range(2):
         b.render(True)
         b.sync()

@@ -296,3 +296,62 @@ def test_kdtree_film_vs_reference_golden(gpu_available, name, spp, depth, integ)
     g = gpu_film(load(name), 64, 64, spp, depth, "ltr", integrator=integ, accel="kdtree")
     ref = np.load(os.path.join(GOLDEN, f"film_{name}{'_dl' if integ == 'direct' else ''}_kd.npy"))
     assert relmse(g, ref) <= 1e-4
+
+
+# The 8-wide tree (C2_ACCEL_BVH8): the same SAH tree collapsed 8 wide; with the
+# lowest-entity tie rule the results do not depend on the tree, so films and hits equal the
+# oracle's and the 4-wide tree's exactly.
+@pytest.mark.parametrize("name,spp,depth", [("cornell", 4, 5), ("zoo", 8, 6), ("disks", 8, 6), ("textured", 8, 6)])
+@pytest.mark.parametrize("integ", ["path", "direct"])
+def test_bvh8_film_bit_identical_to_oracle(gpu_available, name, spp, depth, integ):
+    print(f"case bvh8 film {name} {integ}", flush=True)
+    sc = load(name)
+    g = gpu_film(sc, 64, 64, spp, depth, "rtl", integrator=integ, accel="bvh8")
+    o, _ = pyoracle.OracleScene(sc, tie=pyoracle.TIE_LOWEST, integrator=integ).render(64, 64, spp, depth,
+                                                                                       arg_order="rtl")
+    assert np.array_equal(g, o), f"{(g != o).any(-1).sum()} pixels differ"
+
+
+@pytest.mark.parametrize("name", ["icosphere", "zoo", "disks"])
+def test_bvh8_hits_equal_bvh4(gpu_available, name):
+    print(f"case bvh8 hits {name}", flush=True)
+    g = np.load(os.path.join(GOLDEN, f"hits_{name}.npz"))
+    sc = load(name)
+    b4, b8 = c2.GpuBVH(sc), c2.GpuBVH(sc, accel="bvh8")
+    closest = g["rays"].copy()
+    closest[:, 7] = np.inf
+    i4, t4 = b4.intersect(closest)
+    i8, t8 = b8.intersect(closest)
+    assert np.array_equal(i4, i8) and np.array_equal(t4, t8)
+    assert (i8 == g["ids"]).mean() >= 0.9999
+    assert np.array_equal(b8.intersect_test(g["rays"]), g["any"].astype(bool))
+    b4.close()
+    b8.close()
+
+
+def test_bvh8_sphere_field_equals_bvh4(gpu_available):
+    """A deeper tree (60 spheres of 1,280 triangles): 8-wide films and ray casts equal the
+    4-wide ones bit for bit."""
+    from crystal2_amd import scenes
+    print("case bvh8 sphere field", flush=True)
+    sc = scenes.build_spheres_scene(60, 3, seed=3)
+    f4 = gpu_film(sc, 96, 54, 4, 8)
+    f8 = gpu_film(sc, 96, 54, 4, 8, accel="bvh8")
+    assert np.array_equal(f4, f8)
+    rng = np.random.default_rng(5)
+    n = 1 << 16
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = rng.uniform(-12, 12, (n, 3))
+    rays[:, 1] = rng.uniform(0.3, 6, n)
+    d = rng.normal(size=(n, 3))
+    rays[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = np.inf
+    b4, b8 = c2.GpuBVH(sc), c2.GpuBVH(sc, accel="bvh8")
+    i4, t4 = b4.intersect(rays)
+    i8, t8 = b8.intersect(rays)
+    assert np.array_equal(i4, i8) and np.array_equal(t4, t8)
+    assert 0.05 < (i4 >= 0).mean() < 0.95
+    rays[:, 7] = 1.0
+    assert np.array_equal(b4.intersect_test(rays), b8.intersect_test(rays))
+    b4.close()
+    b8.close()
