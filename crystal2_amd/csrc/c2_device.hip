@@ -174,6 +174,7 @@ enum : int32_t {
     F_NEE = 4,        // this vertex ran UniformSampleAllLights (resolve adds beta * U)
     F_MIS = 8,        // MIS ray queued for this vertex
     F_TRAIN = 16,     // radiance-cache training path (index in bits 8..31)
+    F_QUERY = 32,     // the path ended in the radiance cache; meta.z holds its query slot
 };
 // Flags of a bounce's vertex record (vtx_i.w)
 enum : int32_t {
@@ -218,7 +219,8 @@ struct PathBuffers {
     uint8_t* sh_vis; // [slot] 1 = unoccluded, 0 = occluded
     int32_t* queue[2];
     // radiance cache
-    float4* nrc_feat;   // [query][kNrcRec4]: packed query features, throughput and path, by query slot
+    float4* nrc_feat;   // [query][kNrcRec4]: packed query features, then the throughput (replaced by
+                        // throughput x prediction by the inference), by query slot
     int32_t* tlist;     // training path -> path
     float4* trec;       // [training path][depth][kNrcQ4 + 2]: packed query, beta, L after emission
     uint8_t* trec_valid;
@@ -903,7 +905,9 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                     const CacheQuery cq = cache_query(S, it, d, B.texel + i);
                     float4* rec = B.nrc_feat + (int64_t)cpos * kNrcRec4;
                     for (int k = 0; k < kNrcQ4; k++) rec[k] = cq.q[k];
-                    rec[kNrcQ4] = make_float4(beta.x, beta.y, beta.z, __int_as_float(p));
+                    rec[kNrcQ4] = make_float4(beta.x, beta.y, beta.z, 0.f);
+                    dim = cpos;  // the Halton state is dead: k_accumulate finds the prediction here
+                    flags = F_QUERY;
                 } else if (it.material < 0) {
                     // transparent: SpawnRay(dir), no dimensions consumed, :63-68. The
                     // direct-lighting integrator ends the path at a transparent emitter
@@ -1238,7 +1242,12 @@ __global__ void __launch_bounds__(kBlock) k_accumulate(PathBuffers B, PassParams
         for (int js = 0; js < P.nsamp; js++) {
             const int i = js * P.pix_count + t;
             float4 L4 = B.L[i];
-            film_add(mode, film, comp, film64, B.meta[i].x, V3{L4.x, L4.y, L4.z});
+            const int4 m = B.meta[i];
+            if (m.w & F_QUERY) {  // L + beta * cache prediction (k_nrc_infer wrote the product)
+                const float4 c = B.nrc_feat[(int64_t)m.z * kNrcRec4 + kNrcQ4];
+                L4 = make_float4(L4.x + c.x, L4.y + c.y, L4.z + c.z, L4.w);
+            }
+            film_add(mode, film, comp, film64, m.x, V3{L4.x, L4.y, L4.z});
         }
     }
 }
@@ -2164,7 +2173,7 @@ struct c2_pt {
 };
 
 namespace c2 {
-void nrc_launch_infer(c2_nrc* c, const int32_t* countp, int32_t maxn, const float4* recs, float4* L, hipStream_t s);
+void nrc_launch_infer(c2_nrc* c, const int32_t* countp, int32_t maxn, float4* recs, hipStream_t s);
 void nrc_launch_train(c2_nrc* c, const float4* feats, const float4* targets, const int32_t* countp, int32_t js,
                       int32_t ns, int32_t maxn, hipStream_t s, bool adam);
 }  // namespace c2
@@ -2634,7 +2643,7 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                             timing_event(&e0);
                             timing_event(&e1);
                             HIP_CHECK(hipEventRecord(e0, s));
-                            nrc_launch_infer(pt->nrc, LB.counters + 4, (int32_t)npath, LB.nrc_feat, LB.L, s);
+                            nrc_launch_infer(pt->nrc, LB.counters + 4, (int32_t)npath, LB.nrc_feat, s);
                             HIP_CHECK(hipEventRecord(e1, s));
                             pt->cache_events.push_back({0, {e0, e1}});
                         }

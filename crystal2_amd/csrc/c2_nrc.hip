@@ -208,24 +208,26 @@ __device__ __forceinline__ void output_layer(const FwdLds& L, const half8_t in[2
     }
 }
 
-// Inference over dense queries 0..n-1. L == nullptr: kNrcQ4 float4 per query, results in
-// out[q] (float4). Otherwise the renderer's records (kNrcRec4 float4: the query, then the
-// path's throughput and index p): L[p].rgb += throughput * max(0, prediction). The record
-// carries the throughput and the path, so the update waits on one scattered load (L[p]).
-// (No-return float atomic adds instead of the read-modify-write measured 1.75x slower: 4.5 M
-// scattered atomics per launch.) A wavefront takes 4 tiles of 16 queries per iteration;
+// Inference over dense queries 0..n-1. recs == nullptr: kNrcQ4 float4 per query in feats,
+// results in out[q] (float4). Otherwise the renderer's records in recs (kNrcRec4 float4: the
+// query, then the path's throughput): the throughput's float4 becomes throughput x max(0,
+// prediction), which k_accumulate adds to the path's radiance, so every access of the kernel
+// is dense. (Adding it to the path's radiance here took a scattered read-modify-write per
+// query; as no-return float atomics it measured 1.75x slower still.) A wavefront takes 4 tiles of
+// 16 queries per iteration;
 // four blocks of 40 KB LDS fit a CU, and the MFMA accumulators live in VGPRs (the file is
 // built with -amdgpu-mfma-vgpr-form), so four waves per SIMD.
 __global__ void __launch_bounds__(kNrcBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_nrc_infer(NrcDev N, const int32_t* __restrict__ countp, int32_t nfixed, const float4* __restrict__ feats,
-            float4* __restrict__ L, float4* __restrict__ out, unsigned long long* tot) {
+k_nrc_infer(NrcDev N, const int32_t* __restrict__ countp, int32_t nfixed, const float4* feats, float4* recs,
+            float4* __restrict__ out, unsigned long long* tot) {
     constexpr int NT = kInferTiles;
     __shared__ __align__(16) FwdLds S;
     stage_forward<kNrcBlock>(N, S);
     __syncthreads();
     const int n = countp ? min(*countp, nfixed) : nfixed;
     if (tot && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(tot, (unsigned long long)n);
-    const int stride = L ? kNrcRec4 : kNrcQ4;
+    const int stride = recs ? kNrcRec4 : kNrcQ4;
+    const float4* src = recs ? recs : feats;  // a record's query floats (slots 0..2) are only read
     const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, g = l >> 4, c = l & 15;
     for (int base = (blockIdx.x * 4 + wv) * 16 * NT; base < n; base += gridDim.x * 4 * 16 * NT) {  // wave-uniform
         half8_t x[2][NT];
@@ -233,8 +235,8 @@ k_nrc_infer(NrcDev N, const int32_t* __restrict__ countp, int32_t nfixed, const 
 #pragma unroll
         for (int nt = 0; nt < NT; nt++) {
             const int q = min(base + 16 * nt + c, n - 1);  // past the end: a copy of the last query
-            q0[nt] = feats[(int64_t)q * stride];
-            qg[nt] = feats[(int64_t)q * stride + (g == 2 ? 2 : 1)];
+            q0[nt] = src[(int64_t)q * stride];
+            qg[nt] = src[(int64_t)q * stride + (g == 2 ? 2 : 1)];
         }
 #pragma unroll
         for (int nt = 0; nt < NT; nt++) {
@@ -254,11 +256,10 @@ k_nrc_infer(NrcDev N, const int32_t* __restrict__ countp, int32_t nfixed, const 
                 const int q = base + 16 * nt + c;
                 if (q >= n) continue;
                 const float r = fmaxf(y[nt][0], 0.f), gg = fmaxf(y[nt][1], 0.f), bb = fmaxf(y[nt][2], 0.f);
-                if (L) {
-                    const float4 tb = feats[(int64_t)q * kNrcRec4 + kNrcQ4];  // throughput, path
-                    const int p = __float_as_int(tb.w);
-                    const float4 L4 = L[p];
-                    L[p] = make_float4(L4.x + tb.x * r, L4.y + tb.y * gg, L4.z + tb.z * bb, L4.w);
+                if (recs) {
+                    float4* tb = recs + (int64_t)q * kNrcRec4 + kNrcQ4;  // the throughput
+                    const float4 b4 = *tb;
+                    *tb = make_float4(b4.x * r, b4.y * gg, b4.z * bb, 0.f);
                 } else {
                     out[q] = make_float4(r, gg, bb, 0.f);
                 }
@@ -552,9 +553,9 @@ static int nrc_grid(int64_t rows, int per_block, int cap) {
 constexpr int kInferBlocks = 1024;  // 4 blocks of 40 KB LDS per CU
 
 // Launches used by the renderer (c2_device.hip) on its stream.
-void nrc_launch_infer(c2_nrc* c, const int32_t* countp, int32_t maxn, const float4* recs, float4* L, hipStream_t s) {
+void nrc_launch_infer(c2_nrc* c, const int32_t* countp, int32_t maxn, float4* recs, hipStream_t s) {
     hipLaunchKernelGGL(k_nrc_infer, dim3(nrc_grid(maxn, 4 * 16 * kInferTiles, kInferBlocks)), dim3(kNrcBlock), 0, s,
-                       c->dev, countp, maxn, recs, L, (float4*)nullptr, c->totals);
+                       c->dev, countp, maxn, (const float4*)nullptr, recs, (float4*)nullptr, c->totals);
 }
 
 // One training step on slice js of ns of the *countp rows collected on the device (at most
