@@ -1042,9 +1042,11 @@ __global__ void __launch_bounds__(kBlock) k_cache_collect(PathBuffers B, PassPar
 // and the BSDF built once per light, then per light j the light sample (a shadow ray in
 // the vertex's next free slot) and the test whether the shared BSDF-sampled direction
 // needs the MIS ray (Pdf_Li != 0).
-// (Three waves per SIMD asked for: without it the allocator settles at 169 VGPRs, one past
-// the 168 of three waves; with it 168 and the same 20 B of scratch.)
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3)))
+// (Four waves per SIMD asked for: 128 VGPRs and 168 B of scratch per lane instead of 165
+// VGPRs at three waves. The kernel waits on memory more than it issues, so the fourth wave
+// pays for the spills: +1.5-1.9% on the 1080p sphere field; five waves (96 VGPRs, 292 B)
+// lose 2%.)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
     __shared__ int32_t sm[kWaves * kAppendLights + 1];
     int n = B.counters[q];
