@@ -25,6 +25,14 @@ import time
 
 import numpy as np
 
+# A renderer's two lanes and their side streams overlap only on different hardware queues.
+# HIP spreads a process's streams over GPU_MAX_HW_QUEUES queues (4 by default) in an order
+# that depends on every stream created before, and this benchmark creates many renderers:
+# with 4 queues the same renderer, created five times in one process, ran at 405, 367, 367,
+# 407 and 408 Msamples/s; with 16, at 407-408 every time. The runtime reads the setting when
+# it starts, so it is set before the library is loaded (a value already set is kept).
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
