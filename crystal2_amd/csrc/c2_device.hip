@@ -85,6 +85,25 @@ static_assert(kLanes >= 1 && kLanes <= 4, "1 to 4 lanes");
 #define C2_SAMPLES_PER_CHUNK 4
 #endif
 constexpr int kSamplesPerChunk = C2_SAMPLES_PER_CHUNK;
+// Small frames: a launch carries more samples, up to kSmallFrameSamples, so that it holds
+// about kSmallFramePaths paths (a 256 x 256 frame: 16 samples per launch instead of 4;
+// frames from 512 x 512 up keep kSamplesPerChunk). Launches of a few hundred thousand paths
+// are bound by launch latency, not by the GPU.
+#ifndef C2_SMALL_FRAME_PATHS
+#define C2_SMALL_FRAME_PATHS (1 << 20)
+#endif
+#ifndef C2_SMALL_FRAME_SAMPLES
+#define C2_SMALL_FRAME_SAMPLES 16
+#endif
+constexpr int64_t kSmallFramePaths = C2_SMALL_FRAME_PATHS;
+constexpr int kSmallFrameSamples = C2_SMALL_FRAME_SAMPLES;
+// Samples per launch of a single renderer whose largest pass has npix paths per sample
+// (`base` samples for frames of kSmallFramePaths / base paths and more).
+inline int samples_per_launch(int base, int64_t npix) {
+    if (npix <= 0) return base;
+    const int64_t want = (kSmallFramePaths + npix - 1) / npix;
+    return (int)std::max<int64_t>(base, std::min<int64_t>(want, kSmallFrameSamples));
+}
 #ifndef C2_BATCH_SAMPLES_PER_CHUNK
 #define C2_BATCH_SAMPLES_PER_CHUNK 16
 #endif
@@ -2366,9 +2385,10 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     int64_t maxPass = 0;
     for (auto& t : pt->passes) maxPass = std::max(maxPass, t.npix);
     // default: the samples per launch of the largest pass (enqueue_passes)
-    const int64_t dflt = maxPass * (per_sample_passes                          ? kBatchSamplesPerChunk
-                                    : opts->integrator == C2_INTEGRATOR_DIRECT ? kDirectSamplesPerChunk
-                                                                               : kSamplesPerChunk);
+    const int64_t dflt =
+        maxPass * (per_sample_passes                          ? kBatchSamplesPerChunk
+                   : opts->integrator == C2_INTEGRATOR_DIRECT ? samples_per_launch(kDirectSamplesPerChunk, maxPass)
+                                                              : samples_per_launch(kSamplesPerChunk, maxPass));
     int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : dflt;
     // keep the path buffers within ~16 GiB: 8 GiB in each of two lanes, or 16 GiB in a
     // batch's one lane
@@ -2560,11 +2580,14 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
             in.chunk = pt->chunk;
             // samples per chunk: several whole passes when the buffers hold them (not with ray
             // capture, which reads one pass)
+            int64_t maxPass = 0;
+            for (const auto& t : pt->passes) maxPass = std::max<int64_t>(maxPass, t.npix);
             in.samples_per_chunk = pt->cap_bounce >= 0                      ? 1
                                    : per_sample_passes                      ? kBatchSamplesPerChunk
                                    : P.cache_bounce >= 0                   ? kCacheSamplesPerChunk
-                                   : P.integrator == C2_INTEGRATOR_DIRECT ? kDirectSamplesPerChunk
-                                                                           : kSamplesPerChunk;
+                                   : P.integrator == C2_INTEGRATOR_DIRECT
+                                       ? samples_per_launch(kDirectSamplesPerChunk, maxPass)
+                                       : samples_per_launch(kSamplesPerChunk, maxPass);
             in.lanes = L;
             if (plan_passes(in, plan) != 0) fail(C2_ERR_INVALID, "render: sample range outside the pass tables");
         }

@@ -277,6 +277,7 @@ typedef struct c2_render_opts {
     int32_t film_mode;        /* C2_FILM_*                                             */
     int32_t paths_in_flight;  /* paths per launch; 0 = four samples of every pixel
                                  (a launch then carries up to four consecutive samples;
+                                 frames under 2^18 pixels up to 16, about 2^20 paths;
                                  below one pass, a pass is split into chunks)        */
     int32_t queue_blocks;     /* grid cap (blocks) of the per-bounce queue kernels,
                                  which loop grid-stride; 0 = the library default     */
