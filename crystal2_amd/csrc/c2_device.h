@@ -603,9 +603,12 @@ constexpr int kKdStack = 24;  // > the tree's depth limit of 20: at most one ent
 // The KD traversal's stack as the BVH's: the top kKdRing entries (node, window start,
 // window end) of each lane in an LDS ring at [slot][threadIdx], deeper entries in a
 // private overflow array. A register array indexed by the stack pointer would hold all
-// kKdStack entries in VGPRs (one wavefront per SIMD).
-constexpr int kKdRing = 8;
-struct KdLds {  // 24 KB per block; declare as `__shared__ KdLds lds;`
+// kKdStack entries in VGPRs (one wavefront per SIMD). Four resident entries (12 KB per
+// 256-thread block) leave the KD kernels at 8 waves per SIMD, which their 45-61 VGPRs
+// allow; eight (24 KB) held them at 6 and rendered the 100-sphere field at 1080p 7% slower
+// (42.4 against 45.3 Msamples/s, identical films).
+constexpr int kKdRing = 4;
+struct KdLds {  // 12 KB per block; declare as `__shared__ KdLds lds;`
     int32_t n[kKdRing * kTravBlock];
     float a[kKdRing * kTravBlock], b[kKdRing * kTravBlock];
 };
