@@ -329,6 +329,50 @@ def test_bvh8_hits_equal_bvh4(gpu_available, name):
     b8.close()
 
 
+@pytest.mark.parametrize("accel", ["bvh", "kdtree"])
+def test_wave_uniform_rays(gpu_available, accel):
+    """Each wavefront of 64 lanes holding one ray repeated, so every traversal step is
+    wave-uniform (the node, and for the KD-tree the leaf records, come in through scalar
+    loads), against the same rays one per lane (mostly vector loads): ids, t and any-hit
+    results equal bit for bit, and equal the oracle's (exactly for the KD-tree, whose
+    first-found rule the oracle restates; to the raycast bound for the BVH)."""
+    from crystal2_amd import scenes
+    print(f"case wave-uniform rays {accel}", flush=True)
+    sc = scenes.build_spheres_scene(60, 3, seed=3)
+    rng = np.random.default_rng(9)
+    n = 1024
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = rng.uniform(-12, 12, (n, 3))
+    rays[:, 1] = rng.uniform(4, 9, n)
+    tgt = np.stack([rng.uniform(-10, 10, n), rng.uniform(0, 2, n), rng.uniform(-10, 10, n)], 1)
+    d = tgt - rays[:, 0:3]
+    rays[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = np.inf
+    uni = np.repeat(rays, 64, axis=0)  # ray k fills lanes 64k .. 64k + 63 (one wavefront)
+    acc = c2.GpuBVH(sc, accel=accel)
+    i1, t1 = acc.intersect(rays)
+    iu, tu = acc.intersect(uni)
+    assert np.array_equal(iu, np.repeat(i1, 64)) and np.array_equal(tu, np.repeat(t1, 64))
+    assert (i1 >= 0).mean() > 0.3
+    kd = accel == "kdtree"
+    o = pyoracle.OracleScene(sc, accel=pyoracle.ACCEL_KDTREE if kd else pyoracle.ACCEL_BVH,
+                             tie=pyoracle.TIE_FIRST if kd else pyoracle.TIE_LOWEST)
+    oi, ot = o.intersect(rays)
+    if kd:
+        assert np.array_equal(i1, oi) and np.array_equal(t1[oi >= 0], ot[oi >= 0])
+    else:
+        assert (i1 == oi).mean() >= 0.999
+        h = (oi >= 0) & (i1 == oi)
+        assert np.all(np.abs(t1[h] - ot[h]) <= 1e-5 * ot[h])
+    sr = rays.copy()
+    sr[:, 7] = rng.uniform(2, 12, n).astype(np.float32)
+    a1 = acc.intersect_test(sr)
+    au = acc.intersect_test(np.repeat(sr, 64, axis=0))
+    assert np.array_equal(au, np.repeat(a1, 64))
+    assert np.array_equal(a1, o.intersect_test(sr))
+    acc.close()
+
+
 def test_bvh8_sphere_field_equals_bvh4(gpu_available):
     """A deeper tree (60 spheres of 1,280 triangles): 8-wide films and ray casts equal the
     4-wide ones bit for bit."""
