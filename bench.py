@@ -677,27 +677,8 @@ def main():
     if args.warmup:
         pt.render(0, args.warmup)
         pt.sync()
-    pt.reset()
-    st0 = pt.stats()
-    # HIP events only around the two kernels the roofline may be quoted on: events around
-    # every launch would slow the timed region by ~4%
-    pt.set_profiling(True, kernels=["extend", "occlude"])
-    pt.sync()
-    t0 = time.perf_counter()
-    pt.render(0, K)
-    pt.sync()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    kt = pt.kernel_times()
-    st = pt.stats()
-    samples = W * H * K
-    value = samples / elapsed / 1e6
-    ext = st["extension_rays"] - st0["extension_rays"]
-    sh = st["shadow_rays"] - st0["shadow_rays"]
-    mis = st["mis_rays"] - st0["mis_rays"]
-    log(f"timed {K} passes in {elapsed:.3f}s: {value:.2f} Msamples/s; rays/sample ext {ext / samples:.2f} "
-        f"shadow {sh / samples:.2f} mis {mis / samples:.3f}")
-    # per-kernel breakdown from a separate, fully instrumented render of 4 passes (not timed)
+    # per-kernel breakdown from a separate, fully instrumented render of 4 passes (not
+    # timed); it also names the dominant tracing kernel, the one the roofline is quoted on
     pt.reset()
     pt.set_profiling(True)
     sb = pt.stats()
@@ -710,6 +691,28 @@ def main():
             log(f"  {name:10s} {v['ms'] / 4:9.2f} ms/pass  {v['launches'] / 4:6.2f} launches/pass  "
                 f"{v['ms'] / v['launches']:.3f} ms avg")
     pt.set_profiling(False)
+    dom = max(("extend", "occlude"), key=lambda k: kti[k]["ms"])
+    pt.reset()
+    st0 = pt.stats()
+    # HIP events in the timed region only around the dominant kernel's launches: events
+    # around every launch slow it by ~4%, around both tracing kernels by ~2%
+    pt.set_profiling(True, kernels=[dom])
+    pt.sync()
+    t0 = time.perf_counter()
+    pt.render(0, K)
+    pt.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kt = pt.kernel_times()
+    st = pt.stats()
+    pt.set_profiling(False)
+    samples = W * H * K
+    value = samples / elapsed / 1e6
+    ext = st["extension_rays"] - st0["extension_rays"]
+    sh = st["shadow_rays"] - st0["shadow_rays"]
+    mis = st["mis_rays"] - st0["mis_rays"]
+    log(f"timed {K} passes in {elapsed:.3f}s: {value:.2f} Msamples/s; rays/sample ext {ext / samples:.2f} "
+        f"shadow {sh / samples:.2f} mis {mis / samples:.3f}")
     # rays per second of each tracing stage over its own launches (wall time of those
     # launches; with two lanes another lane's kernels share the GPU meanwhile)
     stages = {}
@@ -720,7 +723,6 @@ def main():
         stages[stage] = {"rays": int(rays), "ms": round(ms, 3),
                          "mrays_s": round(rays / (ms / 1e3) / 1e6, 1) if ms > 0 else None}
 
-    dom = max(("extend", "occlude"), key=lambda k: kt[k]["ms"])
     rays_dom = ext if dom == "extend" else sh
     bpr = counts["extension" if dom == "extend" else "shadow"][2]
     avg_s = kt[dom]["ms"] / 1e3 / max(1, kt[dom]["launches"])
