@@ -70,13 +70,19 @@ def per_ray_bytes(nodes, tris):
     return 24 + 8 + nodes * 32 + tris * 36
 
 
-def load_traffic(kernel):
+def load_traffic(kernel, rays_per_launch=None):
+    """HBM bytes per launch of `kernel` from the committed counter summary, scaled from the
+    profiled launches' mean ray count to `rays_per_launch` when both are known."""
     path = os.path.join(HERE, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         t = json.load(fh)
-    return t.get(kernel)
+    v = t.get(kernel)
+    prof = t.get("_rays_per_launch", {}).get(kernel)
+    if v is not None and prof and rays_per_launch:
+        v = round(v * rays_per_launch / prof)
+    return v
 
 
 def copy_calibration(mib, reps=3):
@@ -729,7 +735,7 @@ def main():
     bytes_per_launch = rays_dom * bpr / max(1, kt[dom]["launches"])
     achieved = bytes_per_launch / avg_s / 1e9
     kname = {"extend": "k_extend", "occlude": "k_occlude"}[dom]
-    traffic = load_traffic(kname)
+    traffic = load_traffic(kname, rays_dom / max(1, kt[dom]["launches"]))
     # The path has no dense contraction: HBM is the roofline that applies. What the kernel
     # actually moves through HBM (the committed FETCH_SIZE + WRITE_SIZE summary) over its
     # launch time tells whether it is at that roof or limited below it by memory latency

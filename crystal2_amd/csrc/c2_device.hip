@@ -2384,11 +2384,13 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
     }
     int64_t maxPass = 0;
     for (auto& t : pt->passes) maxPass = std::max(maxPass, t.npix);
-    // default: the samples per launch of the largest pass (enqueue_passes)
-    const int64_t dflt =
-        maxPass * (per_sample_passes                          ? kBatchSamplesPerChunk
-                   : opts->integrator == C2_INTEGRATOR_DIRECT ? samples_per_launch(kDirectSamplesPerChunk, maxPass)
-                                                              : samples_per_launch(kSamplesPerChunk, maxPass));
+    // default: the samples per launch of the largest pass (enqueue_passes), half as many again
+    // for a single renderer, whose chunks grow that far when it ends their count on a
+    // multiple of the lanes (plan_passes)
+    const int64_t spl = per_sample_passes                          ? kBatchSamplesPerChunk
+                        : opts->integrator == C2_INTEGRATOR_DIRECT ? samples_per_launch(kDirectSamplesPerChunk, maxPass)
+                                                                   : samples_per_launch(kSamplesPerChunk, maxPass);
+    const int64_t dflt = maxPass * (per_sample_passes ? spl : spl + spl / 2);
     int64_t chunk = opts->paths_in_flight > 0 ? opts->paths_in_flight : dflt;
     // keep the path buffers within ~16 GiB: 8 GiB in each of two lanes, or 16 GiB in a
     // batch's one lane
@@ -2589,6 +2591,8 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                                        ? samples_per_launch(kDirectSamplesPerChunk, maxPass)
                                        : samples_per_launch(kSamplesPerChunk, maxPass);
             in.lanes = L;
+            if (!per_sample_passes && pt->cap_bounce < 0 && P.cache_bounce < 0 && in.samples_per_chunk > 1)
+                in.max_samples_per_chunk = in.samples_per_chunk + in.samples_per_chunk / 2;
             if (plan_passes(in, plan) != 0) fail(C2_ERR_INVALID, "render: sample range outside the pass tables");
         }
         if (L > 1) {

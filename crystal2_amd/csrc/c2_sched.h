@@ -41,7 +41,25 @@ struct SchedInput {
     int64_t chunk;             // paths the path buffers of one lane hold
     int32_t samples_per_chunk; // at most this many whole passes per chunk (1: one pass, split)
     int32_t lanes;
+    // A single renderer's chunks may grow to this many passes (0: samples_per_chunk) so that
+    // their count is a multiple of the lanes: with 2 lanes, 20 passes of 4 would leave a
+    // fifth chunk running alone, 4 chunks of 5 end together.
+    int32_t max_samples_per_chunk;
 };
+
+// Samples per chunk of a single renderer's `count` samples: chunks of `spc`, unless their
+// count is not a multiple of `lanes`; then the next lower multiple of chunks, the samples
+// spread evenly, when no chunk exceeds `cap` samples. Empty: keep chunks of `spc`.
+inline std::vector<int64_t> lane_balanced_chunks(int64_t count, int64_t spc, int64_t cap, int32_t lanes) {
+    std::vector<int64_t> ns;
+    if (count <= 0 || spc < 1 || lanes < 2 || cap <= spc) return ns;
+    const int64_t n = (count + spc - 1) / spc;
+    const int64_t n2 = n - n % lanes;
+    if (n % lanes == 0 || n2 < lanes) return ns;
+    if ((count + n2 - 1) / n2 > cap) return ns;
+    for (int64_t i = 0; i < n2; i++) ns.push_back(count / n2 + (i < count % n2 ? 1 : 0));
+    return ns;
+}
 
 // Fills `out` and returns 0, or returns -1 with `out` empty for inputs no plan can serve: a
 // negative range, a sample without a pass table, a chunk that would exceed the buffers.
@@ -52,6 +70,14 @@ inline int plan_passes_into(const SchedInput& in, std::vector<SchedChunk>& out) 
         return -1;
     const int64_t end = (int64_t)in.sample_begin + in.sample_count;
     if (end > INT32_MAX) return -1;
+    std::vector<int64_t> plan;  // a single renderer's chunk sizes, balanced over the lanes
+    size_t planned = 0;
+    if (!in.per_sample_passes && in.max_samples_per_chunk > in.samples_per_chunk && in.passes[0].npix > 0 &&
+        in.passes[0].npix <= INT32_MAX) {
+        const int64_t fit = in.chunk / in.passes[0].npix;
+        plan = lane_balanced_chunks(in.sample_count, std::min<int64_t>(fit, in.samples_per_chunk),
+                                    std::min<int64_t>(fit, in.max_samples_per_chunk), in.lanes);
+    }
     int64_t nchunk = 0;
     for (int64_t k = in.sample_begin; k < end;) {
         const int64_t tab = in.per_sample_passes ? k : 0;
@@ -72,6 +98,7 @@ inline int plan_passes_into(const SchedInput& in, std::vector<SchedChunk>& out) 
                 }
             ns = std::max<int64_t>(1, lim);
         }
+        if (planned < plan.size()) ns = plan[planned++];
         const int64_t step = ns > 1 ? npix : in.chunk;
         for (int64_t beg = 0; beg < npix; beg += step) {
             const int64_t cnt = std::min<int64_t>(step, npix - beg);

@@ -275,10 +275,12 @@ typedef struct c2_render_opts {
     int32_t arg_order;        /* C2_ARGORDER_*: order of Get1D/Get2D at
                                  PathTracingIntegrator.cpp:79 (RTL = GCC/MSVC)       */
     int32_t film_mode;        /* C2_FILM_*                                             */
-    int32_t paths_in_flight;  /* paths per launch; 0 = four samples of every pixel
-                                 (a launch then carries up to four consecutive samples;
-                                 frames under 2^18 pixels up to 16, about 2^20 paths;
-                                 below one pass, a pass is split into chunks)        */
+    int32_t paths_in_flight;  /* paths per launch (path buffers of one lane); 0 = six
+                                 samples of every pixel: a launch then carries four
+                                 consecutive samples (frames under 2^18 pixels up to
+                                 16, about 2^20 paths), up to six when that ends the
+                                 call's launches on a multiple of the lanes; below one
+                                 pass, a pass is split into chunks                   */
     int32_t queue_blocks;     /* grid cap (blocks) of the per-bounce queue kernels,
                                  which loop grid-stride; 0 = the library default     */
     int32_t win_x0, win_y0;   /* pixel window (film coordinates, y = 0 at the bottom); */

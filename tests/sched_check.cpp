@@ -147,6 +147,8 @@ int main() {
                     in.chunk = chunk;
                     in.samples_per_chunk = spc;
                     in.lanes = lanes;
+                    // single renderers: chunks may grow by half to end on a multiple of the lanes
+                    in.max_samples_per_chunk = (!c.batch && U(0, 1)) ? spc + spc / 2 : 0;
                     std::vector<SchedChunk> plan;
                     const int rc = plan_passes(in, plan);
                     CHECK(rc == 0, "trial %d: plan refused", trial);
@@ -184,6 +186,38 @@ int main() {
         bad.sample_begin = 0;
         bad.lanes = 0;
         CHECK(plan_passes(bad, plan) == -1, "zero lanes were planned");
+    }
+    {  // a single renderer's chunk sizes with chunks allowed to grow from 4 to 6 samples
+        std::vector<SchedPass> tab{{0, 1, 1000}};
+        struct Case {
+            int count, lanes;
+            std::vector<int> want;
+        };
+        const Case cases[] = {{20, 2, {5, 5, 5, 5}},     {16, 2, {4, 4, 4, 4}}, {12, 2, {6, 6}},
+                              {4, 2, {4}},               {9, 2, {5, 4}},        {13, 2, {4, 4, 4, 1}},
+                              {28, 2, {5, 5, 5, 5, 4, 4}}, {20, 3, {4, 4, 4, 4, 4}}, {20, 1, {4, 4, 4, 4, 4}},
+                              {18, 3, {6, 6, 6}}};
+        for (const Case& cs : cases) {
+            SchedInput in{};
+            in.passes = tab.data();
+            in.npasses = 1;
+            in.sample_begin = 5;
+            in.sample_count = cs.count;
+            in.chunk = 1000 * 6;
+            in.samples_per_chunk = 4;
+            in.max_samples_per_chunk = 6;
+            in.lanes = cs.lanes;
+            std::vector<SchedChunk> plan;
+            CHECK(plan_passes(in, plan) == 0, "plan of %d samples refused", cs.count);
+            std::vector<int> got;
+            int next = in.sample_begin;
+            for (const SchedChunk& ch : plan) {
+                got.push_back(ch.nsamp);
+                CHECK(ch.sample == next, "chunk starts at sample %d, expected %d", ch.sample, next);
+                next += ch.nsamp;
+            }
+            CHECK(got == cs.want, "chunk sizes of %d samples on %d lanes", cs.count, cs.lanes);
+        }
     }
     std::printf("sched_check: %d plans, %d chunks, %d failures\n", plans, chunks, failures);
     return failures ? 1 : 0;
