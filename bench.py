@@ -594,8 +594,8 @@ def batch_bench(n=64, seed=6400, depth=8):
            "speedup": round(single_s / batch_s, 3), "films_bit_identical": f"{exact}/{n}",
            "schedule": tune["schedule"], "graph_nodes": st["graph_nodes"],
            "autotune": {"paths_in_flight": tune["paths_in_flight"], "queue_blocks": tune["queue_blocks"],
-                        "candidates": tune["candidates"], "pass0_ms": round(tune["pass_ms"], 3),
-                        "bvh_width": tune["bvh_width"], "pass0_ms_width8": round(tune["pass_ms_width8"], 3),
+                        "candidates": tune["candidates"], "passes_ms": round(tune["pass_ms"], 3),
+                        "bvh_width": tune["bvh_width"], "passes_ms_width8": round(tune["pass_ms_width8"], 3),
                         "persistent_block_threads": tune["block_threads"],
                         "persistent_waves_per_simd": tune["waves_per_simd"],
                         "persistent_sample0_ms": round(tune["sample0_ms_persistent"], 3),

@@ -3174,19 +3174,24 @@ int c2_batch_autotune(c2_batch* b, c2_batch_tuning* out) {
             return ms;
         };
         int tried = 0;
-        // 1. the wavefront: sample pass 0 over paths in flight x queue-kernel grid caps
+        // 1. the wavefront: the first sample passes (up to 16) over paths in flight x
+        //    queue-kernel grid caps. More than pass 0: larger buffers let the later passes,
+        //    which hold fewer views, share launches (up to kBatchSamplesPerChunk samples), which
+        //    pass 0 alone does not show (on the 64-scene batch pass 0 ranked 12 M and 24 M
+        //    paths in flight within noise, whole renders differ by 15%).
+        const int32_t win = std::max(1, std::min(b->max_spp, 16));
         std::vector<int32_t> chunks;
         for (int32_t c = pt->chunk_cap; c >= (1 << 16) && chunks.size() < 3; c /= 2) chunks.push_back(c);
         if (chunks.empty()) chunks.push_back(pt->chunk_cap);
         const int32_t qbs[] = {4096, 8192, 16384};
         float bestW = INFINITY;
         int32_t bestC = pt->chunk, bestQ = pt->opts.queue_blocks;
-        enqueue_passes(pt, 0, 1, s, true);  // warm-up: code objects, caches
+        enqueue_passes(pt, 0, win, s, true);  // warm-up: code objects, caches
         for (int32_t c : chunks)
             for (int32_t q : qbs) {
                 pt->chunk = c;
                 pt->opts.queue_blocks = q;
-                const float ms = timed([&] { enqueue_passes(pt, 0, 1, s, true); });
+                const float ms = timed([&] { enqueue_passes(pt, 0, win, s, true); });
                 tried++;
                 if (ms < bestW) {
                     bestW = ms;
@@ -3203,7 +3208,7 @@ int c2_batch_autotune(c2_batch* b, c2_batch_tuning* out) {
             const bool save8 = pt->use8;
             pt->use8 = !save8;
             enqueue_passes(pt, 0, 1, s, true);  // warm-up of the other width's kernels
-            const float other = timed([&] { enqueue_passes(pt, 0, 1, s, true); });
+            const float other = timed([&] { enqueue_passes(pt, 0, win, s, true); });
             tried++;
             ms8 = save8 ? bestW : other;
             if (other < bestW) bestW = other;

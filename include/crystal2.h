@@ -416,7 +416,8 @@ typedef struct c2_batch_item {
 typedef struct c2_batch_tuning {
     int32_t paths_in_flight;  /* wavefront: paths per launch                     */
     int32_t queue_blocks;     /* wavefront: grid cap of the queue kernels        */
-    double pass_ms;           /* wavefront: sample pass 0 with the chosen values */
+    double pass_ms;           /* wavefront: sample passes 0 .. min(spp, 16) - 1  */
+                              /* with the chosen values                          */
     int32_t candidates;       /* configurations timed                            */
     int32_t schedule;         /* C2_SCHED_*                                      */
     int32_t block_threads;    /* persistent: threads per block (128, 256, 512)   */
@@ -428,8 +429,8 @@ typedef struct c2_batch_tuning {
     int32_t bvh_width;        /* wavefront: BVH width traversed, 4 or 8 (the     */
                               /* persistent kernel traverses the 4-wide tree);   */
                               /* configure: 0 keeps the current width            */
-    double pass_ms_width8;    /* wavefront: sample pass 0 with the 8-wide tree at */
-                              /* the chosen values (-1: not timed)               */
+    double pass_ms_width8;    /* wavefront: the same passes with the 8-wide tree  */
+                              /* at the chosen values (-1: not timed)            */
 } c2_batch_tuning;
 typedef struct c2_batch_stats {
     int64_t samples, samples_per_render;
@@ -440,11 +441,11 @@ typedef struct c2_batch_stats {
 } c2_batch_stats;
 int  c2_batch_create(const c2_batch_item* items, int32_t n, const c2_render_opts* opts, c2_batch** out);
 void c2_batch_destroy(c2_batch* batch);
-/* Tunes each schedule on sample 0 (wavefront: pass 0 over paths-in-flight x
- * queue_blocks, then the BVH width 4 or 8 at the best of those; persistent: 128 / 256 /
- * 512 threads per block x 0 / 4 / 6 / 8 waves per SIMD), then times both winners over
- * the whole render as captured graphs and keeps the faster. Drops a captured graph and
- * clears the films. */
+/* Tunes each schedule (wavefront: sample passes 0 .. min(spp, 16) - 1 over
+ * paths-in-flight x queue_blocks, then the BVH width 4 or 8 at the best of those;
+ * persistent: sample 0 over 128 / 256 / 512 threads per block x 0 / 4 / 6 / 8 waves per
+ * SIMD), then times both winners over the whole render as captured graphs and keeps the
+ * faster. Drops a captured graph and clears the films. */
 int  c2_batch_autotune(c2_batch* batch, c2_batch_tuning* out);
 /* Sets the schedule and launch shape explicitly (fields of c2_batch_tuning: schedule,
  * block_threads and waves_per_simd, paths_in_flight / queue_blocks when > 0, and
