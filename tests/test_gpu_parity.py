@@ -179,12 +179,34 @@ def test_sample_ranges_and_chunks_are_exact(gpu_available):
     pt.render(1, 3)
     pt.sync()
     assert np.array_equal(pt.film(), full)
-    # default chunks carry two samples of every pixel: ranges split into 1, then 2 + 1
+    # default chunks of a small frame carry up to 16 samples: ranges split into 1, then 3
     pt = c2.WavefrontPathIntegrator(sc, 64, 64, 4, 5)
     pt.render(0, 1)
     pt.render(1, 3)
     pt.sync()
     assert np.array_equal(pt.film(), full)
+
+
+@pytest.mark.parametrize("size,spp,launches", [(64, 48, 2), (512, 20, 4)])
+def test_lane_balanced_chunks_are_exact(gpu_available, size, spp, launches):
+    """A call whose chunks would end on an odd count gets fewer, larger chunks, one set per
+    lane: 48 samples of a 64 x 64 frame (16 per launch) as 24 + 24, 20 samples of 512 x 512
+    (4 per launch) as 5 + 5 + 5 + 5. The films equal those of one-sample calls."""
+    print(f"case lane-balanced chunks {size} {spp}", flush=True)
+    sc = load("cornell")
+    pt = c2.WavefrontPathIntegrator(sc, size, size, spp, 5)
+    for k in range(spp):
+        pt.render(k, 1)
+    pt.sync()
+    ref = pt.film()
+    pt.reset()
+    pt.set_profiling(True, kernels=["generate"])
+    pt.render(0, spp)
+    pt.sync()
+    assert pt.kernel_times()["generate"]["launches"] == launches
+    pt.set_profiling(False)
+    assert np.array_equal(pt.film(), ref)
+    pt.close()
 
 
 def test_film_modes(gpu_available):
