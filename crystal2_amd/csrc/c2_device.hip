@@ -728,13 +728,6 @@ __global__ void k_chunk_reset(PathBuffers B) {
     B.counters[7] = 0;
 }
 
-__global__ void k_reset(PathBuffers B, int next_q) {
-    B.counters[4] = 0;
-    B.counters[next_q] = 0;
-    B.counters[2] = 0;
-    B.counters[3] = 0;
-    for (int c = 0; c < 2 * kClasses; c++) B.ccount[c] = 0;
-}
 
 // Shading class of a hit: material type and its distribution (or, for glass, whether it
 // is rough); the last two classes are "no material" and "no hit".
@@ -828,6 +821,20 @@ __global__ void __launch_bounds__(kBlock) k_class_scatter(PathBuffers B, int q) 
 template <int ACC>
 __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, PassParams P, int q) {
     __shared__ TravLdsFor<ACC, false> lds;
+    // The bounce's append counters (next queue, shadow rays, MIS rays, cache queries) and
+    // the class histogram start at zero. Their readers in the previous bounce have finished:
+    // they ran before this launch on the lane's stream, and the side stream joined before
+    // k_resolve. (A kernel of its own for this cost a launch per bounce.)
+    static_assert(2 * kClasses <= kBlock, "one thread per histogram entry");
+    if (blockIdx.x == 0 && threadIdx.x < 2 * kClasses) {
+        if (threadIdx.x == 0) {
+            B.counters[4] = 0;
+            B.counters[q ^ 1] = 0;
+            B.counters[2] = 0;
+            B.counters[3] = 0;
+        }
+        B.ccount[threadIdx.x] = 0;
+    }
     int n = B.counters[q];
     const int32_t* queue = B.queue[q];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[0], (unsigned long long)n);
@@ -2653,7 +2660,6 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                     const int steps = P.integrator == C2_INTEGRATOR_DIRECT ? (P.max_depth > 0 ? kDlChainLimit : 0)
                                                                           : P.max_depth;
                     for (int b = 0; b < steps; b++) {
-                        hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, LB, q ^ 1);
                         if (pt->cap_bounce == b && pt->cap_kind == 0) {
                             hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, LB, 0, q,
                                                P.lmax, pt->cap_n, pt->cap_rays, pt->cap_ignore,
