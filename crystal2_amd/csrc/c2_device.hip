@@ -61,6 +61,13 @@ static_assert(kBlock == kTravBlock, "traversal kernels index their LDS stack by 
 // per launch): a small grid is dispatched and drained early instead of holding block slots
 // beside the shadow-ray kernel for its whole duration
 constexpr int kMisGrid = 64;
+// Chunks of fewer paths trace their shadow and MIS rays in line on the lane's stream rather
+// than forking the shadow rays to the side stream (direct lighting always does). Measured on
+// the sphere field (Msamples/s, in line against forked): 1280 x 720, 4.6 M paths per chunk,
+// 383.7 against 376-380; 960 x 540, 2.6 M, 334-335 both; 1920 x 1080, 10.4 M, 421-423 against
+// 424-425. The Cornell box (1 M paths): 601 against 591, its direct lighting 2,057 against
+// 1,523, and direct lighting on the 1080p sphere field 1,400-1,406 against 1,384-1,396.
+constexpr int64_t kInlineShadowPaths = 6ll << 20;
 // Direct lighting: wavefront steps per path (transparent surfaces crossed + 1).
 constexpr int kDlChainLimit = 8;
 // Default grid cap of the per-bounce queue kernels. Their live counts are only known
@@ -2717,19 +2724,25 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         // shadow-ray grid (behind the cross-stream wait) fills the device: on the
                         // side stream instead, its blocks waited for slots that k_occlude's
                         // blocks freed, and its launches spanned most of k_occlude's.
-                        HIP_CHECK(hipEventRecord(evf, s));
-                        HIP_CHECK(hipStreamWaitEvent(side, evf, 0));
+                        // Small chunks and direct lighting's short steps run both in line on
+                        // the lane's stream: the fork and join cost more than the overlap gains.
+                        const bool inl = P.integrator == C2_INTEGRATOR_DIRECT || npath < kInlineShadowPaths;
+                        hipStream_t os = inl ? s : side;
+                        if (!inl) {
+                            HIP_CHECK(hipEventRecord(evf, s));
+                            HIP_CHECK(hipStreamWaitEvent(side, evf, 0));
+                        }
                         int gs = std::min(grid_for(npath * P.lmax), qcap);
                         {
-                            KTimer kt(pt, C2_K_OCCLUDE, side);
+                            KTimer kt(pt, C2_K_OCCLUDE, os);
                             if (acc == kAccKd)
-                                hipLaunchKernelGGL(k_occlude<kAccKd>, dim3(gs), dim3(kBlock), 0, side, S, LB);
+                                hipLaunchKernelGGL(k_occlude<kAccKd>, dim3(gs), dim3(kBlock), 0, os, S, LB);
                             else if (acc == kAccBvh8)
-                                hipLaunchKernelGGL(k_occlude<kAccBvh8>, dim3(gs), dim3(kBlock), 0, side, S, LB);
+                                hipLaunchKernelGGL(k_occlude<kAccBvh8>, dim3(gs), dim3(kBlock), 0, os, S, LB);
                             else
-                                hipLaunchKernelGGL(k_occlude<kAccBvh4>, dim3(gs), dim3(kBlock), 0, side, S, LB);
+                                hipLaunchKernelGGL(k_occlude<kAccBvh4>, dim3(gs), dim3(kBlock), 0, os, S, LB);
                         }
-                        HIP_CHECK(hipEventRecord(evj, side));
+                        if (!inl) HIP_CHECK(hipEventRecord(evj, side));
                         {
                             KTimer kt(pt, C2_K_MIS, s);
                             if (acc == kAccKd)
@@ -2742,7 +2755,7 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                                 hipLaunchKernelGGL(k_mis<kAccBvh4>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, s,
                                                    S, LB, P);
                         }
-                        HIP_CHECK(hipStreamWaitEvent(s, evj, 0));
+                        if (!inl) HIP_CHECK(hipStreamWaitEvent(s, evj, 0));
                         {
                             KTimer kt(pt, C2_K_RESOLVE, s);
                             hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kBlock), 0, s, SB, P, q);
