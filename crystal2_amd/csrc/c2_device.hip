@@ -1179,12 +1179,12 @@ k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
     }
 }
 
-template <int ACC>
-__global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
-    __shared__ TravLdsFor<ACC, true> lds;
+// The shadow rays of blocks blk of nblk (grid-stride).
+template <int ACC, class Lds>
+__device__ __forceinline__ void occlude_blocks(const DevScene& S, const PathBuffers& B, Lds& lds, int blk, int nblk) {
     int n = B.counters[2];
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[1], (unsigned long long)n);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (blk == 0 && threadIdx.x == 0) atomicAdd(&B.totals[1], (unsigned long long)n);
+    for (int i = blk * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
         int s = B.sh_list[i];
         float4 o = B.sh_o[s], d = B.sh_d[s];
         Hit h;
@@ -1195,14 +1195,20 @@ __global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
     }
 }
 
-// The shared MIS ray: Scene::Intersect(isec.SpawnRay(wi)); only the light owning the
-// hit entity can receive f * Le * w / pdf_bsdf, :255-273
 template <int ACC>
-__global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassParams P) {
-    __shared__ TravLdsFor<ACC, false> lds;
+__global__ void __launch_bounds__(kBlock) k_occlude(DevScene S, PathBuffers B) {
+    __shared__ TravLdsFor<ACC, true> lds;
+    occlude_blocks<ACC>(S, B, lds, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// The shared MIS ray: Scene::Intersect(isec.SpawnRay(wi)); only the light owning the
+// hit entity can receive f * Le * w / pdf_bsdf, :255-273. Blocks blk of nblk (grid-stride).
+template <int ACC, class Lds>
+__device__ __forceinline__ void mis_blocks(const DevScene& S, const PathBuffers& B, const PassParams& P, Lds& lds,
+                                           int blk, int nblk) {
     int n = B.counters[3];
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[2], (unsigned long long)n);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (blk == 0 && threadIdx.x == 0) atomicAdd(&B.totals[2], (unsigned long long)n);
+    for (int i = blk * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
         const int v = B.mis_list[i];  // the vertex's queue position
         float4 mp = B.vtx_p[v], mn = B.vtx_n[v], mw = B.mis_w[v], mf = B.mis_f[v];
         V3 sp{mp.x, mp.y, mp.z}, sn{mn.x, mn.y, mn.z}, wi{mw.x, mw.y, mw.z}, f{mf.x, mf.y, mf.z};
@@ -1214,6 +1220,25 @@ __global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassP
         B.mis_L[v] = make_float4(Lb.x, Lb.y, Lb.z, 0.f);
         B.mis_hit[v] = lh;
     }
+}
+
+template <int ACC>
+__global__ void __launch_bounds__(kBlock) k_mis(DevScene S, PathBuffers B, PassParams P) {
+    __shared__ TravLdsFor<ACC, false> lds;
+    mis_blocks<ACC>(S, B, P, lds, (int)blockIdx.x, (int)gridDim.x);
+}
+
+// Both in one launch, for chunks that trace them in line: blocks [0, gs) take the shadow
+// rays, the rest the MIS rays (one launch less per bounce; its registers are those of the
+// MIS rays, so large chunks keep the two kernels).
+template <int ACC>
+__global__ void __launch_bounds__(kBlock) k_occlude_mis(DevScene S, PathBuffers B, PassParams P, int gs) {
+    __shared__ union {
+        TravLdsFor<ACC, true> any;
+        TravLdsFor<ACC, false> closest;
+    } lds;
+    if ((int)blockIdx.x < gs) occlude_blocks<ACC>(S, B, lds.any, (int)blockIdx.x, gs);
+    else mis_blocks<ACC>(S, B, P, lds.closest, (int)blockIdx.x - gs, (int)gridDim.x - gs);
 }
 
 // L += beta * sum over eligible lights in light order of EsimateDirect's result:
@@ -2733,7 +2758,20 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                             HIP_CHECK(hipStreamWaitEvent(side, evf, 0));
                         }
                         int gs = std::min(grid_for(npath * P.lmax), qcap);
-                        {
+                        // small chunks: shadow and MIS rays in one launch
+                        const bool fused = inl && npath < kInlineShadowPaths;
+                        if (fused) {
+                            KTimer kt(pt, C2_K_OCCLUDE, s);
+                            const int gm = std::min(g, kMisGrid);
+                            if (acc == kAccKd)
+                                hipLaunchKernelGGL(k_occlude_mis<kAccKd>, dim3(gs + gm), dim3(kBlock), 0, s, S, LB, P, gs);
+                            else if (acc == kAccBvh8)
+                                hipLaunchKernelGGL(k_occlude_mis<kAccBvh8>, dim3(gs + gm), dim3(kBlock), 0, s, S, LB, P,
+                                                   gs);
+                            else
+                                hipLaunchKernelGGL(k_occlude_mis<kAccBvh4>, dim3(gs + gm), dim3(kBlock), 0, s, S, LB, P,
+                                                   gs);
+                        } else {
                             KTimer kt(pt, C2_K_OCCLUDE, os);
                             if (acc == kAccKd)
                                 hipLaunchKernelGGL(k_occlude<kAccKd>, dim3(gs), dim3(kBlock), 0, os, S, LB);
@@ -2743,7 +2781,7 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                                 hipLaunchKernelGGL(k_occlude<kAccBvh4>, dim3(gs), dim3(kBlock), 0, os, S, LB);
                         }
                         if (!inl) HIP_CHECK(hipEventRecord(evj, side));
-                        {
+                        if (!fused) {
                             KTimer kt(pt, C2_K_MIS, s);
                             if (acc == kAccKd)
                                 hipLaunchKernelGGL(k_mis<kAccKd>, dim3(std::min(g, kMisGrid)), dim3(kBlock), 0, s, S,
@@ -2760,7 +2798,7 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                             KTimer kt(pt, C2_K_RESOLVE, s);
                             hipLaunchKernelGGL(k_resolve, dim3(g), dim3(kBlock), 0, s, SB, P, q);
                         }
-                        pt->launches += 7;
+                        pt->launches += fused ? 5 : 6;  // extend, shade, nee, occlude (+ mis), resolve
                         q ^= 1;
                     }
                 }
