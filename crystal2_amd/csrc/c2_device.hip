@@ -2175,6 +2175,9 @@ struct c2_pt {
     std::unique_ptr<c2_accel> accel;
     bool use8 = false;  // traverse the 8-wide tree (accelerator C2_ACCEL_BVH8, or a batch's choice)
     bool group = true;  // group the queue by shading class from bounce kSortFrom (k_class_*)
+    // direct lighting: some entity is transparent without a light, so a path may pass
+    // through it (otherwise every path ends at its first hit, and one step is all)
+    bool dl_chain = true;
     bool profiling = false;
     uint32_t prof_mask = 0;  // kernel classes timed with events
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
@@ -2348,6 +2351,9 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
         for (const int4& e : SG.ents)
             if (e.z >= 0) classes |= 1ull << shade_class_host(SG.mats[e.z]);
         pt->group = __builtin_popcountll(classes) >= kGroupMinClasses;
+        bool pass_through = false;
+        for (const int4& e : SG.ents) pass_through = pass_through || (e.z < 0 && e.w < 0);
+        pt->dl_chain = pass_through;
     }
     PassParams& P = pt->pp;
     pt->use8 = opts->accel == C2_ACCEL_BVH8;
@@ -2689,8 +2695,11 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                     // transparent surfaces (the reference recurses through them without
                     // consuming depth); depth 0 renders black, as level == maxDepth at
                     // level 0 does.
-                    const int steps = P.integrator == C2_INTEGRATOR_DIRECT ? (P.max_depth > 0 ? kDlChainLimit : 0)
-                                                                          : P.max_depth;
+                    // (without transparent entities that carry no light, every direct-lighting
+                    // path ends at its first hit: the later steps would run on empty queues)
+                    const int steps = P.integrator == C2_INTEGRATOR_DIRECT
+                                          ? (P.max_depth > 0 ? (pt->dl_chain ? kDlChainLimit : 1) : 0)
+                                          : P.max_depth;
                     for (int b = 0; b < steps; b++) {
                         if (pt->cap_bounce == b && pt->cap_kind == 0) {
                             hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, LB, 0, q,
