@@ -830,8 +830,9 @@ __global__ void __launch_bounds__(kBlock) k_extend(DevScene S, PathBuffers B, Pa
     __shared__ TravLdsFor<ACC, false> lds;
     // The bounce's append counters (next queue, shadow rays, MIS rays, cache queries) and
     // the class histogram start at zero. Their readers in the previous bounce have finished:
-    // they ran before this launch on the lane's stream, and the side stream joined before
-    // k_resolve. (A kernel of its own for this cost a launch per bounce.)
+    // they ran before this launch on the lane's stream, and shadow rays forked to the side
+    // stream were joined before k_resolve. (A kernel of its own for this cost a launch per
+    // bounce.)
     static_assert(2 * kClasses <= kBlock, "one thread per histogram entry");
     if (blockIdx.x == 0 && threadIdx.x < 2 * kClasses) {
         if (threadIdx.x == 0) {
