@@ -401,11 +401,12 @@ struct SurfVertex {
 // (UniformSampleAllLights, :175-278: uL = Get2D(), uB = Get2D(), one Get1D per eligible
 // non-delta light), the BSDF-sampled MIS direction (shared by all lights), and for the
 // path tracer the continuation sample and Russian roulette (:79-102).
+template <int MAXT = kMatAll>
 __device__ __forceinline__ void shade_surface(const DevScene& S, const View& V, const Inter& it, V3 d, V3 beta,
                                               int32_t idx, int32_t dim, int bounce, int32_t texel, int integrator,
                                               int arg_order, SurfVertex& sv) {
     const HaltonDev hd{V.exp0, V.scale1};
-    Bsdf bs = make_bsdf(S, it, (uint32_t)idx, (uint32_t)(dim + 2), texel);
+    Bsdf bs = make_bsdf<MAXT>(S, it, (uint32_t)idx, (uint32_t)(dim + 2), texel);
     V3 wo = -d;
     float uL0 = halton_value(S, hd, idx, dim), uL1 = halton_value(S, hd, idx, dim + 1);
     float uB0 = halton_value(S, hd, idx, dim + 2), uB1 = halton_value(S, hd, idx, dim + 3);
@@ -896,6 +897,7 @@ __global__ void __launch_bounds__(kBlock) k_surface(DevScene S, PathBuffers B, P
 
 // Vertex stage: emission, BSDF, the BSDF-sampled MIS direction (shared by all
 // lights), continuation sample and Russian roulette; the light loop runs in k_nee.
+template <int MAXT>
 __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, PassParams P, int q, int bounce) {
     __shared__ int32_t sm[kWaves + 1];
     int n = B.counters[q];
@@ -977,7 +979,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(DevScene S, PathBuffers B, Pas
                     // the Kd texel k_surface found, read only for a textured material
                     const int32_t texel = S.materials[it.material].kd_texture > 0 ? B.texel[i] : 0;
                     SurfVertex sv;
-                    shade_surface(S, V, it, d, beta, idx, dim, bounce, texel, P.integrator, P.arg_order, sv);
+                    shade_surface<MAXT>(S, V, it, d, beta, idx, dim, bounce, texel, P.integrator, P.arg_order, sv);
                     B.vtx_p[i] = make_float4(it.p.x, it.p.y, it.p.z, sv.uL0);
                     B.vtx_n[i] = make_float4(it.n.x, it.n.y, it.n.z, sv.uL1);
                     B.vtx_du[i] = make_float4(it.dpdu.x, it.dpdu.y, it.dpdu.z, __int_as_float(sv.fBnz));
@@ -1092,6 +1094,7 @@ __global__ void __launch_bounds__(kBlock) k_cache_collect(PathBuffers B, PassPar
 // VGPRs at three waves. The kernel waits on memory more than it issues, so the fourth wave
 // pays for the spills: +1.5-1.9% on the 1080p sphere field; five waves (96 VGPRs, 292 B)
 // lose 2%.)
+template <int MAXT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
     __shared__ int32_t sm[kWaves * kAppendLights + 1];
@@ -1146,7 +1149,7 @@ k_nee(DevScene S, PathBuffers B, PassParams P, int q) {
                 const DevLight Lt = S.lights[li];
                 // rebuilt per light from the (cached) material record rather than kept
                 // live across the loop: fewer registers, more waves per SIMD
-                const Bsdf bs = make_bsdf(S, it, hidx, 0u, texel);
+                const Bsdf bs = make_bsdf<MAXT>(S, it, hidx, 0u, texel);
                 // (i) light sampling, :199-230
                 light_term(S, Lt, delta, it, bs, wo, uL0, uL1, [&](V3 so, V3 sd, V3 C) {
                     ro = make_float4(so.x, so.y, so.z, __int_as_float(delta ? -1 : Lt.entity));
@@ -2179,6 +2182,9 @@ struct c2_pt {
     // direct lighting: some entity is transparent without a light, so a path may pass
     // through it (otherwise every path ends at its first hit, and one step is all)
     bool dl_chain = true;
+    // every material is Diffuse or Mirror: k_shade and k_nee run their instantiations
+    // without the other materials' code (fewer registers, more waves per SIMD)
+    bool simple = false;
     bool profiling = false;
     uint32_t prof_mask = 0;  // kernel classes timed with events
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
@@ -2353,8 +2359,13 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
             if (e.z >= 0) classes |= 1ull << shade_class_host(SG.mats[e.z]);
         pt->group = __builtin_popcountll(classes) >= kGroupMinClasses;
         bool pass_through = false;
-        for (const int4& e : SG.ents) pass_through = pass_through || (e.z < 0 && e.w < 0);
+        int maxType = -1;
+        for (const int4& e : SG.ents) {
+            pass_through = pass_through || (e.z < 0 && e.w < 0);
+            if (e.z >= 0) maxType = std::max(maxType, SG.mats[e.z].type);
+        }
         pt->dl_chain = pass_through;
+        pt->simple = maxType <= C2_MAT_MIRROR;
     }
     PassParams& P = pt->pp;
     pt->use8 = opts->accel == C2_ACCEL_BVH8;
@@ -2732,7 +2743,10 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         }
                         {
                             KTimer kt(pt, C2_K_SHADE, s);
-                            hipLaunchKernelGGL(k_shade, dim3(g), dim3(kBlock), 0, s, S, SB, P, q, b);
+                            if (pt->simple)
+                                hipLaunchKernelGGL(k_shade<C2_MAT_MIRROR>, dim3(g), dim3(kBlock), 0, s, S, SB, P, q, b);
+                            else
+                                hipLaunchKernelGGL(k_shade<kMatAll>, dim3(g), dim3(kBlock), 0, s, S, SB, P, q, b);
                         }
                         if (P.cache_bounce == b) {
                             hipEvent_t e0, e1;
@@ -2745,7 +2759,10 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         }
                         {
                             KTimer kt(pt, C2_K_NEE, s);
-                            hipLaunchKernelGGL(k_nee, dim3(g), dim3(kBlock), 0, s, S, SB, P, q);
+                            if (pt->simple)
+                                hipLaunchKernelGGL(k_nee<C2_MAT_MIRROR>, dim3(g), dim3(kBlock), 0, s, S, SB, P, q);
+                            else
+                                hipLaunchKernelGGL(k_nee<kMatAll>, dim3(g), dim3(kBlock), 0, s, S, SB, P, q);
                         }
                         if (pt->cap_bounce == b && pt->cap_kind == 1) {
                             hipLaunchKernelGGL(k_capture, dim3(grid_for(pt->cap_n)), dim3(kBlock), 0, s, LB, 1, q,

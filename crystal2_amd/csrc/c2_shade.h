@@ -702,6 +702,11 @@ __device__ __forceinline__ V3 bxdf_sample(const Bsdf& b, float sx, float sy, V3 
 // Material::ComputeScatteringFunctions -> one BxDF (all supported materials
 // build exactly one), src/Materials/*.cpp
 // texel: the Kd texel of the hit (flat index) for a textured Diffuse or Phong material
+// MAXT: the largest C2_MAT_* type the scene holds (kMatAll: any). A kernel instantiated with
+// C2_MAT_MIRROR keeps only the Lambertian and specular-reflection code after inlining, and
+// with it far fewer registers; the caller guarantees that no other type reaches it.
+constexpr int kMatAll = C2_MAT_ORENNAYAR;
+template <int MAXT = kMatAll>
 __device__ __forceinline__ Bsdf make_bsdf(const DevScene& S, const Inter& it, uint32_t idx, uint32_t rdim,
                                           int32_t texel) {
     Bsdf b;
@@ -718,6 +723,7 @@ __device__ __forceinline__ Bsdf make_bsdf(const DevScene& S, const Inter& it, ui
     b.rdim = rdim;
     if (it.material < 0) return b;
     const c2_material m = S.materials[it.material];
+    if constexpr (MAXT < kMatAll) __builtin_assume(m.type >= 0 && m.type <= MAXT);
     b.R = V3{m.color[0], m.color[1], m.color[2]};
     // Diffuse / Phong with a "Kd" texture: R = texture(uv), DiffuseMaterial.cpp:16-20
     if (m.kd_texture > 0) b.R = texel_value(S, texel);
