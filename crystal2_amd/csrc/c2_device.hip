@@ -2182,9 +2182,10 @@ struct c2_pt {
     // direct lighting: some entity is transparent without a light, so a path may pass
     // through it (otherwise every path ends at its first hit, and one step is all)
     bool dl_chain = true;
-    // every material is Diffuse or Mirror: k_shade and k_nee run their instantiations
-    // without the other materials' code (fewer registers, more waves per SIMD)
-    bool simple = false;
+    // the largest material type k_shade and k_nee are instantiated for: C2_MAT_MIRROR
+    // (Diffuse and Mirror only), C2_MAT_MICRO (also Glass and Microfacet) or kMatAll; the
+    // smaller instantiations leave out the other materials' code (fewer registers)
+    int mat_bound = kMatAll;
     bool profiling = false;
     uint32_t prof_mask = 0;  // kernel classes timed with events
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
@@ -2365,7 +2366,7 @@ void pt_build(c2_pt* pt, const c2_scene_desc* const* descs, const ViewSpec* vs, 
             if (e.z >= 0) maxType = std::max(maxType, SG.mats[e.z].type);
         }
         pt->dl_chain = pass_through;
-        pt->simple = maxType <= C2_MAT_MIRROR;
+        pt->mat_bound = maxType <= C2_MAT_MIRROR ? C2_MAT_MIRROR : maxType <= C2_MAT_MICRO ? C2_MAT_MICRO : kMatAll;
     }
     PassParams& P = pt->pp;
     pt->use8 = opts->accel == C2_ACCEL_BVH8;
@@ -2743,8 +2744,10 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         }
                         {
                             KTimer kt(pt, C2_K_SHADE, s);
-                            if (pt->simple)
+                            if (pt->mat_bound == C2_MAT_MIRROR)
                                 hipLaunchKernelGGL(k_shade<C2_MAT_MIRROR>, dim3(g), dim3(kBlock), 0, s, S, SB, P, q, b);
+                            else if (pt->mat_bound == C2_MAT_MICRO)
+                                hipLaunchKernelGGL(k_shade<C2_MAT_MICRO>, dim3(g), dim3(kBlock), 0, s, S, SB, P, q, b);
                             else
                                 hipLaunchKernelGGL(k_shade<kMatAll>, dim3(g), dim3(kBlock), 0, s, S, SB, P, q, b);
                         }
@@ -2759,8 +2762,10 @@ void enqueue_passes(c2_pt* pt, int32_t sample_begin, int32_t sample_count, hipSt
                         }
                         {
                             KTimer kt(pt, C2_K_NEE, s);
-                            if (pt->simple)
+                            if (pt->mat_bound == C2_MAT_MIRROR)
                                 hipLaunchKernelGGL(k_nee<C2_MAT_MIRROR>, dim3(g), dim3(kBlock), 0, s, S, SB, P, q);
+                            else if (pt->mat_bound == C2_MAT_MICRO)
+                                hipLaunchKernelGGL(k_nee<C2_MAT_MICRO>, dim3(g), dim3(kBlock), 0, s, S, SB, P, q);
                             else
                                 hipLaunchKernelGGL(k_nee<kMatAll>, dim3(g), dim3(kBlock), 0, s, S, SB, P, q);
                         }
