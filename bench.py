@@ -406,25 +406,25 @@ def _relmse(x, r):
     return float(np.mean((x.astype(np.float64) - r) ** 2 / (r.astype(np.float64) ** 2 + 1e-2)))
 
 
-def equal_time(scene, W, H, depth, seconds=0.0, judge_spp=4096, crop=256, cache_bounce=2, train_ratio=48,
+def equal_time(scene, W, H, depth, seconds=0.0, truth_spp=4096, crop=256, cache_bounce=2, train_ratio=48,
                warmup_passes=8, budget_passes=64):
-    """relMSE at equal wall time: plain PT vs radiance-cache PT, both judged against a
-    plain render of `judge_spp` samples with an FP64 film on a centre crop. The judge uses
-    Halton samples [judge_offset, judge_offset + judge_spp), the runs samples from 0.
+    """relMSE at equal wall time: plain PT vs radiance-cache PT, both measured against a
+    plain render of `truth_spp` samples with an FP64 film on a centre crop. The ground truth uses
+    Halton samples [truth_offset, truth_offset + truth_spp), the runs samples from 0.
     The budget is fixed by the workload, not by the headline's step count: the wall time of
     `budget_passes` plain passes rendered in one call (or `seconds` when given). The cache
     run's `warmup_passes` training passes (samples outside both runs) count in its time.
     train_ratio 48 trains about 2^16 path vertices per pass at 1920x1080."""
     import crystal2_amd as c2
     win = ((W - crop) // 2, (H - crop) // 2, crop, crop)
-    judge_offset = 8192
+    truth_offset = 8192
     t0 = time.perf_counter()
-    judge = c2.WavefrontPathIntegrator(scene, W, H, judge_spp, depth, film="f64", window=win)
-    judge.render(judge_offset, judge_spp)
-    judge.sync()
-    ref = _crop(judge.film(), W, H, win)
-    judge.close()
-    log(f"judge: {judge_spp} spp FP64 on {crop}x{crop} in {time.perf_counter() - t0:.1f}s")
+    truth = c2.WavefrontPathIntegrator(scene, W, H, truth_spp, depth, film="f64", window=win)
+    truth.render(truth_offset, truth_spp)
+    truth.sync()
+    ref = _crop(truth.film(), W, H, win)
+    truth.close()
+    log(f"ground truth: {truth_spp} spp FP64 on {crop}x{crop} in {time.perf_counter() - t0:.1f}s")
 
     def run(pt, budget, checkpoints, start=0):
         # passes rendered in one call up to the next checkpoint (a renderer overlaps the
@@ -489,8 +489,8 @@ def equal_time(scene, W, H, depth, seconds=0.0, judge_spp=4096, crop=256, cache_
     return {
         "seconds": seconds,
         "budget": (f"the wall time of {budget_passes} plain passes in one call" if budget_passes else "given"),
-        "judge": {"spp": judge_spp, "film": "f64", "window": list(win), "samples": [judge_offset,
-                                                                                  judge_offset + judge_spp]},
+        "ground_truth": {"spp": truth_spp, "film": "f64", "window": list(win),
+                         "samples": [truth_offset, truth_offset + truth_spp]},
         "plain": {"passes": plain_curve[-1][1], "relmse": plain_curve[-1][2], "curve": plain_curve},
         "cache": {"passes": npass, "warmup_passes": warmup_passes, "warmup_s": round(warm, 4),
                   "relmse": cache_curve[-1][2], "curve": cache_curve, "cache_bounce": cache_bounce,
@@ -627,7 +627,7 @@ def main():
                     help="skip the KD-tree / textures / disk / material-model throughput lines")
     ap.add_argument("--equal-seconds", type=float, default=0.0,
                     help="equal-time budget in seconds (0: the wall time of 64 plain passes)")
-    ap.add_argument("--judge-spp", type=int, default=4096)
+    ap.add_argument("--truth-spp", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run a plain fp32 negate kernel over MIB MiB (reads MIB, writes MIB): "
@@ -761,7 +761,7 @@ def main():
         log(f"raycast: {ray_cast}")
     eq = None
     if not args.no_cache:
-        eq = equal_time(scene, W, H, args.depth, seconds=args.equal_seconds, judge_spp=args.judge_spp)
+        eq = equal_time(scene, W, H, args.depth, seconds=args.equal_seconds, truth_spp=args.truth_spp)
         log(f"equal time {eq['seconds']:.2f}s: plain relMSE {eq['plain']['relmse']:.4e} ({eq['plain']['passes']} "
             f"passes), cache relMSE {eq['cache']['relmse']:.4e} ({eq['cache']['passes']} passes); "
             f"cache reaches plain relMSE at {eq['cache_time_to_plain_relmse_frac']} of the time; "

@@ -111,14 +111,14 @@ def test_training_fits_a_smooth_field(gpu_available):
 @pytest.mark.gpu
 def test_cache_gain_in_renderer(gpu_available):
     """Paths cut at bounce 2: the trained cache must beat the same cut without it
-    (a zero network = plain truncation) against a high-spp plain judge."""
+    (a zero network = plain truncation) against a high-spp plain ground truth."""
     import bench
     import crystal2_amd as c2
     sc = bench.build_scene(n_spheres=60, level=2, seed=4)
     W, H, depth = 160, 90, 8
-    judge = c2.WavefrontPathIntegrator(sc, W, H, 256, depth, film="f64")
-    judge.render()
-    ref = judge.film()
+    truth = c2.WavefrontPathIntegrator(sc, W, H, 256, depth, film="f64")
+    truth.render()
+    ref = truth.film()
     lo, hi = bench.scene_bounds(sc)
     results = {}
     for name in ("trained", "zero"):
